@@ -1,0 +1,140 @@
+"""FP64 CPU oracle of the fastpli simulation hot path -- TEST INFRASTRUCTURE.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
+leg may import this module, and only as the checker / the timed CPU
+baseline.  The product package ``fastpli_amd`` never imports it.
+
+``liboracle.so`` is built from ``fpl_oracle.cpp`` (``make -C oracle``), a plain
+C++17 + OpenMP restatement of fastpli's ``PliGenerator::RunTissueGeneration``
+(src/simulation/generator.cpp:165-383,475-526) and
+``PliSimulator::RunSimulation`` (src/simulation/simulator.cpp:182-773) in the
+reference's own precision, compiled without FMA contraction.
+
+Parity pins (see tests/test_oracle_*.py):
+  * generate_tissue: bit-exact labels and vectors against the reference's
+    golden ``simpli_rep.h5`` (cube.dat, 24^3); closed-form voxel tests of
+    ``tests/simulation/simpli_test.py:65-218``; brute force.
+  * run_simulation: golden images of all 5 tilts (Slerp) at 1e-5 relative;
+    closed forms for uniform columns; rho-rotation identity.
+  * Lerp, flip_z, step_size != 1, tilted sensor (untilt_sensor_view=False)
+    and refraction are pinned only by invariants (parity partially unpinned:
+    no reference vectors exist for them).
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+WARN_OVERLAP = 1 << 0
+WARN_SINGLE_POINT = 1 << 1
+WARN_ZERO_VEC = 1 << 2
+WARN_NOT_NORM = 1 << 3
+WARN_NO_LIGHT = 1 << 4
+
+INTERP = {'NN': 0, 'Lerp': 1, 'Slerp': 2}
+
+_p = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+_int = ctypes.c_int
+
+
+def build():
+    """Compile liboracle.so with make (g++, no FMA contraction)."""
+    import subprocess
+    subprocess.run(['make', '-s', '-C', _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, 'liboracle.so')
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.oracle_generate.restype = _int
+        L.oracle_generate.argtypes = [
+            _i64, _i64, _i64, _p, _d, _i64, _p, _p, _p, _p, _p, _p, _p, _p,
+            _int, _int, _p, _p, _p, _p
+        ]
+        L.oracle_simulate.restype = _int
+        L.oracle_simulate.argtypes = [
+            _i64, _i64, _i64, _p, _p, _p, _i64, _d, _d, _d, _d, _d, _int, _int,
+            _int, _p, _i64, _d, _d, _int, _p, _p
+        ]
+        L.oracle_count_steps.restype = _i64
+        L.oracle_count_steps.argtypes = [_i64, _i64, _i64, _d, _d, _d]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def generate(fiber_bundles, dim, origin, voxel_size, only_label=False,
+             n_threads=1):
+    """Voxelize ``FiberBundles`` -> (label int32, vector f32 or (0,), props f64, warn).
+
+    ``props[0, 1]`` is left 0 (the Simpli layer sets the background absorption).
+    """
+    points, fiber_ptr, bundle_ptr, layer_ptr, table = fiber_bundles.to_flat()
+    scale, dn, mu, model = table
+    n_bundles = len(bundle_ptr) - 1
+    max_layer = int(np.max(np.diff(layer_ptr))) if n_bundles else 0
+    n_props = n_bundles * max_layer + 1
+    nx, ny, nz = (int(d) for d in dim)
+    label = np.zeros((nx, ny, nz), dtype=np.int32)
+    vector = (np.zeros((0,), dtype=np.float32) if only_label else np.zeros(
+        (nx, ny, nz, 3), dtype=np.float32))
+    props = np.zeros((n_props, 2), dtype=np.float64)
+    warn = np.zeros(1, dtype=np.uint32)
+    origin = np.ascontiguousarray(origin, dtype=np.float64)
+    orient = np.ascontiguousarray(model.astype(np.uint8))
+    rc = lib().oracle_generate(nx, ny, nz, _ptr(origin), float(voxel_size),
+                               n_bundles, _ptr(bundle_ptr), _ptr(fiber_ptr),
+                               _ptr(points), _ptr(layer_ptr), _ptr(scale),
+                               _ptr(dn), _ptr(mu), _ptr(orient),
+                               int(bool(only_label)), int(n_threads),
+                               _ptr(label), _ptr(vector) if not only_label else None,
+                               _ptr(props), _ptr(warn))
+    if rc != 0:
+        raise ValueError(f'oracle_generate failed: {rc}')
+    return label, vector, props, int(warn[0])
+
+
+def simulate(label, vector, props, theta, phi, *, step_size=1.0,
+             light_intensity=26000.0, voxel_size=1.0, wavelength=525.0,
+             tissue_refrection=1.0, interpolate='Slerp',
+             untilt_sensor_view=True, flip_z=False, filter_rotations=None,
+             n_threads=1):
+    """March one tilt -> (images f64 (nx, ny, n_rho), warn)."""
+    label = np.ascontiguousarray(label, dtype=np.int32)
+    vector = np.ascontiguousarray(vector, dtype=np.float32)
+    props = np.ascontiguousarray(props, dtype=np.float64)
+    rho = np.ascontiguousarray(filter_rotations, dtype=np.float64)
+    nx, ny, nz = label.shape
+    out = np.empty((nx, ny, rho.size), dtype=np.float64)
+    warn = np.zeros(1, dtype=np.uint32)
+    rc = lib().oracle_simulate(nx, ny, nz, _ptr(label), _ptr(vector),
+                               _ptr(props), props.shape[0], float(step_size),
+                               float(light_intensity), float(voxel_size),
+                               float(wavelength), float(tissue_refrection),
+                               INTERP[interpolate], int(bool(untilt_sensor_view)),
+                               int(bool(flip_z)), _ptr(rho), rho.size,
+                               float(theta), float(phi), int(n_threads),
+                               _ptr(out), _ptr(warn))
+    if rc != 0:
+        raise ValueError(f'oracle_simulate failed: {rc}')
+    return out, int(warn[0])
+
+
+def count_steps(dim, step_size, theta, phi):
+    """Ray-steps of one tilt (rays x z-march iterations)."""
+    nx, ny, nz = (int(d) for d in dim)
+    return int(lib().oracle_count_steps(nx, ny, nz, float(step_size),
+                                        float(theta), float(phi)))
