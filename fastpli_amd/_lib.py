@@ -1,0 +1,206 @@
+"""ctypes binding of libfastpli_amd.so (the C ABI declared in include/fastpli_amd.h).
+
+The library is required: there is no CPU fallback.  Importing this module
+without the built library, or calling into it without a HIP device, raises.
+"""
+
+import ctypes
+import os
+import warnings
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'lib', 'libfastpli_amd.so')
+
+FPL_OK = 0
+FPL_E_INVALID = -1
+FPL_E_LABEL_NEG = -2
+FPL_E_LABEL_MAX = -3
+FPL_E_BG_DN = -4
+FPL_E_DEVICE = -5
+FPL_E_NOMEM = -6
+
+WARN_OVERLAP = 1 << 0
+WARN_SINGLE_POINT = 1 << 1
+WARN_ZERO_VEC = 1 << 2
+WARN_NOT_NORM = 1 << 3
+WARN_NO_LIGHT = 1 << 4
+
+# messages of the reference (generator.cpp:215-216,357; simulator.cpp:294-301,715)
+WARN_MESSAGES = {
+    WARN_OVERLAP: 'objects overlap',
+    WARN_SINGLE_POINT: 'fiber object with only single point',
+    WARN_ZERO_VEC: 'optical_axis is 0 for tissue > 0',
+    WARN_NOT_NORM: 'optical_axis not normalized',
+    WARN_NO_LIGHT: 'light_positions is empty',
+}
+
+INTERP = {'NN': 0, 'Lerp': 1, 'Slerp': 2}
+
+_c_i64 = ctypes.c_int64
+_c_p = ctypes.c_void_p
+
+
+class FplVolume(ctypes.Structure):
+    _fields_ = [('dim', _c_i64 * 3), ('origin', ctypes.c_double * 3),
+                ('voxel_size', ctypes.c_double)]
+
+
+class FplFiberBundles(ctypes.Structure):
+    _fields_ = [('n_bundles', _c_i64), ('n_fibers', _c_i64), ('n_points', _c_i64),
+                ('points', _c_p), ('fiber_ptr', _c_p), ('bundle_ptr', _c_p),
+                ('layer_ptr', _c_p), ('layer_scale', _c_p), ('layer_dn', _c_p),
+                ('layer_mu', _c_p), ('layer_model', _c_p)]
+
+
+class FplSetup(ctypes.Structure):
+    _fields_ = [('step_size', ctypes.c_double), ('light_intensity', ctypes.c_double),
+                ('voxel_size', ctypes.c_double), ('wavelength', ctypes.c_double),
+                ('tissue_refrection', ctypes.c_double), ('interpolate', ctypes.c_int32),
+                ('untilt_sensor_view', ctypes.c_int32), ('flip_z', ctypes.c_int32),
+                ('n_rho', ctypes.c_int32), ('filter_rotations', _c_p)]
+
+
+_LIB = None
+
+# every symbol include/fastpli_amd.h declares
+EXPORTS = ('fpl_num_properties', 'fpl_generate', 'fpl_simulate', 'fpl_count_voxel_steps',
+           'fpl_profile_enable', 'fpl_profile_read', 'fpl_release_workspace', 'fpl_version')
+
+
+def lib():
+    """Load the shared library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f'{LIB_PATH} is missing: build it with '
+                              '`python -m fastpli_amd.csrc.build`')
+        L = ctypes.CDLL(LIB_PATH)
+        L.fpl_num_properties.restype = _c_i64
+        L.fpl_num_properties.argtypes = [ctypes.POINTER(FplFiberBundles)]
+        L.fpl_generate.restype = ctypes.c_int
+        L.fpl_generate.argtypes = [
+            ctypes.POINTER(FplVolume), ctypes.POINTER(FplFiberBundles), ctypes.c_int, _c_p,
+            _c_p, _c_p, ctypes.POINTER(ctypes.c_uint32), _c_p, ctypes.c_char_p, ctypes.c_size_t
+        ]
+        L.fpl_simulate.restype = ctypes.c_int
+        L.fpl_simulate.argtypes = [
+            ctypes.POINTER(FplSetup), _c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_i64, _c_p,
+            ctypes.POINTER(ctypes.c_uint32), _c_p, ctypes.c_char_p, ctypes.c_size_t
+        ]
+        L.fpl_count_voxel_steps.restype = _c_i64
+        L.fpl_count_voxel_steps.argtypes = [
+            ctypes.POINTER(FplSetup), _c_p, ctypes.c_double, ctypes.c_double
+        ]
+        L.fpl_profile_enable.restype = None
+        L.fpl_profile_enable.argtypes = [ctypes.c_int]
+        L.fpl_profile_read.restype = ctypes.c_int
+        L.fpl_profile_read.argtypes = [
+            ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+            ctypes.POINTER(ctypes.c_int64)
+        ]
+        L.fpl_release_workspace.restype = None
+        L.fpl_release_workspace.argtypes = []
+        L.fpl_version.restype = ctypes.c_char_p
+        L.fpl_version.argtypes = []
+        _LIB = L
+    return _LIB
+
+
+def raise_warnings(mask, stacklevel=3):
+    for bit, msg in WARN_MESSAGES.items():
+        if mask & bit:
+            warnings.warn(msg, UserWarning, stacklevel=stacklevel)
+
+
+def check(rc, msg):
+    if rc == FPL_OK:
+        return
+    text = msg.value.decode(errors='replace') if msg.value else f'error {rc}'
+    if rc in (FPL_E_DEVICE, FPL_E_NOMEM):
+        raise RuntimeError(text)
+    raise ValueError(text)
+
+
+def _ptr(a):
+    """Raw address of a numpy array or torch tensor."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+def _stream():
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+def make_setup(step_size, light_intensity, voxel_size, wavelength, tissue_refrection,
+               interpolate, untilt_sensor_view, flip_z, filter_rotations):
+    """Returns (FplSetup, keep-alive rotations array)."""
+    rho = np.ascontiguousarray(filter_rotations, dtype=np.float64).ravel()
+    s = FplSetup(float(step_size), float(light_intensity), float(voxel_size), float(wavelength),
+                 float(tissue_refrection), INTERP[interpolate], int(bool(untilt_sensor_view)),
+                 int(bool(flip_z)), int(rho.size), rho.ctypes.data if rho.size else None)
+    return s, rho
+
+
+def generate(dim, origin, voxel_size, points, fiber_ptr, bundle_ptr, layer_ptr, layer_table,
+             only_label, label, vector):
+    """Call fpl_generate on device tensors; returns (props ndarray, warn mask)."""
+    scale, dn, mu, model = (np.ascontiguousarray(a) for a in layer_table)
+    bundle_ptr = np.ascontiguousarray(bundle_ptr, dtype=np.int64)
+    layer_ptr = np.ascontiguousarray(layer_ptr, dtype=np.int64)
+    vol = FplVolume((_c_i64 * 3)(*[int(d) for d in dim]),
+                    (ctypes.c_double * 3)(*[float(o) for o in origin]), float(voxel_size))
+    fbs = FplFiberBundles(
+        len(bundle_ptr) - 1,
+        int(fiber_ptr.shape[0]) - 1, int(points.shape[0]), _ptr(points), _ptr(fiber_ptr),
+        bundle_ptr.ctypes.data, layer_ptr.ctypes.data,
+        scale.ctypes.data if scale.size else None, dn.ctypes.data if dn.size else None,
+        mu.ctypes.data if mu.size else None, model.ctypes.data if model.size else None)
+    L = lib()
+    n_props = L.fpl_num_properties(ctypes.byref(fbs))
+    props = np.zeros((n_props, 2), dtype=np.float64)
+    warn = ctypes.c_uint32(0)
+    msg = ctypes.create_string_buffer(512)
+    rc = L.fpl_generate(ctypes.byref(vol), ctypes.byref(fbs), int(bool(only_label)),
+                        _ptr(label), _ptr(vector), props.ctypes.data, ctypes.byref(warn),
+                        _stream(), msg, 512)
+    check(rc, msg)
+    return props, warn.value
+
+
+def simulate(setup, dim, label, vector, props, tilts, images):
+    """Call fpl_simulate on device tensors; returns the warn mask."""
+    props = np.ascontiguousarray(props, dtype=np.float64)
+    tilts = np.ascontiguousarray(tilts, dtype=np.float64).reshape(-1, 2)
+    dim_arr = np.ascontiguousarray(dim, dtype=np.int64)
+    warn = ctypes.c_uint32(0)
+    msg = ctypes.create_string_buffer(512)
+    rc = lib().fpl_simulate(ctypes.byref(setup), dim_arr.ctypes.data, _ptr(label), _ptr(vector),
+                            props.ctypes.data, props.shape[0], tilts.ctypes.data,
+                            tilts.shape[0], _ptr(images), ctypes.byref(warn), _stream(), msg,
+                            512)
+    check(rc, msg)
+    return warn.value
+
+
+def count_voxel_steps(setup, dim, theta, phi):
+    dim_arr = np.ascontiguousarray(dim, dtype=np.int64)
+    return int(lib().fpl_count_voxel_steps(ctypes.byref(setup), dim_arr.ctypes.data,
+                                           float(theta), float(phi)))
+
+
+def profile_enable(on=True):
+    lib().fpl_profile_enable(int(bool(on)))
+
+
+def profile_read(name):
+    """(total milliseconds, launches) of a kernel since profile_enable()."""
+    ms = ctypes.c_double(0)
+    n = ctypes.c_int64(0)
+    lib().fpl_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(n))
+    return ms.value, n.value

@@ -1,0 +1,198 @@
+// Library-wide entry points and the per-stream device workspace.
+#include <cmath>
+#include <map>
+#include <string>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "fpl_internal.h"
+
+namespace fpl {
+namespace {
+
+struct Workspace {
+   void *ptr[WS_NUM_SLOTS] = {};
+   size_t cap[WS_NUM_SLOTS] = {};
+   void *host = nullptr;
+   size_t host_cap = 0;
+};
+
+std::mutex g_mutex;
+std::map<std::pair<int, void *>, Workspace> g_ws;
+
+Workspace &ws_for(void *stream) {
+   int dev = 0;
+   (void)hipGetDevice(&dev);
+   std::lock_guard<std::mutex> lock(g_mutex);
+   return g_ws[{dev, stream}];
+}
+
+} // namespace
+
+Status ws_get(void *stream, int slot, size_t bytes, void **ptr) {
+   Workspace &w = ws_for(stream);
+   if (bytes == 0) bytes = 1;
+   if (w.cap[slot] < bytes) {
+      if (w.ptr[slot]) {
+         // the stream may still use the old buffer
+         FPL_HIP(hipStreamSynchronize((hipStream_t)stream));
+         FPL_HIP(hipFree(w.ptr[slot]));
+         w.ptr[slot] = nullptr;
+         w.cap[slot] = 0;
+      }
+      // grow by 1/8 to avoid re-allocating on every slightly larger call
+      size_t want = bytes + bytes / 8;
+      want = (want + 255) & ~size_t(255);
+      hipError_t e = hipMalloc(&w.ptr[slot], want);
+      if (e != hipSuccess) {
+         (void)hipGetLastError();
+         w.ptr[slot] = nullptr;
+         return make_status(FPL_E_NOMEM, "device allocation of %zu bytes failed: %s", want,
+                            hipGetErrorString(e));
+      }
+      w.cap[slot] = want;
+   }
+   *ptr = w.ptr[slot];
+   return Status();
+}
+
+Status host_stage(void *stream, size_t bytes, void **ptr) {
+   Workspace &w = ws_for(stream);
+   if (w.host_cap < bytes) {
+      if (w.host) {
+         FPL_HIP(hipStreamSynchronize((hipStream_t)stream));
+         FPL_HIP(hipHostFree(w.host));
+         w.host = nullptr;
+         w.host_cap = 0;
+      }
+      size_t want = bytes < 4096 ? 4096 : bytes;
+      FPL_HIP(hipHostMalloc(&w.host, want, hipHostMallocDefault));
+      w.host_cap = want;
+   }
+   *ptr = w.host;
+   return Status();
+}
+
+namespace {
+std::mutex g_prof_mutex;
+bool g_prof_on = false;
+std::map<std::string, std::pair<double, int64_t>> g_prof;
+} // namespace
+
+KernelTimer::KernelTimer(const char *n, hipStream_t s) : name(n), stream(s) {
+   {
+      std::lock_guard<std::mutex> lock(g_prof_mutex);
+      on = g_prof_on;
+   }
+   if (!on) return;
+   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      on = false;
+      return;
+   }
+   (void)hipEventRecord(a, stream);
+}
+
+void KernelTimer::stop() {
+   if (on) (void)hipEventRecord(b, stream);
+}
+
+void KernelTimer::collect() {
+   if (!on) return;
+   float ms = 0.0f;
+   if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess) {
+      std::lock_guard<std::mutex> lock(g_prof_mutex);
+      auto &e = g_prof[name];
+      e.first += ms;
+      e.second += 1;
+   }
+   on = false;
+}
+
+KernelTimer::~KernelTimer() {
+   if (a) (void)hipEventDestroy(a);
+   if (b) (void)hipEventDestroy(b);
+}
+
+} // namespace fpl
+
+extern "C" {
+
+void fpl_profile_enable(int enable) {
+   std::lock_guard<std::mutex> lock(fpl::g_prof_mutex);
+   fpl::g_prof_on = enable != 0;
+   if (enable) fpl::g_prof.clear();
+}
+
+int fpl_profile_read(const char *name, double *total_ms, int64_t *launches) {
+   std::lock_guard<std::mutex> lock(fpl::g_prof_mutex);
+   auto it = fpl::g_prof.find(name ? name : "");
+   if (it == fpl::g_prof.end()) {
+      if (total_ms) *total_ms = 0;
+      if (launches) *launches = 0;
+      return -1;
+   }
+   if (total_ms) *total_ms = it->second.first;
+   if (launches) *launches = it->second.second;
+   return 0;
+}
+
+const char *fpl_version(void) { return "fastpli_amd 0.1.0 (gfx950)"; }
+
+void fpl_release_workspace(void) {
+   int dev = 0;
+   (void)hipGetDevice(&dev);
+   std::lock_guard<std::mutex> lock(fpl::g_mutex);
+   for (auto it = fpl::g_ws.begin(); it != fpl::g_ws.end();) {
+      if (it->first.first != dev) {
+         ++it;
+         continue;
+      }
+      (void)hipStreamSynchronize((hipStream_t)it->first.second);
+      for (int s = 0; s < fpl::WS_NUM_SLOTS; s++)
+         if (it->second.ptr[s]) (void)hipFree(it->second.ptr[s]);
+      if (it->second.host) (void)hipHostFree(it->second.host);
+      it = fpl::g_ws.erase(it);
+   }
+}
+
+int64_t fpl_num_properties(const fpl_fiber_bundles *fbs) {
+   if (!fbs) return 1;
+   int64_t max_layer = 0;
+   for (int64_t b = 0; b < fbs->n_bundles; b++) {
+      int64_t n = fbs->layer_ptr[b + 1] - fbs->layer_ptr[b];
+      if (n > max_layer) max_layer = n;
+   }
+   return fbs->n_bundles * max_layer + 1;
+}
+
+// z-march iteration count of simulator.cpp:259-261: every ray starts at
+// z = 0 + 0.5 * step_z and stops once floor(z) leaves [0, nz).
+int64_t fpl_count_voxel_steps(const fpl_setup *setup, const int64_t dim[3], double theta,
+                              double phi) {
+   (void)phi;
+   double c = std::cos(theta);
+   double st;
+   {
+      // vm::rot_pi_cases::cos (vemath.hpp:510-520)
+      double v = std::fmod(theta, 2 * M_PI);
+      if (v == 0)
+         st = 1;
+      else if (v == M_PI_2 || v == M_PI + M_PI_2 || v == -M_PI_2 || v == -(M_PI + M_PI_2))
+         st = 0;
+      else if (v == M_PI || v == -M_PI)
+         st = -1;
+      else
+         st = c;
+   }
+   const double step_z = st * setup->step_size;
+   double z = 0.0 + step_z * 0.5;
+   int64_t n = 0;
+   for (; std::floor(z) >= 0 && std::floor(z) < (double)dim[2]; z = z + step_z) {
+      n++;
+      if (n > (int64_t)1 << 40) break;
+   }
+   return n * dim[0] * dim[1] * (int64_t)setup->n_rho;
+}
+
+} // extern "C"

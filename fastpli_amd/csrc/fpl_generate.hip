@@ -1,0 +1,679 @@
+// Tissue generation (voxelization of layered fiber polylines) for gfx950.
+//
+// Semantics: fastpli PliGenerator::RunTissueGeneration / FillVoxelsAroundFiberSegment
+// (src/simulation/generator.cpp:165-383) -- for every voxel, the segments whose
+// integer bounding box contains the voxel are visited in global
+// (bundle, fiber, segment) order; a segment updates the voxel when its
+// float-rounded FP64 squared distance is within the outer layer and not larger
+// than the stored one (later segment wins ties).
+//
+// GPU structure (no atomics on the volume, every voxel written exactly once):
+//   k_points      one thread per point: translate/resize (fiber.cpp:128-150),
+//                 fiber and bundle AABBs (fiber.cpp:83-99, fiber_bundle.cpp:147-154)
+//                 reduced with ordered-integer atomic min/max (min/max is exact
+//                 and order independent)
+//   k_fibers      one thread per fiber: bundle/fiber culling (generator.cpp:208-221)
+//   k_segments    one thread per segment: FP64 record + integer bbox
+//                 (generator.cpp:293-305) + number of 8x8x32 voxel tiles it touches
+//   scan          hipcub exclusive sum of per-segment tile counts
+//   k_emit        (tile, segment) pairs in segment order
+//   radix sort    stable by tile -> per tile list in global segment order
+//   k_voxelize    one workgroup per tile; each wave owns a 4x4x32 column
+//                 block, each lane 8 voxels; candidate records are wave-uniform
+//                 loads; per voxel the reference's update rule runs in order.
+//
+// This file must be compiled with -ffp-contract=off: the distance test must
+// round every FP64 operation exactly as the reference's scalar code.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "fpl_internal.h"
+
+namespace fpl {
+namespace gen {
+
+constexpr int TX = 8, TY = 8, TZ = 32; // voxel tile of one workgroup
+constexpr int WG = 256;                // 4 waves; wave w owns the 4x4x32 block (w/2, w%2)
+
+struct SegRec {      // 128 bytes, one per segment (indexed by its first point)
+   double p[3];      // start point, voxel units
+   double q[3];      // end point
+   double v[3];      // q - p
+   double c2;        // dot(v, v)
+   double rp, rq;    // radii at p and q
+   int32_t lo[3];    // integer bbox [lo, hi) after intersection with the volume
+   int32_t hi[3];
+   int32_t bundle;
+   int32_t pad;
+};
+static_assert(sizeof(SegRec) == 128, "SegRec layout");
+
+struct BundleDev {
+   int32_t layer_off;  // into the layer tables
+   int32_t n_layers;
+   int32_t label_base; // 1 + b * max_layer
+   float f_last_sq;    // float(scale_sq.back())
+   double sqrt_last;   // sqrt(scale_sq.back())
+   int64_t fiber_begin, fiber_end;
+};
+
+struct Params {
+   int64_t nx, ny, nz;
+   int64_t n_points, n_fibers, n_bundles;
+   double off[3];  // -origin
+   int32_t shift;  // origin != 0
+   double scale;   // 1.0 / voxel_size
+   int32_t max_layer;
+   int32_t ntx, nty, ntz;
+};
+
+// ordered encoding: unsigned compare == double compare (no NaN)
+__device__ __forceinline__ unsigned long long ord_enc(double d) {
+   unsigned long long b = (unsigned long long)__double_as_longlong(d);
+   return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double ord_dec(unsigned long long e) {
+   unsigned long long b = (e >> 63) ? (e & 0x7fffffffffffffffull) : ~e;
+   return __longlong_as_double((long long)b);
+}
+
+// box layout per object: lo x,y,z then hi x,y,z (6 ordered u64)
+__device__ __forceinline__ void box_unite(unsigned long long *box, const double lo[3],
+                                          const double hi[3]) {
+   for (int i = 0; i < 3; i++) {
+      atomicMin(&box[i], ord_enc(lo[i]));
+      atomicMax(&box[3 + i], ord_enc(hi[i]));
+   }
+}
+
+// aabb::AABB(p, q) unsorted constructor (aabb.hpp:54-71)
+__device__ __forceinline__ void box_of(const double p[3], const double q[3], double lo[3],
+                                       double hi[3]) {
+   for (int i = 0; i < 3; i++) {
+      if (p[i] < q[i]) {
+         lo[i] = p[i];
+         hi[i] = q[i];
+      } else {
+         lo[i] = q[i];
+         hi[i] = p[i];
+      }
+   }
+}
+
+__device__ __forceinline__ int64_t find_fiber(const int64_t *fiber_ptr, int64_t n_fibers,
+                                              int64_t i) {
+   // largest f with fiber_ptr[f] <= i
+   int64_t lo = 0, hi = n_fibers; // answer in [lo, hi)
+   while (hi - lo > 1) {
+      int64_t mid = (lo + hi) >> 1;
+      if (fiber_ptr[mid] <= i)
+         lo = mid;
+      else
+         hi = mid;
+   }
+   return lo;
+}
+
+__device__ __forceinline__ int find_bundle(const BundleDev *bd, int64_t nb, int64_t f) {
+   int lo = 0, hi = (int)nb;
+   while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (bd[mid].fiber_begin <= f)
+         lo = mid;
+      else
+         hi = mid;
+   }
+   return lo;
+}
+
+__device__ __forceinline__ void load_point(const double *P, int64_t i, const Params &pr,
+                                           double p[3], double &r) {
+   double x = P[4 * i + 0], y = P[4 * i + 1], z = P[4 * i + 2];
+   r = P[4 * i + 3];
+   if (pr.shift) { // Fiber::Translate (fiber.cpp:128-133)
+      x = x + pr.off[0];
+      y = y + pr.off[1];
+      z = z + pr.off[2];
+   }
+   // Fiber::Resize (fiber.cpp:144-150)
+   p[0] = x * pr.scale;
+   p[1] = y * pr.scale;
+   p[2] = z * pr.scale;
+   r = r * pr.scale;
+}
+
+__global__ void k_init_boxes(unsigned long long *boxes, int64_t n_boxes, uint32_t *flags) {
+   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n_boxes * 6) {
+      const int c = (int)(i % 6);
+      boxes[i] = c < 3 ? ord_enc(INFINITY) : ord_enc(-INFINITY);
+   }
+   if (i == 0) flags[0] = 0;
+}
+
+__global__ void k_points(Params pr, const double *__restrict__ P,
+                         const int64_t *__restrict__ fiber_ptr, const BundleDev *__restrict__ bd,
+                         unsigned long long *fiber_box, unsigned long long *bundle_box) {
+   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= pr.n_points) return;
+   const int64_t f = find_fiber(fiber_ptr, pr.n_fibers, i);
+   const int b = find_bundle(bd, pr.n_bundles, f);
+   const int64_t end = fiber_ptr[f + 1];
+   double p[3], r;
+   load_point(P, i, pr, p, r);
+   double lo[3], hi[3];
+   if (i == fiber_ptr[f]) {
+      // first point: AABB(p - r, p + r) (fiber.cpp:90-91)
+      double a[3] = {p[0] - r, p[1] - r, p[2] - r};
+      double c[3] = {p[0] + r, p[1] + r, p[2] + r};
+      box_of(a, c, lo, hi);
+      box_unite(fiber_box + 6 * f, lo, hi);
+      box_unite(bundle_box + 6 * b, lo, hi);
+   }
+   if (i + 1 < end) {
+      double q[3], rq;
+      load_point(P, i + 1, pr, q, rq);
+      const double rm = (r < rq) ? rq : r; // std::max
+      box_of(p, q, lo, hi);
+      for (int k = 0; k < 3; k++) {
+         lo[k] -= rm;
+         hi[k] += rm;
+      }
+      box_unite(fiber_box + 6 * f, lo, hi);
+      box_unite(bundle_box + 6 * b, lo, hi);
+   }
+}
+
+__device__ __forceinline__ bool overlaps_volume(const unsigned long long *box, const Params &pr) {
+   // aabb::Overlap(volume_bb, box), strict (aabb.hpp:128-137)
+   const double vhi[3] = {(double)pr.nx, (double)pr.ny, (double)pr.nz};
+   for (int k = 0; k < 3; k++) {
+      const double lo = ord_dec(box[k]), hi = ord_dec(box[3 + k]);
+      if (0.0 >= hi || vhi[k] <= lo) return false;
+   }
+   return true;
+}
+
+__global__ void k_fibers(Params pr, const int64_t *__restrict__ fiber_ptr,
+                         const BundleDev *__restrict__ bd,
+                         const unsigned long long *__restrict__ fiber_box,
+                         const unsigned long long *__restrict__ bundle_box, uint8_t *fiber_ok,
+                         uint32_t *flags) {
+   int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (f >= pr.n_fibers) return;
+   const int b = find_bundle(bd, pr.n_bundles, f);
+   const int64_t n = fiber_ptr[f + 1] - fiber_ptr[f];
+   const bool bundle_ok = overlaps_volume(bundle_box + 6 * b, pr) && bd[b].n_layers > 0;
+   uint8_t ok = 0;
+   if (bundle_ok) {
+      if (n <= 1)
+         atomicOr(&flags[0], FPL_WARN_SINGLE_POINT);
+      else if (overlaps_volume(fiber_box + 6 * f, pr))
+         ok = 1;
+   }
+   fiber_ok[f] = ok;
+}
+
+__global__ void k_segments(Params pr, const double *__restrict__ P,
+                           const int64_t *__restrict__ fiber_ptr,
+                           const BundleDev *__restrict__ bd, const uint8_t *__restrict__ fiber_ok,
+                           SegRec *segs, int64_t *seg_count) {
+   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= pr.n_points) return;
+   const int64_t f = find_fiber(fiber_ptr, pr.n_fibers, i);
+   int64_t count = 0;
+   if (fiber_ok[f] && i + 1 < fiber_ptr[f + 1]) {
+      const int b = find_bundle(bd, pr.n_bundles, f);
+      SegRec R;
+      double rp, rq;
+      load_point(P, i, pr, R.p, rp);
+      load_point(P, i + 1, pr, R.q, rq);
+      R.rp = rp;
+      R.rq = rq;
+      for (int k = 0; k < 3; k++) R.v[k] = R.q[k] - R.p[k];
+      double c2 = 0.0;
+      c2 = c2 + R.v[0] * R.v[0];
+      c2 = c2 + R.v[1] * R.v[1];
+      c2 = c2 + R.v[2] * R.v[2];
+      R.c2 = c2;
+      // generator.cpp:295-305
+      const double max_r = ((rp < rq) ? rq : rp) * bd[b].sqrt_last;
+      double lo[3], hi[3];
+      box_of(R.p, R.q, lo, hi);
+      const double vhi[3] = {(double)pr.nx, (double)pr.ny, (double)pr.nz};
+      bool empty = false;
+      int64_t ilo[3], ihi[3];
+      for (int k = 0; k < 3; k++) {
+         lo[k] -= max_r;
+         hi[k] += max_r;
+         if (0.0 > lo[k]) lo[k] = 0.0; // Intersect (aabb.hpp:108-115)
+         if (vhi[k] < hi[k]) hi[k] = vhi[k];
+         ilo[k] = (int64_t)lo[k];
+         ihi[k] = (int64_t)(hi[k] + 0.5);
+         if (!(ilo[k] < ihi[k])) empty = true;
+      }
+      if (!empty) {
+         for (int k = 0; k < 3; k++) {
+            R.lo[k] = (int32_t)ilo[k];
+            R.hi[k] = (int32_t)ihi[k];
+         }
+         R.bundle = b;
+         R.pad = 0;
+         segs[i] = R;
+         const int64_t tx0 = ilo[0] / TX, tx1 = (ihi[0] - 1) / TX;
+         const int64_t ty0 = ilo[1] / TY, ty1 = (ihi[1] - 1) / TY;
+         const int64_t tz0 = ilo[2] / TZ, tz1 = (ihi[2] - 1) / TZ;
+         count = (tx1 - tx0 + 1) * (ty1 - ty0 + 1) * (tz1 - tz0 + 1);
+      }
+   }
+   seg_count[i] = count;
+}
+
+__global__ void k_emit(Params pr, const SegRec *__restrict__ segs,
+                       const int64_t *__restrict__ seg_count, const int64_t *__restrict__ seg_off,
+                       uint32_t *keys, uint32_t *vals) {
+   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= pr.n_points) return;
+   if (seg_count[i] == 0) return;
+   const SegRec &R = segs[i];
+   int64_t o = seg_off[i];
+   const int tx0 = R.lo[0] / TX, tx1 = (R.hi[0] - 1) / TX;
+   const int ty0 = R.lo[1] / TY, ty1 = (R.hi[1] - 1) / TY;
+   const int tz0 = R.lo[2] / TZ, tz1 = (R.hi[2] - 1) / TZ;
+   for (int tx = tx0; tx <= tx1; tx++)
+      for (int ty = ty0; ty <= ty1; ty++)
+         for (int tz = tz0; tz <= tz1; tz++) {
+            keys[o] = (uint32_t)(((int64_t)tx * pr.nty + ty) * pr.ntz + tz);
+            vals[o] = (uint32_t)i;
+            o++;
+         }
+}
+
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) p[i] = v;
+}
+
+__global__ void k_ranges(const uint32_t *__restrict__ keys, int64_t n, uint32_t *range) {
+   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (p >= n) return;
+   const uint32_t t = keys[p];
+   if (p == 0 || keys[p - 1] != t) range[2 * (int64_t)t] = (uint32_t)p;
+   if (p == n - 1 || keys[p + 1] != t) range[2 * (int64_t)t + 1] = (uint32_t)(p + 1);
+}
+
+// vm::unit (vemath.hpp:390-402) rounded to float
+__device__ __forceinline__ void unit_to_float(double x, double y, double z, float *out) {
+   double n2 = 0.0;
+   n2 = n2 + x * x;
+   n2 = n2 + y * y;
+   n2 = n2 + z * z;
+   if (n2 > 0) {
+      const double inv = 1.0 / sqrt(n2);
+      x = x * inv;
+      y = y * inv;
+      z = z * inv;
+   }
+   out[0] = (float)x;
+   out[1] = (float)y;
+   out[2] = (float)z;
+}
+
+struct Closest {
+   double c[3];
+   double t;
+};
+
+// ShortestPointToLineSegmentVecCalculation (generator.cpp:475-494)
+__device__ __forceinline__ Closest closest(const double pt[3], const SegRec &R) {
+   const double w0 = pt[0] - R.p[0], w1 = pt[1] - R.p[1], w2 = pt[2] - R.p[2];
+   double c1 = 0.0;
+   c1 = c1 + w0 * R.v[0];
+   c1 = c1 + w1 * R.v[1];
+   c1 = c1 + w2 * R.v[2];
+   Closest r;
+   if (c1 < 0) {
+      r.c[0] = R.p[0];
+      r.c[1] = R.p[1];
+      r.c[2] = R.p[2];
+      r.t = 0.0;
+   } else if (R.c2 <= c1) {
+      r.c[0] = R.q[0];
+      r.c[1] = R.q[1];
+      r.c[2] = R.q[2];
+      r.t = 1.0;
+   } else {
+      const double b = c1 / R.c2;
+      r.c[0] = R.p[0] + R.v[0] * b;
+      r.c[1] = R.p[1] + R.v[1] * b;
+      r.c[2] = R.p[2] + R.v[2] * b;
+      r.t = b;
+   }
+   return r;
+}
+
+__device__ __forceinline__ int tile_of_block(int bid, int n) {
+   // keep spatially adjacent tiles (which share segment records) on one XCD:
+   // the hardware deals consecutive workgroups round-robin over 8 XCDs
+   const int full = (n / 8) * 8;
+   if (bid >= full) return bid;
+   const int per = full / 8;
+   return (bid % 8) * per + bid / 8;
+}
+
+__global__ void __launch_bounds__(WG)
+    k_voxelize(Params pr, const SegRec *__restrict__ segs, const uint32_t *__restrict__ list,
+               const uint32_t *__restrict__ range, const BundleDev *__restrict__ bd,
+               const double *__restrict__ layer_sq, const uint8_t *__restrict__ layer_model,
+               int only_label, int32_t *__restrict__ label, float *__restrict__ vector,
+               uint32_t *flags) {
+   const int n_tiles = pr.ntx * pr.nty * pr.ntz;
+   const int t = tile_of_block((int)blockIdx.x, n_tiles);
+   const int tz = t % pr.ntz;
+   const int ty = (t / pr.ntz) % pr.nty;
+   const int tx = t / (pr.ntz * pr.nty);
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int xw = tx * TX + 4 * (wave >> 1);
+   const int yw = ty * TY + 4 * (wave & 1);
+   const int zw = tz * TZ;
+   const int x = xw + (lane >> 4);
+   const int y = yw + ((lane >> 2) & 3);
+   const int lz = lane & 3;
+
+   float best_d[8];
+   int32_t best_seg[8];
+   int32_t best_label[8];
+#pragma unroll
+   for (int k = 0; k < 8; k++) {
+      best_d[k] = INFINITY;
+      best_seg[k] = -1;
+      best_label[k] = 0;
+   }
+   uint32_t overlap = 0;
+   const int max_layer = pr.max_layer;
+
+   const uint32_t beg = range[2 * t], end = range[2 * t + 1];
+   for (uint32_t it = beg; it < end; it++) {
+      const uint32_t s = list[it];
+      const SegRec &R = segs[s];
+      const int lo0 = R.lo[0], lo1 = R.lo[1], lo2 = R.lo[2];
+      const int hi0 = R.hi[0], hi1 = R.hi[1], hi2 = R.hi[2];
+      // wave-uniform rejection of the 4x4x32 block
+      if (hi0 <= xw || lo0 >= xw + 4 || hi1 <= yw || lo1 >= yw + 4 || hi2 <= zw ||
+          lo2 >= zw + TZ)
+         continue;
+      if (x < lo0 || x >= hi0 || y < lo1 || y >= hi1) continue;
+      const BundleDev B = bd[R.bundle];
+      const float f_last = B.f_last_sq;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+         const int zb = zw + 4 * k;
+         if (hi2 <= zb || lo2 >= zb + 4) continue;
+         const int z = zb + lz;
+         if (z < lo2 || z >= hi2) continue;
+         const double pt[3] = {x + 0.5, y + 0.5, z + 0.5};
+         const Closest cp = closest(pt, R);
+         const double d0 = cp.c[0] - pt[0], d1 = cp.c[1] - pt[1], d2v = cp.c[2] - pt[2];
+         double dd = 0.0;
+         dd = dd + d0 * d0;
+         dd = dd + d1 * d1;
+         dd = dd + d2v * d2v;
+         const float dist_sq = (float)dd;
+         const double rr = R.rp * (1 - cp.t) + R.rq * cp.t;
+         const float r_sq = (float)(rr * rr);
+         if (dist_sq > f_last * r_sq) continue;
+         if (best_d[k] >= dist_sq) {
+            // std::lower_bound over the bundle's double scale^2 list with the
+            // float ratio (generator.cpp:345-349)
+            const double ratio = (double)(dist_sq / r_sq);
+            int ly = 0;
+            while (ly < B.n_layers && layer_sq[B.layer_off + ly] < ratio) ly++;
+            const int32_t new_label = ly + B.label_base;
+            const int32_t old = best_label[k];
+            if (old != 0 && abs(old - new_label) >= max_layer) overlap = 1;
+            best_d[k] = dist_sq;
+            best_seg[k] = (int32_t)s;
+            best_label[k] = new_label;
+         }
+      }
+   }
+
+   const bool in_xy = x < pr.nx && y < pr.ny;
+#pragma unroll
+   for (int k = 0; k < 8; k++) {
+      const int z = zw + 4 * k + lz;
+      if (!in_xy || z >= pr.nz) continue;
+      const int64_t ind = ((int64_t)x * pr.ny + y) * pr.nz + z;
+      label[ind] = best_label[k];
+      if (!only_label) {
+         float vo[3] = {0.0f, 0.0f, 0.0f};
+         if (best_seg[k] >= 0) {
+            const SegRec &R = segs[best_seg[k]];
+            const BundleDev B = bd[R.bundle];
+            int ly = best_label[k] - B.label_base;
+            // a float ratio just above the last scale^2 gives the past-the-end
+            // layer index; its orientation is read as the outermost layer's
+            if (ly > B.n_layers - 1) ly = B.n_layers - 1;
+            const uint8_t m = layer_model[B.layer_off + ly];
+            if (m == 'p') {
+               unit_to_float(R.v[0], R.v[1], R.v[2], vo);
+            } else if (m == 'r') {
+               const double pt[3] = {x + 0.5, y + 0.5, z + 0.5};
+               const Closest cp = closest(pt, R);
+               unit_to_float(pt[0] - cp.c[0], pt[1] - cp.c[1], pt[2] - cp.c[2], vo);
+            }
+         }
+         vector[3 * ind + 0] = vo[0];
+         vector[3 * ind + 1] = vo[1];
+         vector[3 * ind + 2] = vo[2];
+      }
+   }
+   if (__ballot(overlap != 0) != 0 && lane == 0) atomicOr(&flags[0], FPL_WARN_OVERLAP);
+}
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, int32_t *label,
+           float *vector, double *props, uint32_t *warn, hipStream_t st) {
+   if (!vol || !fbs) return make_status(FPL_E_INVALID, "null argument");
+   const int64_t nx = vol->dim[0], ny = vol->dim[1], nz = vol->dim[2];
+   if (nx <= 0 || ny <= 0 || nz <= 0)
+      return make_status(FPL_E_INVALID, "dim.global[any] <= 0: [%lld,%lld,%lld]", (long long)nx,
+                         (long long)ny, (long long)nz);
+   if (!(vol->voxel_size > 0))
+      return make_status(FPL_E_INVALID, "voxel_size <= 0: %f", vol->voxel_size);
+   if (!label || (!only_label && !vector))
+      return make_status(FPL_E_INVALID, "output pointers are null");
+   if (fbs->n_bundles < 0 || fbs->n_fibers < 0 || fbs->n_points < 0)
+      return make_status(FPL_E_INVALID, "negative sizes");
+   if (fbs->n_points >= ((int64_t)1 << 31))
+      return make_status(FPL_E_INVALID, "too many fiber points: %lld", (long long)fbs->n_points);
+   if (fbs->bundle_ptr[fbs->n_bundles] != fbs->n_fibers)
+      return make_status(FPL_E_INVALID, "fbs and prs not the same size");
+
+   // ---- host tables: bundles, sorted layers, property list
+   const int64_t nb = fbs->n_bundles;
+   int32_t max_layer = 0;
+   std::vector<BundleDev> bdev(nb > 0 ? nb : 1);
+   std::vector<double> lsq;
+   std::vector<uint8_t> lmodel;
+   std::vector<double> ldn, lmu;
+   for (int64_t b = 0; b < nb; b++) {
+      std::vector<int64_t> order;
+      for (int64_t l = fbs->layer_ptr[b]; l < fbs->layer_ptr[b + 1]; l++) order.push_back(l);
+      // fiber::Bundle sorts layers by scale (fiber_bundle.cpp:83-86)
+      std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t c) {
+         return fbs->layer_scale[a] < fbs->layer_scale[c];
+      });
+      BundleDev &B = bdev[b];
+      B.layer_off = (int32_t)lsq.size();
+      B.n_layers = (int32_t)order.size();
+      for (int64_t l : order) {
+         const uint8_t m = fbs->layer_model[l];
+         if (m != 'b' && m != 'p' && m != 'r')
+            return make_status(FPL_E_INVALID, "Orientation must be \"b\", \"p\", or \"r\"");
+         lsq.push_back(fbs->layer_scale[l] * fbs->layer_scale[l]);
+         lmodel.push_back(m);
+         ldn.push_back(fbs->layer_dn[l]);
+         lmu.push_back(fbs->layer_mu[l]);
+      }
+      if (B.n_layers > max_layer) max_layer = B.n_layers;
+      const double last = B.n_layers ? lsq.back() : 0.0;
+      B.f_last_sq = (float)last;
+      B.sqrt_last = std::sqrt(last);
+      B.fiber_begin = fbs->bundle_ptr[b];
+      B.fiber_end = fbs->bundle_ptr[b + 1];
+   }
+   for (int64_t b = 0; b < nb; b++) bdev[b].label_base = (int32_t)(1 + b * max_layer);
+   if (lsq.empty()) {
+      lsq.push_back(0.0);
+      lmodel.push_back('b');
+   }
+   // GetPropertyList (generator.cpp:496-526)
+   const int64_t n_props = nb * max_layer + 1;
+   if (props) {
+      for (int64_t i = 0; i < 2 * n_props; i++) props[i] = 0.0;
+      for (int64_t b = 0; b < nb; b++)
+         for (int l = 0; l < bdev[b].n_layers; l++) {
+            const int64_t id = 1 + l + b * max_layer;
+            const int64_t src = bdev[b].layer_off + l;
+            props[2 * id + 1] = lmu[src];
+            props[2 * id + 0] = lmodel[src] == 'b' ? 0.0 : ldn[src];
+         }
+   }
+
+   Params pr;
+   pr.nx = nx;
+   pr.ny = ny;
+   pr.nz = nz;
+   pr.n_points = fbs->n_points;
+   pr.n_fibers = fbs->n_fibers;
+   pr.n_bundles = nb;
+   pr.shift = (vol->origin[0] != 0 || vol->origin[1] != 0 || vol->origin[2] != 0);
+   for (int k = 0; k < 3; k++) pr.off[k] = -vol->origin[k];
+   pr.scale = 1.0 / vol->voxel_size;
+   pr.max_layer = max_layer;
+   pr.ntx = (int32_t)((nx + TX - 1) / TX);
+   pr.nty = (int32_t)((ny + TY - 1) / TY);
+   pr.ntz = (int32_t)((nz + TZ - 1) / TZ);
+   const int64_t n_tiles = (int64_t)pr.ntx * pr.nty * pr.ntz;
+   if (n_tiles >= ((int64_t)1 << 31))
+      return make_status(FPL_E_INVALID, "volume too large: %lld tiles", (long long)n_tiles);
+
+   // ---- small tables to the device
+   const size_t sz_bd = sizeof(BundleDev) * bdev.size();
+   const size_t sz_lsq = sizeof(double) * lsq.size();
+   const size_t off_lsq = (sz_bd + 255) & ~size_t(255);
+   const size_t off_lm = (off_lsq + sz_lsq + 255) & ~size_t(255);
+   const size_t small_bytes = off_lm + lmodel.size();
+   void *small = nullptr, *stage = nullptr;
+   FPL_TRY(ws_get(st, WS_SMALL, small_bytes, &small));
+   FPL_TRY(host_stage(st, small_bytes + 64, &stage));
+   FPL_HIP(hipStreamSynchronize(st)); // stage may still be read by a previous copy
+   memcpy((char *)stage, bdev.data(), sz_bd);
+   memcpy((char *)stage + off_lsq, lsq.data(), sz_lsq);
+   memcpy((char *)stage + off_lm, lmodel.data(), lmodel.size());
+   FPL_HIP(hipMemcpyAsync(small, stage, small_bytes, hipMemcpyHostToDevice, st));
+   const BundleDev *d_bd = (const BundleDev *)small;
+   const double *d_lsq = (const double *)((char *)small + off_lsq);
+   const uint8_t *d_lm = (const uint8_t *)((char *)small + off_lm);
+
+   // ---- scratch
+   const int64_t np = fbs->n_points, nf = fbs->n_fibers;
+   void *boxes, *fok, *segs, *cnt, *off, *flags;
+   FPL_TRY(ws_get(st, WS_BOXES, sizeof(unsigned long long) * 6 * (nf + nb + 1), &boxes));
+   FPL_TRY(ws_get(st, WS_FIBER_OK, nf + 1, &fok));
+   FPL_TRY(ws_get(st, WS_SEGS, sizeof(SegRec) * (np + 1), &segs));
+   FPL_TRY(ws_get(st, WS_SEG_COUNT, sizeof(int64_t) * (np + 1), &cnt));
+   FPL_TRY(ws_get(st, WS_SEG_OFF, sizeof(int64_t) * (np + 2), &off));
+   FPL_TRY(ws_get(st, WS_FLAGS, 64, &flags));
+   unsigned long long *fiber_box = (unsigned long long *)boxes;
+   unsigned long long *bundle_box = fiber_box + 6 * nf;
+   uint32_t *d_flags = (uint32_t *)flags;
+
+   const int B = 256;
+   hipLaunchKernelGGL(k_init_boxes, dim3(grid_for(6 * (nf + nb) + 1, B)), dim3(B), 0, st,
+                      fiber_box, nf + nb, d_flags);
+   if (np > 0) {
+      hipLaunchKernelGGL(k_points, dim3(grid_for(np, B)), dim3(B), 0, st, pr, fbs->points,
+                         fbs->fiber_ptr, d_bd, fiber_box, bundle_box);
+   }
+   if (nf > 0) {
+      hipLaunchKernelGGL(k_fibers, dim3(grid_for(nf, B)), dim3(B), 0, st, pr, fbs->fiber_ptr, d_bd,
+                         fiber_box, bundle_box, (uint8_t *)fok, d_flags);
+   }
+   int64_t total = 0;
+   if (np > 0) {
+      hipLaunchKernelGGL(k_segments, dim3(grid_for(np, B)), dim3(B), 0, st, pr, fbs->points,
+                         fbs->fiber_ptr, d_bd, (const uint8_t *)fok, (SegRec *)segs,
+                         (int64_t *)cnt);
+      // exclusive scan over np + 1 entries (the last one is 0) gives the total
+      FPL_HIP(hipMemsetAsync((int64_t *)cnt + np, 0, sizeof(int64_t), st));
+      size_t tmp_bytes = 0;
+      FPL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, (int64_t *)cnt, (int64_t *)off,
+                                               (int)(np + 1), st));
+      void *tmp;
+      FPL_TRY(ws_get(st, WS_CUB_TEMP, tmp_bytes, &tmp));
+      FPL_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, (int64_t *)cnt, (int64_t *)off,
+                                               (int)(np + 1), st));
+      FPL_HIP(hipMemcpyAsync(stage, (int64_t *)off + np, sizeof(int64_t), hipMemcpyDeviceToHost,
+                             st));
+      FPL_HIP(hipStreamSynchronize(st));
+      total = *(int64_t *)stage;
+   }
+   if (total >= ((int64_t)1 << 31))
+      return make_status(FPL_E_INVALID, "too many segment/tile pairs: %lld", (long long)total);
+
+   void *ka, *va, *kb, *vb, *rng;
+   FPL_TRY(ws_get(st, WS_KEYS_A, sizeof(uint32_t) * (total + 1), &ka));
+   FPL_TRY(ws_get(st, WS_VALS_A, sizeof(uint32_t) * (total + 1), &va));
+   FPL_TRY(ws_get(st, WS_KEYS_B, sizeof(uint32_t) * (total + 1), &kb));
+   FPL_TRY(ws_get(st, WS_VALS_B, sizeof(uint32_t) * (total + 1), &vb));
+   FPL_TRY(ws_get(st, WS_TILE_RANGE, sizeof(uint32_t) * 2 * n_tiles, &rng));
+   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(2 * n_tiles, B)), dim3(B), 0, st,
+                      (uint32_t *)rng, 2 * n_tiles, 0u);
+   if (total > 0) {
+      hipLaunchKernelGGL(k_emit, dim3(grid_for(np, B)), dim3(B), 0, st, pr, (const SegRec *)segs,
+                         (const int64_t *)cnt, (const int64_t *)off, (uint32_t *)ka,
+                         (uint32_t *)va);
+      int nbits = 1;
+      while (((int64_t)1 << nbits) < n_tiles) nbits++;
+      size_t tmp_bytes = 0;
+      FPL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (uint32_t *)ka,
+                                                 (uint32_t *)kb, (uint32_t *)va, (uint32_t *)vb,
+                                                 (int)total, 0, nbits, st));
+      void *tmp;
+      FPL_TRY(ws_get(st, WS_CUB_TEMP, tmp_bytes, &tmp));
+      FPL_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, (uint32_t *)ka, (uint32_t *)kb,
+                                                 (uint32_t *)va, (uint32_t *)vb, (int)total, 0,
+                                                 nbits, st));
+      hipLaunchKernelGGL(k_ranges, dim3(grid_for(total, B)), dim3(B), 0, st, (uint32_t *)kb, total,
+                         (uint32_t *)rng);
+   }
+   KernelTimer kt("voxelize", st);
+   hipLaunchKernelGGL(k_voxelize, dim3((unsigned)n_tiles), dim3(WG), 0, st, pr,
+                      (const SegRec *)segs, (const uint32_t *)vb, (const uint32_t *)rng, d_bd,
+                      d_lsq, d_lm, only_label, label, vector, d_flags);
+   kt.stop();
+   FPL_HIP(hipGetLastError());
+   FPL_HIP(hipMemcpyAsync(stage, d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+   FPL_HIP(hipStreamSynchronize(st));
+   kt.collect();
+   if (warn) *warn = *(uint32_t *)stage;
+   return Status();
+}
+
+} // namespace gen
+} // namespace fpl
+
+extern "C" int fpl_generate(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label,
+                            int32_t *label, float *vector, double *props, uint32_t *warn,
+                            void *stream, char *msg, size_t msg_len) {
+   if (warn) *warn = 0;
+   fpl::Status s = fpl::gen::run(vol, fbs, only_label, label, vector, props, warn,
+                                 (hipStream_t)stream);
+   return fpl::finish(s, msg, msg_len);
+}

@@ -1,0 +1,97 @@
+// Internal host-side helpers shared by the translation units of the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/fastpli_amd.h"
+
+namespace fpl {
+
+// Error carrier: every entry point fills `msg` and returns a status code.
+struct Status {
+   int code = FPL_OK;
+   std::string msg;
+   bool ok() const { return code == FPL_OK; }
+};
+
+inline Status make_status(int code, const char *fmt, ...) {
+   char buf[512];
+   va_list ap;
+   va_start(ap, fmt);
+   vsnprintf(buf, sizeof(buf), fmt, ap);
+   va_end(ap);
+   Status s;
+   s.code = code;
+   s.msg = buf;
+   return s;
+}
+
+inline int finish(const Status &s, char *msg, size_t msg_len) {
+   if (msg && msg_len > 0) {
+      size_t n = s.msg.size() < msg_len - 1 ? s.msg.size() : msg_len - 1;
+      memcpy(msg, s.msg.data(), n);
+      msg[n] = '\0';
+   }
+   return s.code;
+}
+
+#define FPL_HIP(expr)                                                         \
+   do {                                                                       \
+      hipError_t e_ = (expr);                                                 \
+      if (e_ != hipSuccess)                                                   \
+         return ::fpl::make_status(FPL_E_DEVICE, "%s failed: %s", #expr,      \
+                                   hipGetErrorString(e_));                    \
+   } while (0)
+
+#define FPL_TRY(expr)                                                         \
+   do {                                                                       \
+      ::fpl::Status s_ = (expr);                                              \
+      if (!s_.ok()) return s_;                                                \
+   } while (0)
+
+// Grow-only device scratch buffers, one set per (device, stream).  Each
+// call of an entry point owns the set of its stream for the call's duration;
+// concurrent calls on different streams use different sets.
+enum Slot : int {
+   WS_BOXES = 0,   // generation: fiber + bundle AABBs (ordered u64)
+   WS_FIBER_OK,    // generation: per fiber flags
+   WS_SEGS,        // generation: segment records
+   WS_SEG_COUNT,   // generation: per segment tile count
+   WS_SEG_OFF,     // generation: exclusive scan of counts
+   WS_KEYS_A,      // generation: (tile) keys
+   WS_VALS_A,      // generation: (segment) values
+   WS_KEYS_B,
+   WS_VALS_B,
+   WS_TILE_RANGE,  // generation: per tile [begin, end)
+   WS_CUB_TEMP,    // hipcub temporary storage
+   WS_SMALL,       // small per-call tables (bundles, layers, tilts, props)
+   WS_FLAGS,       // warning / error words
+   WS_NUM_SLOTS
+};
+
+Status ws_get(void *stream, int slot, size_t bytes, void **ptr);
+
+// Pinned host staging for the small device->host readbacks of a call.
+Status host_stage(void *stream, size_t bytes, void **ptr);
+
+// Optional kernel timing (fpl_profile_enable).  Usage around one launch:
+//   KernelTimer kt("march", stream);  launch;  kt.stop();
+// and after the call's final stream synchronization: kt.collect().
+struct KernelTimer {
+   const char *name;
+   hipStream_t stream;
+   hipEvent_t a = nullptr, b = nullptr;
+   bool on = false;
+   KernelTimer(const char *n, hipStream_t s);
+   void stop();
+   void collect(); // requires the stream to have passed event b
+   ~KernelTimer();
+};
+
+} // namespace fpl

@@ -1,0 +1,556 @@
+// Ray march (Stokes vector propagation through per-voxel retarders) for gfx950.
+//
+// Semantics: fastpli PliSimulator::RunSimulation (src/simulation/simulator.cpp:182-411).
+// One thread per ray (CCD pixel); all tilts of a call run in one launch
+// (blockIdx.y = tilt).
+//
+// Formulation (exact in exact arithmetic, derived from simulator.cpp:207-214,
+// 322-343, 355-360):
+//   * the per-voxel update for rotation rho is R3(2 rho) M0 R3(-2 rho), R3 a
+//     rotation of the (s1, s2) plane, and the input state (I0/2, 0, 0, -I0/2)
+//     is R3-invariant, so s(rho) = R3(2 rho) s(0): one state per ray gives all
+//     n_rho intensities I(rho) = 1/2 (s0 - cos 2rho s1 + sin 2rho s2);
+//   * attenuation is a common scalar: s = A * (I0/2) * (1, u), A = exp(-sum mu t),
+//     u a unit vector on the Poincare sphere rotated by angle -delta about the
+//     in-plane axis n = (cos 2phi, -sin 2phi, 0) at every birefringent voxel;
+//   * cos^2(alpha) = (vx^2 + vy^2) / |v|^2 and (cos 2phi, sin 2phi) =
+//     (vx^2 - vy^2, 2 vx vy) / (vx^2 + vy^2) replace asin/atan2/cos/pow.
+// Precision: u is accumulated in float-float (TwoSum) from FP32 increments
+// (Rodrigues form, 1 - cos d = 2 sin^2(d/2)), sum mu t likewise; ray positions
+// are stepped in FP64 exactly as the reference so that the visited voxels are
+// identical.
+#include <cmath>
+#include <vector>
+
+#include "fpl_internal.h"
+
+namespace fpl {
+namespace sim {
+
+constexpr int WG = 256;
+constexpr int MAX_LDS_PROPS = 1024;
+constexpr int MAX_RHO = 64;
+
+struct PropDev {
+   float K;     // pi/2 * dn * 4 * thickness / lambda  (delta per unit |v| cos^2 alpha)
+   float mu_t;  // mu * 1e3 * thickness
+   int32_t bg;  // dn == 0 or label 0 -> attenuation only
+   int32_t pad;
+};
+
+struct TiltDev {
+   double step[3];       // light_step = dir * step_size
+   double half_step[3];  // light_step * 0.5
+   double half_shift[2]; // 0.5 * shift (untilted sensor start)
+   double shift[2];      // shift (tilted sensor start)
+   double pc[3], pt[3], e1[3], e2[3], rinv[9];
+   double a, b, c, d, D;
+   float R[9];           // Mat3RotZYZ(phi, theta, -phi) for the optical axis
+   int32_t rotate;       // theta != 0
+   int32_t pad;
+};
+
+struct Params {
+   int64_t nx, ny, nz;
+   int64_t n_props;
+   int32_t n_tilts, n_rho;
+   int32_t interp;       // 0 NN, 1 Lerp, 2 Slerp
+   int32_t untilt;       // untilt_sensor_view
+   int32_t flip_z;
+   double quarter_I0;    // light_intensity / 4
+   double c2r[MAX_RHO], s2r[MAX_RHO];
+};
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ void two_sum(float &hi, float &lo, float b) {
+   const float s = hi + b;
+   const float bb = s - hi;
+   const float err = (hi - (s - bb)) + (b - bb);
+   hi = s;
+   lo += err;
+}
+
+struct Field {
+   const int32_t *__restrict__ label;
+   const float *__restrict__ vec;
+   int64_t nx, ny, nz;
+   int32_t flip_z;
+
+   // PliSimulator::GetLabel (simulator.cpp:425-443)
+   __device__ __forceinline__ int32_t label_at(int64_t x, int64_t y, int64_t z) const {
+      if (flip_z) z = nz - 1 - z;
+      if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) return 0;
+      return label[(x * ny + y) * nz + z];
+   }
+   // PliSimulator::GetVec(int...) (simulator.cpp:557-579): callers guarantee bounds
+   __device__ __forceinline__ float3 vec_at(int64_t x, int64_t y, int64_t z) const {
+      if (flip_z) z = nz - 1 - z;
+      const int64_t i = 3 * ((x * ny + y) * nz + z);
+      return make_float3(vec[i], vec[i + 1], vec[i + 2]);
+   }
+};
+
+__device__ __forceinline__ bool is_zero(const float3 &v) {
+   return v.x == 0.0f && v.y == 0.0f && v.z == 0.0f;
+}
+__device__ __forceinline__ float dot3(const float3 &a, const float3 &b) {
+   return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+
+// LinearOrientationInterpolation (simulator.cpp:449-471)
+__device__ __forceinline__ float3 lerp_orient(const float3 &v, const float3 &u, float t) {
+   if (is_zero(v)) return u;
+   if (is_zero(u)) return v;
+   const float3 a = make_float3(v.x * (1.0f - t), v.y * (1.0f - t), v.z * (1.0f - t));
+   const float3 b = make_float3(u.x * t, u.y * t, u.z * t);
+   if (dot3(a, b) >= 0.0f) return make_float3(a.x + b.x, a.y + b.y, a.z + b.z);
+   return make_float3(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+
+// sin(a theta) / sin(theta) = a 2F1(1 - a, 1 + a; 3/2; sin^2(theta/2)), theta in [0, pi/2]
+__device__ __forceinline__ float slerp_weight(float a, float z) {
+   float term = 1.0f, sum = 1.0f;
+   const float a2 = a * a;
+   for (int k = 0; k < 32; k++) {
+      const float k1 = (float)(k + 1);
+      term *= (k1 * k1 - a2) * z / ((k1 + 0.5f) * k1);
+      sum += term;
+      if (fabsf(term) < 1e-8f) break;
+   }
+   return a * sum;
+}
+
+// SphericalInterpolation (simulator.cpp:473-494) without trigonometric calls
+__device__ __forceinline__ float3 slerp_orient(const float3 &v, float3 u, float t) {
+   if (is_zero(v)) return u;
+   if (is_zero(u)) return v;
+   float d = dot3(v, u);
+   if (d < 0.0f) {
+      u = make_float3(-u.x, -u.y, -u.z);
+      d = -d;
+   }
+   const float3 df = make_float3(v.x - u.x, v.y - u.y, v.z - u.z);
+   if (dot3(df, df) < 1e-12f) return v;
+   if (t == 0.0f) return v;
+   const float c = fminf(d * rsqrtf(dot3(v, v) * dot3(u, u)), 1.0f);
+   const float z = 0.5f * (1.0f - c);
+   const float wa = slerp_weight(1.0f - t, z);
+   const float wb = slerp_weight(t, z);
+   return make_float3(v.x * wa + u.x * wb, v.y * wa + u.y * wb, v.z * wa + u.z * wb);
+}
+
+__device__ __forceinline__ int64_t clampi(int64_t v, int64_t n) {
+   return v < 0 ? 0 : (v > n - 1 ? n - 1 : v);
+}
+
+// PliSimulator::GetVec Lerp/Slerp (simulator.cpp:496-555).  Levels whose two
+// corner indices coincide are skipped: with weight 0 and equal arguments both
+// interpolations return their first argument exactly.
+__device__ float3 get_vec_interp(const Field &F, double px, double py, double pz, int32_t lab,
+                                 int slerp) {
+   const int64_t x0 = clampi((int64_t)floor(px - 0.5), F.nx);
+   const int64_t y0 = clampi((int64_t)floor(py - 0.5), F.ny);
+   const int64_t z0 = clampi((int64_t)floor(pz - 0.5), F.nz);
+   const int64_t x1 = clampi((int64_t)ceil(px - 0.5), F.nx);
+   const int64_t y1 = clampi((int64_t)ceil(py - 0.5), F.ny);
+   const int64_t z1 = clampi((int64_t)ceil(pz - 0.5), F.nz);
+   if (x0 == x1 && y0 == y1 && z0 == z1) return F.vec_at(x0, y0, z0);
+   const float xd = (x0 == x1) ? 0.0f : (float)((px - x0 - 0.5) / (double)(x1 - x0));
+   const float yd = (y0 == y1) ? 0.0f : (float)((py - y0 - 0.5) / (double)(y1 - y0));
+   const float zd = (z0 == z1) ? 0.0f : (float)((pz - z0 - 0.5) / (double)(z1 - z0));
+   auto corner = [&](int64_t x, int64_t y, int64_t z) {
+      if (F.label_at(x, y, z) != lab) return make_float3(0.0f, 0.0f, 0.0f);
+      return F.vec_at(x, y, z);
+   };
+   auto ip = [&](const float3 &a, const float3 &b, float t) {
+      return slerp ? slerp_orient(a, b, t) : lerp_orient(a, b, t);
+   };
+   const bool dx = x0 != x1, dy = y0 != y1, dz = z0 != z1;
+   // level x
+   float3 c00 = corner(x0, y0, z0);
+   if (dx) c00 = ip(c00, corner(x1, y0, z0), xd);
+   float3 c01 = c00, c10 = c00, c11 = c00;
+   if (dz) {
+      c01 = corner(x0, y0, z1);
+      if (dx) c01 = ip(c01, corner(x1, y0, z1), xd);
+   }
+   if (dy) {
+      c10 = corner(x0, y1, z0);
+      if (dx) c10 = ip(c10, corner(x1, y1, z0), xd);
+      if (dz) {
+         c11 = corner(x0, y1, z1);
+         if (dx) c11 = ip(c11, corner(x1, y1, z1), xd);
+      }
+   }
+   // level y
+   float3 c0 = c00, c1 = c01;
+   if (dy) {
+      c0 = ip(c00, c10, yd);
+      if (dz) c1 = ip(c01, c11, yd);
+   }
+   // level z
+   if (dz) return ip(c0, c1, zd);
+   return c0;
+}
+
+// CalcStartingLightPositionsTilted (simulator.cpp:659-703) for one pixel,
+// every FP64 operation rounded separately as in the reference.
+__device__ void tilted_start(const TiltDev &T, int64_t i, int64_t j, double &tx, double &ty) {
+#pragma clang fp contract(off)
+   const double e = ((double)i + 0.5 - T.pc[0]) - T.pt[0];
+   const double f = ((double)j + 0.5 - T.pc[1]) - T.pt[1];
+   const double Ds = e * T.d - f * T.b;
+   const double Dt = T.a * f - T.c * e;
+   const double s = Ds / T.D;
+   const double t = Dt / T.D;
+   double p[3];
+   for (int k = 0; k < 3; k++) p[k] = ((T.pc[k] + T.pt[k]) + T.e1[k] * s) + T.e2[k] * t;
+   const double u0 = p[0] - T.pc[0], u1 = p[1] - T.pc[1], u2 = p[2] - T.pc[2];
+   double acc0 = 0.0, acc1 = 0.0;
+   acc0 = acc0 + u0 * T.rinv[0];
+   acc0 = acc0 + u1 * T.rinv[1];
+   acc0 = acc0 + u2 * T.rinv[2];
+   acc1 = acc1 + u0 * T.rinv[3];
+   acc1 = acc1 + u1 * T.rinv[4];
+   acc1 = acc1 + u2 * T.rinv[5];
+   tx = (acc0 + T.pc[0]) - T.shift[0];
+   ty = (acc1 + T.pc[1]) - T.shift[1];
+}
+
+// ------------------------------------------------------------------ kernel
+__global__ void __launch_bounds__(WG)
+    k_march(const Params *__restrict__ prm, const TiltDev *__restrict__ tilts,
+            const PropDev *__restrict__ props, const int32_t *__restrict__ label,
+            const float *__restrict__ vec, double *__restrict__ images, uint32_t *flags) {
+   __shared__ PropDev sp[MAX_LDS_PROPS];
+   const Params &P = *prm;
+   const int np_lds = P.n_props < MAX_LDS_PROPS ? (int)P.n_props : MAX_LDS_PROPS;
+   for (int k = threadIdx.x; k < np_lds; k += WG) sp[k] = props[k];
+   __syncthreads();
+
+   const TiltDev &T = tilts[blockIdx.y];
+   const int64_t ray = (int64_t)blockIdx.x * WG + threadIdx.x;
+   const int64_t nxy = P.nx * P.ny;
+   if (ray >= nxy) return;
+   const int64_t i = ray / P.ny, j = ray % P.ny;
+   Field F{label, vec, P.nx, P.ny, P.nz, P.flip_z};
+
+   double px, py, pz = 0.0;
+   if (P.untilt) {
+      px = ((double)i + 0.5) - T.half_shift[0];
+      py = ((double)j + 0.5) - T.half_shift[1];
+   } else {
+      tilted_start(T, i, j, px, py);
+   }
+   px = px + T.half_step[0];
+   py = py + T.half_step[1];
+   pz = pz + T.half_step[2];
+   const double sx = T.step[0], sy = T.step[1], sz = T.step[2];
+   const double nzd = (double)P.nz;
+
+   // Poincare state u = (u1, u2, u3) in float-float, starts at (0, 0, -1)
+   float h1 = 0.0f, h2 = 0.0f, h3 = -1.0f, l1 = 0.0f, l2 = 0.0f, l3 = 0.0f;
+   float mh = 0.0f, ml = 0.0f; // sum of mu * t
+   uint32_t warn = 0, err = 0;
+
+   for (;;) {
+      const double fz = floor(pz);
+      if (!(fz >= 0.0 && fz < nzd)) break;
+      const int64_t ix = (int64_t)floor(px), iy = (int64_t)floor(py), iz = (int64_t)fz;
+      int32_t lab = F.label_at(ix, iy, iz);
+      if (lab < 0 || lab >= P.n_props) {
+         err = 1;
+         lab = 0;
+      }
+      const PropDev pd = lab < MAX_LDS_PROPS ? sp[lab] : props[lab];
+      if (pd.bg) {
+         if (pd.mu_t != 0.0f) two_sum(mh, ml, pd.mu_t);
+      } else {
+         float3 v = (P.interp == 0) ? F.vec_at(ix, iy, iz)
+                                    : get_vec_interp(F, px, py, pz, lab, P.interp == 2);
+         if (is_zero(v)) {
+            warn |= FPL_WARN_ZERO_VEC; // skipped without attenuation (simulator.cpp:292-297)
+         } else {
+            const float n2 = dot3(v, v);
+            if (n2 - 1.0f > 1e-5f) warn |= FPL_WARN_NOT_NORM;
+            const float dr = pd.K * sqrtf(n2);
+            if (T.rotate) {
+               const float3 w = v;
+               v.x = T.R[0] * w.x + T.R[1] * w.y + T.R[2] * w.z;
+               v.y = T.R[3] * w.x + T.R[4] * w.y + T.R[5] * w.z;
+               v.z = T.R[6] * w.x + T.R[7] * w.y + T.R[8] * w.z;
+            }
+            const float xx = v.x * v.x, yy = v.y * v.y;
+            const float rxy = xx + yy;
+            const float rr = rxy + v.z * v.z;
+            const float delta = dr * (rxy / rr);
+            float C = 1.0f, S = 0.0f;
+            if (rxy > 0.0f) {
+               const float inv = 1.0f / rxy;
+               C = (xx - yy) * inv;
+               S = 2.0f * v.x * v.y * inv;
+            }
+            float sh, ch;
+            sincosf(0.5f * delta, &sh, &ch);
+            const float sd = 2.0f * sh * ch;  // sin(delta)
+            const float omc = 2.0f * sh * sh; // 1 - cos(delta)
+            const float nv = C * h1 - S * h2;
+            const float d1 = sd * S * h3 + omc * (C * nv - h1);
+            const float d2 = sd * C * h3 + omc * (-S * nv - h2);
+            const float d3 = -sd * (C * h2 + S * h1) - omc * h3;
+            two_sum(h1, l1, d1);
+            two_sum(h2, l2, d2);
+            two_sum(h3, l3, d3);
+            if (pd.mu_t != 0.0f) two_sum(mh, ml, pd.mu_t);
+         }
+      }
+      px = px + sx;
+      py = py + sy;
+      pz = pz + sz;
+   }
+
+   const double A = exp(-((double)mh + (double)ml));
+   const double u1 = (double)h1 + (double)l1, u2 = (double)h2 + (double)l2;
+   const double base = P.quarter_I0 * A;
+   double *out = images + ((int64_t)blockIdx.y * nxy + ray) * P.n_rho;
+   for (int k = 0; k < P.n_rho; k++) out[k] = base * (1.0 - (P.c2r[k] * u1 - P.s2r[k] * u2));
+
+   const uint32_t f = warn | (err ? 0x80000000u : 0u);
+   if (f) atomicOr(&flags[0], f);
+}
+
+// min / max of the label field (CheckInput, simulator.cpp:149-167)
+__global__ void k_label_range(const int32_t *__restrict__ label, int64_t n, int32_t *mm) {
+   int32_t lo = 0x7fffffff, hi = (int32_t)0x80000000;
+   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+        i += (int64_t)gridDim.x * blockDim.x) {
+      const int32_t v = label[i];
+      lo = v < lo ? v : lo;
+      hi = v > hi ? v : hi;
+   }
+   for (int o = 32; o > 0; o >>= 1) {
+      const int32_t a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
+      lo = a < lo ? a : lo;
+      hi = b > hi ? b : hi;
+   }
+   if ((threadIdx.x & 63) == 0) {
+      atomicMin(&mm[0], lo);
+      atomicMax(&mm[1], hi);
+   }
+}
+
+__global__ void k_init_flags(uint32_t *flags, int32_t *mm) {
+   flags[0] = 0;
+   mm[0] = 0x7fffffff;
+   mm[1] = (int32_t)0x80000000;
+}
+
+// ------------------------------------------------------------------ host
+double pi_sin(double v) { // vm::rot_pi_cases::sin (vemath.hpp:499-508)
+   v = std::fmod(v, 2 * M_PI);
+   if (v == 0 || v == M_PI || v == -M_PI) return 0;
+   if (v == M_PI_2 || v == -(M_PI + M_PI_2)) return 1;
+   if (v == M_PI + M_PI_2 || v == -M_PI_2) return -1;
+   return std::sin(v);
+}
+double pi_cos(double v) { // vm::rot_pi_cases::cos (vemath.hpp:510-520)
+   v = std::fmod(v, 2 * M_PI);
+   if (v == 0) return 1;
+   if (v == M_PI_2 || v == M_PI + M_PI_2 || v == -M_PI_2 || v == -(M_PI + M_PI_2)) return 0;
+   if (v == M_PI || v == -M_PI) return -1;
+   return std::cos(v);
+}
+
+struct M3 {
+   double a[9];
+};
+M3 mat_mul(const M3 &A, const M3 &B) {
+   M3 R;
+   for (int m = 0; m < 3; m++)
+      for (int n = 0; n < 3; n++) {
+         double acc = 0.0;
+         for (int k = 0; k < 3; k++) acc += A.a[3 * m + k] * B.a[3 * k + n];
+         R.a[3 * m + n] = acc;
+      }
+   return R;
+}
+void mat_vec(const M3 &A, const double v[3], double out[3]) {
+   for (int m = 0; m < 3; m++) {
+      double acc = 0.0;
+      acc = acc + v[0] * A.a[3 * m + 0];
+      acc = acc + v[1] * A.a[3 * m + 1];
+      acc = acc + v[2] * A.a[3 * m + 2];
+      out[m] = acc;
+   }
+}
+// vm::rot_pi_cases::Mat3RotZYZ (vemath.hpp:527-544)
+M3 rot_zyz(double g, double b, double a) {
+   auto rz = [](double t) {
+      const double s = pi_sin(t), c = pi_cos(t);
+      return M3{{c, -s, 0, s, c, 0, 0, 0, 1}};
+   };
+   auto ry = [](double t) {
+      const double s = pi_sin(t), c = pi_cos(t);
+      return M3{{c, 0, s, 0, 1, 0, -s, 0, c}};
+   };
+   return mat_mul(rz(g), mat_mul(ry(b), rz(a)));
+}
+
+TiltDev make_tilt(const fpl_setup &S, const int64_t dim[3], double theta, double phi) {
+   TiltDev T;
+   memset(&T, 0, sizeof(T));
+   // LightDirectionUnitVector (simulator.cpp:585-590)
+   const double dir[3] = {pi_cos(phi) * pi_sin(theta), pi_sin(phi) * pi_sin(theta),
+                          pi_cos(theta)};
+   for (int k = 0; k < 3; k++) {
+      T.step[k] = dir[k] * S.step_size;
+      T.half_step[k] = T.step[k] * 0.5;
+   }
+   const double ct = std::cos(theta);
+   const double shift[3] = {dir[0] / ct * (double)dim[2], dir[1] / ct * (double)dim[2],
+                            dir[2] / ct * (double)dim[2]};
+   T.half_shift[0] = 0.5 * shift[0];
+   T.half_shift[1] = 0.5 * shift[1];
+   T.shift[0] = shift[0];
+   T.shift[1] = shift[1];
+   // CalcStartingLightPositionsTilted constants (simulator.cpp:628-656)
+   const double ang = std::asin(std::sin(-theta) * S.tissue_refrection);
+   const M3 rot = rot_zyz(phi, ang, -phi);
+   const M3 rinv = rot_zyz(phi, -ang, -phi);
+   const double pc[3] = {dim[0] * 0.5, dim[1] * 0.5, dim[2] * 0.5};
+   const double pt0[3] = {-pc[0], -pc[1], pc[2]};
+   const double e10[3] = {1, 0, 0}, e20[3] = {0, 1, 0};
+   mat_vec(rot, pt0, T.pt);
+   mat_vec(rot, e10, T.e1);
+   mat_vec(rot, e20, T.e2);
+   for (int k = 0; k < 3; k++) T.pc[k] = pc[k];
+   for (int k = 0; k < 9; k++) T.rinv[k] = rinv.a[k];
+   T.a = T.e1[0];
+   T.b = T.e2[0];
+   T.c = T.e1[1];
+   T.d = T.e2[1];
+   T.D = T.a * T.d - T.c * T.b;
+   const M3 R = rot_zyz(phi, theta, -phi);
+   for (int k = 0; k < 9; k++) T.R[k] = (float)R.a[k];
+   T.rotate = theta != 0;
+   return T;
+}
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, const float *vector,
+           const double *props, int64_t n_props, const double *tilts, int64_t n_tilts,
+           double *images, uint32_t *warn, hipStream_t st) {
+   if (!setup || !dim || !props || !tilts) return make_status(FPL_E_INVALID, "null argument");
+   const fpl_setup &S = *setup;
+   // setup::Setup::Check (setup.hpp:48-71)
+   if (S.step_size <= 0) return make_status(FPL_E_INVALID, "step_size <= 0: %f", S.step_size);
+   if (S.light_intensity < 0)
+      return make_status(FPL_E_INVALID, "light intensity < 0: %f", S.light_intensity);
+   if (S.voxel_size <= 0) return make_status(FPL_E_INVALID, "voxel_size <= 0: %f", S.voxel_size);
+   if (S.wavelength <= 0) return make_status(FPL_E_INVALID, "wavelength <= 0: %f", S.wavelength);
+   if (S.tissue_refrection < 1)
+      return make_status(FPL_E_INVALID, "tissue_refrection < 1: %f", S.tissue_refrection);
+   if (S.n_rho <= 0 || !S.filter_rotations)
+      return make_status(FPL_E_INVALID, "filter_rotations is empty: []");
+   if (S.n_rho > MAX_RHO)
+      return make_status(FPL_E_INVALID, "at most %d filter_rotations are supported", MAX_RHO);
+   if (S.interpolate < 0 || S.interpolate > 2)
+      return make_status(FPL_E_INVALID, "Only NN, Lerp or Slerp are supported");
+   if (dim[0] <= 0 || dim[1] <= 0 || dim[2] <= 0)
+      return make_status(FPL_E_INVALID, "global_dim[any] <= 0: [%lld,%lld,%lld]",
+                         (long long)dim[0], (long long)dim[1], (long long)dim[2]);
+   if (n_props <= 0) return make_status(FPL_E_INVALID, "properties are empty");
+   if (n_tilts <= 0 || n_tilts > 65535) return make_status(FPL_E_INVALID, "bad number of tilts");
+   if (!label || !vector || !images) return make_status(FPL_E_INVALID, "null array");
+   // CheckInput (simulator.cpp:169-175)
+   if (props[0] != 0) return make_status(FPL_E_BG_DN, "background birefringence has to be 0");
+   for (int64_t l = 0; l < n_props; l++)
+      if (props[2 * l + 1] < 0)
+         return make_status(FPL_E_INVALID, "mu < 0: %f", props[2 * l + 1]);
+
+   const double lambda = S.wavelength * 1e-9;
+   const double thickness = S.voxel_size * 1e-6 * S.step_size;
+
+   std::vector<PropDev> pdev(n_props);
+   for (int64_t l = 0; l < n_props; l++) {
+      const double dn = props[2 * l], mu = props[2 * l + 1] * 1e3;
+      PropDev &p = pdev[l];
+      p.bg = (dn == 0 || l == 0);
+      p.K = (float)(M_PI_2 * (dn * 4.0 * thickness / lambda));
+      p.mu_t = (float)(mu * thickness);
+      p.pad = 0;
+   }
+   Params prm;
+   memset(&prm, 0, sizeof(prm));
+   prm.nx = dim[0];
+   prm.ny = dim[1];
+   prm.nz = dim[2];
+   prm.n_props = n_props;
+   prm.n_tilts = (int32_t)n_tilts;
+   prm.n_rho = S.n_rho;
+   prm.interp = S.interpolate;
+   prm.untilt = S.untilt_sensor_view != 0;
+   prm.flip_z = S.flip_z != 0;
+   prm.quarter_I0 = 0.25 * S.light_intensity;
+   for (int k = 0; k < S.n_rho; k++) {
+      prm.c2r[k] = std::cos(2.0 * S.filter_rotations[k]);
+      prm.s2r[k] = std::sin(2.0 * S.filter_rotations[k]);
+   }
+   std::vector<TiltDev> tdev(n_tilts);
+   for (int64_t t = 0; t < n_tilts; t++) tdev[t] = make_tilt(S, dim, tilts[2 * t], tilts[2 * t + 1]);
+
+   const size_t sz_p = sizeof(Params);
+   const size_t off_t = (sz_p + 255) & ~size_t(255);
+   const size_t off_pr = (off_t + sizeof(TiltDev) * n_tilts + 255) & ~size_t(255);
+   const size_t bytes = off_pr + sizeof(PropDev) * n_props;
+   void *small, *stage, *flags;
+   FPL_TRY(ws_get(st, WS_SMALL, bytes, &small));
+   FPL_TRY(ws_get(st, WS_FLAGS, 64, &flags));
+   FPL_TRY(host_stage(st, bytes + 64, &stage));
+   FPL_HIP(hipStreamSynchronize(st));
+   memcpy((char *)stage, &prm, sz_p);
+   memcpy((char *)stage + off_t, tdev.data(), sizeof(TiltDev) * n_tilts);
+   memcpy((char *)stage + off_pr, pdev.data(), sizeof(PropDev) * n_props);
+   FPL_HIP(hipMemcpyAsync(small, stage, bytes, hipMemcpyHostToDevice, st));
+   uint32_t *d_flags = (uint32_t *)flags;
+   int32_t *d_mm = (int32_t *)((char *)flags + 16);
+
+   const int64_t n_vox = dim[0] * dim[1] * dim[2];
+   hipLaunchKernelGGL(k_init_flags, dim3(1), dim3(1), 0, st, d_flags, d_mm);
+   {
+      int64_t g = grid_for(n_vox, 256);
+      if (g > 4096) g = 4096;
+      hipLaunchKernelGGL(k_label_range, dim3((unsigned)g), dim3(256), 0, st, label, n_vox, d_mm);
+   }
+   const int64_t nxy = dim[0] * dim[1];
+   KernelTimer kt("march", st);
+   hipLaunchKernelGGL(k_march, dim3(grid_for(nxy, WG), (unsigned)n_tilts), dim3(WG), 0, st,
+                      (const Params *)small, (const TiltDev *)((char *)small + off_t),
+                      (const PropDev *)((char *)small + off_pr), label, vector, images, d_flags);
+   kt.stop();
+   FPL_HIP(hipGetLastError());
+   FPL_HIP(hipMemcpyAsync(stage, flags, 32, hipMemcpyDeviceToHost, st));
+   FPL_HIP(hipStreamSynchronize(st));
+   kt.collect();
+   const uint32_t f = *(uint32_t *)stage;
+   const int32_t lmin = ((int32_t *)((char *)stage + 16))[0];
+   const int32_t lmax = ((int32_t *)((char *)stage + 16))[1];
+   if (lmin < 0) return make_status(FPL_E_LABEL_NEG, "label < 0 detected");
+   if ((int64_t)lmax >= n_props) return make_status(FPL_E_LABEL_MAX, "label exceed properties.size()");
+   if (warn) *warn = f & 0x7fffffffu;
+   return Status();
+}
+
+} // namespace sim
+} // namespace fpl
+
+extern "C" int fpl_simulate(const fpl_setup *setup, const int64_t dim[3], const int32_t *label,
+                            const float *vector, const double *props, int64_t n_props,
+                            const double *tilts, int64_t n_tilts, double *images, uint32_t *warn,
+                            void *stream, char *msg, size_t msg_len) {
+   if (warn) *warn = 0;
+   fpl::Status s = fpl::sim::run(setup, dim, label, vector, props, n_props, tilts, n_tilts, images,
+                                 warn, (hipStream_t)stream);
+   return fpl::finish(s, msg, msg_len);
+}

@@ -1,0 +1,271 @@
+"""Simpli: the fastpli simulation API on MI355X.
+
+Drop-in counterpart of fastpli's ``src/fastpli/simulation/simpli.py`` for the
+simulation hot path: ``generate_tissue`` (:61-104) and ``run_simulation``
+(:125-187) keep their signatures, validation, warnings, output dtypes and
+layouts.  The work runs in libfastpli_amd.so (HIP kernels for gfx950) instead
+of the pybind11/OpenMP core; there is no CPU fallback.
+
+Extensions of this package (not in the reference):
+  * ``generate_tissue_device`` / ``run_simulation`` with torch tensors on the
+    GPU keep the fields resident in HBM (no host round trip);
+  * ``run_simulation_tilts`` runs several tilts in one device call;
+  * ``count_voxel_steps`` gives the exact Mueller voxel-step count of a run.
+"""
+
+import warnings
+
+import numpy as np
+
+from .. import _lib
+from ._base import _SimpliBase
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError('fastpli_amd needs a HIP device (torch.cuda.is_available() is False)')
+    return torch
+
+
+def _is_tensor(x):
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        return False
+    return isinstance(x, torch.Tensor)
+
+
+class Simpli(_SimpliBase):
+    """Simulation of 3D-PLI images (tissue generation + light propagation)."""
+
+    def __init__(self, mpi_comm=None):
+        super().__init__()
+        if mpi_comm is not None:
+            raise NotImplementedError(
+                'MPI domain decomposition is not supported: one GPU holds the whole volume')
+        self.mpi = None
+        self._freeze()
+
+    # ------------------------------------------------------------ properties
+    @property
+    def interpolate(self):
+        """optical axis interpolation: 'NN', 'Lerp' or 'Slerp'"""
+        return self._interpolate
+
+    @interpolate.setter
+    def interpolate(self, interpolate):
+        if interpolate not in ('NN', 'Lerp', 'Slerp'):
+            raise ValueError('Only \'NN\', \'Lerp\' or \'Slerp\' are supported')
+        self._interpolate = interpolate
+
+    @property
+    def omp_num_threads(self):
+        """kept for API compatibility; the GPU path does not use OpenMP"""
+        return self._omp_num_threads
+
+    @omp_num_threads.setter
+    def omp_num_threads(self, num_threads):
+        if not isinstance(num_threads, int):
+            raise TypeError('num_threads != int')
+        if num_threads <= 0:
+            raise TypeError('num_threads <= 0')
+        self._omp_num_threads = num_threads
+
+    # ------------------------------------------------------------ generation
+    def _dim_checked(self):
+        dim = [int(d) for d in self._dim]
+        if any(d <= 0 for d in dim):
+            raise ValueError(f'dim.global[any] <= 0: [{dim[0]},{dim[1]},{dim[2]}]')
+        return dim
+
+    def generate_tissue_device(self, only_tissue=False):
+        """Voxelize the fiber bundles into device tensors.
+
+        Returns (tissue int32 (nx, ny, nz), optical_axis float32 (nx, ny, nz, 3)
+        or None, tissue_properties float64 ndarray (n, 2)).
+        """
+        self._check_volume_input()
+        self._check_generation_input()
+        dim = self._dim_checked()
+        torch = _torch()
+        dev = torch.device('cuda', torch.cuda.current_device())
+        points, fiber_ptr, bundle_ptr, layer_ptr, table = self._fiber_bundles.to_flat()
+        points_d = torch.from_numpy(points).to(dev)
+        fiber_ptr_d = torch.from_numpy(fiber_ptr).to(dev)
+        tissue = torch.empty(dim, dtype=torch.int32, device=dev)
+        optical_axis = None if only_tissue else torch.empty(dim + [3], dtype=torch.float32,
+                                                            device=dev)
+        props, warn = _lib.generate(dim, self._dim_origin, self._voxel_size, points_d,
+                                    fiber_ptr_d, bundle_ptr, layer_ptr, table, only_tissue,
+                                    tissue, optical_axis)
+        props[0, 1] = self._background_absorption
+        _lib.raise_warnings(warn)
+        if not bool(torch.any(tissue != 0)):
+            warnings.warn(
+                'All labels are 0. Usually this means, that the VOI contains ' +
+                'no fiber_bundles or that the voxel_size is to large.', UserWarning)
+        return tissue, optical_axis, props
+
+    def generate_tissue(self, only_tissue=False):
+        """Generation of the discretized tissue.
+
+        Returns (tissue int32 (nx, ny, nz), optical_axis float32 (nx, ny, nz, 3)
+        or shape (0,) if only_tissue, tissue_properties float64 (n, 2)).
+        """
+        self._print('Generate Tissue')
+        if self._dim is not None:
+            self._print(f'Memory needed: ~{np.ceil(self.memory_usage()):.0f} MB')
+        tissue, optical_axis, props = self.generate_tissue_device(only_tissue)
+        tissue = tissue.cpu().numpy()
+        if optical_axis is None:
+            optical_axis = np.empty((0,), dtype=np.float32)
+        else:
+            optical_axis = optical_axis.cpu().numpy()
+        return tissue, optical_axis, props
+
+    # ------------------------------------------------------------ simulation
+    def _setup(self):
+        self._check_simulation_input()
+        return _lib.make_setup(self._step_size, self._light_intensity, self._voxel_size,
+                               self._wavelength, self._tissue_refrection, self._interpolate,
+                               self._untilt_sensor_view, self._flip_z_beam,
+                               self._filter_rotations)
+
+    def _prepare_fields(self, tissue, optical_axis, tissue_properties):
+        """Validation of simpli.py:154-180; returns device tensors + props."""
+        tissue_properties = np.array(tissue_properties)
+        if tissue_properties.ndim != 2:
+            raise ValueError('tissue_properties.ndim !=2')
+        if tissue_properties.shape[1] != 2:
+            raise ValueError('tissue_properties.shape[1] != 2')
+        torch = _torch()
+        dev = torch.device('cuda', torch.cuda.current_device())
+        if _is_tensor(tissue) or _is_tensor(optical_axis):
+            tissue_ = torch.as_tensor(tissue, device=dev)
+            optical_axis_ = torch.as_tensor(optical_axis, device=dev)
+            if tissue_.dtype != torch.int32:
+                warnings.warn('tissue is copied', UserWarning)
+                tissue_ = tissue_.to(torch.int32)
+            if optical_axis_.dtype != torch.float32:
+                warnings.warn('optical_axis is copied', UserWarning)
+                optical_axis_ = optical_axis_.to(torch.float32)
+            tissue_ = tissue_.contiguous()
+            optical_axis_ = optical_axis_.contiguous()
+            t_shape, o_shape = tuple(tissue_.shape), tuple(optical_axis_.shape)
+        else:
+            tissue_np = np.asarray(tissue, dtype=np.int32)
+            axis_np = np.asarray(optical_axis, dtype=np.float32)
+            if tissue_np is not tissue:
+                warnings.warn('tissue is copied', UserWarning)
+            if axis_np is not optical_axis:
+                warnings.warn('optical_axis is copied', UserWarning)
+            t_shape, o_shape = tissue_np.shape, axis_np.shape
+            tissue_ = None
+            optical_axis_ = None
+        if len(t_shape) != 3:
+            raise ValueError('tissue.ndim != 3')
+        if len(o_shape) != 4:
+            raise ValueError('optical_axis.ndim != 4')
+        if o_shape[3] != 3:
+            raise ValueError('optical_axis.shape[3] != 3')
+        if tuple(t_shape) != tuple(o_shape[:-1]):
+            raise ValueError('not np.equal(tissue.shape, optical_axis.shape[:-1])')
+        dim = self._dim_checked()
+        if int(np.prod(dim)) != int(np.prod(t_shape)):
+            raise ValueError('dim_.local.x() * dim_.local.y() * dim_.local.z() != '
+                             'label_field_.size()')
+        if tissue_ is None:
+            tissue_ = torch.from_numpy(np.ascontiguousarray(tissue_np)).to(dev)
+            optical_axis_ = torch.from_numpy(np.ascontiguousarray(axis_np)).to(dev)
+        return dim, tissue_, optical_axis_, tissue_properties
+
+    def run_simulation_tilts(self, tissue, optical_axis, tissue_properties, tilts):
+        """Simulate several tilts in one device call.
+
+        Returns images float64 (n_tilts, nx, ny, n_rho): a torch tensor on the
+        GPU when the fields are torch tensors, else a numpy array.
+        """
+        self._print('Run simulation')
+        self._check_volume_input()
+        setup, _keep = self._setup()
+        as_tensor = _is_tensor(tissue) or _is_tensor(optical_axis)
+        dim, tissue_, axis_, props = self._prepare_fields(tissue, optical_axis,
+                                                          tissue_properties)
+        torch = _torch()
+        tilts = np.array(tilts, dtype=np.float64, ndmin=2).reshape(-1, 2)
+        images = torch.empty((tilts.shape[0], dim[0], dim[1], setup.n_rho),
+                             dtype=torch.float64,
+                             device=tissue_.device)
+        warn = _lib.simulate(setup, dim, tissue_, axis_, props, tilts, images)
+        _lib.raise_warnings(warn)
+        if bool(torch.any(images < 0)):
+            raise ValueError('intensity < 0 detected')
+        if as_tensor:
+            return images
+        return images.cpu().numpy()
+
+    def run_simulation(self, tissue, optical_axis, tissue_properties, theta, phi):
+        """Simulate the PLI images of one tilt (theta, phi).
+
+        Returns images float64 (nx, ny, n_rho) with a pixel size of voxel_size.
+        """
+        images = self.run_simulation_tilts(tissue, optical_axis, tissue_properties,
+                                           [[theta, phi]])
+        return images[0]
+
+    def count_voxel_steps(self, theta=0.0, phi=0.0):
+        """Mueller voxel-steps of one tilt: n_rho x rays x z-march iterations."""
+        self._check_volume_input()
+        setup, _keep = self._setup()
+        return _lib.count_voxel_steps(setup, self._dim_checked(), theta, phi)
+
+    # ------------------------------------------------------------ helpers
+    def crop_tilt_pixel(self):
+        """number of boundary pixels affected by tilting (simpli.py:246-262)"""
+        if not self.untilt_sensor_view:
+            raise ValueError('currently only for untilt_sensor_view=True')
+        if self._voxel_size is None:
+            raise ValueError('voxel_size is not set')
+        if self._pixel_size is None:
+            raise ValueError('pixel_size is not set')
+        if self._tilts is None:
+            raise ValueError('tilts are not set')
+        return int(
+            np.ceil(
+                np.tan(np.max(np.abs(self._tilts[:, 0]))) * self._dim[-1] / 2 *
+                self._voxel_size / self._pixel_size))
+
+    def crop_tilt_voxel(self):
+        """number of boundary voxels affected by tilting"""
+        return int(np.round(self.crop_tilt_pixel() * self._pixel_size / self._voxel_size))
+
+    def add_crop_tilt_halo(self):
+        """enlarge the volume by the tilt halo"""
+        self.dim_origin[:2] -= self.crop_tilt_voxel() * self.voxel_size
+        self.dim[:2] += 2 * self.crop_tilt_voxel()
+
+    def rm_crop_tilt_halo(self, data):
+        """remove the tilt halo from images"""
+        delta_voxel = self.crop_tilt_voxel()
+        if delta_voxel == 0:
+            return data
+        return data[delta_voxel:-delta_voxel, delta_voxel:-delta_voxel, :]
+
+    def memory_usage(self, unit='MB', item='all'):
+        """expected memory usage (simpli.py:537-571)"""
+        if unit == 'MB':
+            div = 1024**2
+        elif unit == 'GB':
+            div = 1024**3
+        else:
+            raise ValueError('allowed is only \'MB\', \'GB\'')
+        if self._dim is None:
+            raise TypeError('dimension not set yet')
+        if item == 'tissue':
+            usage = np.prod(self._dim) * (32 + 32) / 8 / div
+        elif item == 'all':
+            usage = np.prod(self._dim) * (32 + 32 + 3 * 32) / 8 / div
+        else:
+            raise ValueError('allowed is only \'tissue\' or \'all\'')
+        return usage

@@ -1,0 +1,148 @@
+/*
+ * fastpli_amd -- C ABI of the MI355X (gfx950) simulation hot path.
+ *
+ * This library replaces the two pybind11 modules of fastpli that sit between
+ * the Python `Simpli` class and its OpenMP C++ core:
+ *
+ *   __generation._Generator   src/simulation/bindings/generator_module.cpp:26-139
+ *   __simulation._Simulator   src/simulation/bindings/simulator_module.cpp:21-94
+ *
+ * Conventions
+ *   - Bulk arrays (fiber points, label field, vector field, images) are DEVICE
+ *     pointers in HBM; small tables (layers, properties, tilts, rotations) are
+ *     host pointers and are copied by the call.
+ *   - Arrays are C-contiguous in the reference's layout: label[x][y][z]
+ *     (z fastest), vector[x][y][z][3], images[tilt][x][y][rho].
+ *   - `stream` is a hipStream_t (NULL = the null stream).  Every call is
+ *     stream-ordered and returns after the result is complete on that stream.
+ *   - Return value: FPL_OK (0) or a negative FPL_E* code; `msg` (may be NULL)
+ *     receives the reference's error text.  The process is never aborted
+ *     (the reference calls exit(), simulator.cpp:62-69).
+ *   - `warn` (may be NULL) receives a bit mask of FPL_WARN_* that the caller
+ *     raises as Python UserWarnings, once per kind and call.
+ */
+#ifndef FASTPLI_AMD_H_
+#define FASTPLI_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status ---- */
+#define FPL_OK 0
+#define FPL_E_INVALID (-1)   /* std::invalid_argument in the reference        */
+#define FPL_E_LABEL_NEG (-2) /* "label < 0 detected" (simulator.cpp:159-162)   */
+#define FPL_E_LABEL_MAX (-3) /* "label exceed properties.size()" (:164-167)   */
+#define FPL_E_BG_DN (-4)     /* "background birefringence has to be 0" (:169) */
+#define FPL_E_DEVICE (-5)    /* HIP runtime failure or no device              */
+#define FPL_E_NOMEM (-6)     /* device allocation failed                      */
+
+/* ----------------------------------------------------------- warnings --- */
+#define FPL_WARN_OVERLAP (1u << 0)      /* "objects overlap" generator.cpp:357      */
+#define FPL_WARN_SINGLE_POINT (1u << 1) /* "fiber object with only single point" :215 */
+#define FPL_WARN_ZERO_VEC (1u << 2)     /* "optical_axis is 0 for tissue > 0" simulator.cpp:294 */
+#define FPL_WARN_NOT_NORM (1u << 3)     /* "optical_axis not normalized" simulator.cpp:300 */
+#define FPL_WARN_NO_LIGHT (1u << 4)     /* "light_positions is empty" simulator.cpp:715 */
+
+/* -------------------------------------------------------------- types --- */
+
+/* Volume: PliGenerator::SetVolume(global_dim, origin, voxel_size),
+ * generator.cpp:45-124 (bound at generator_module.cpp:28-35). */
+typedef struct fpl_volume {
+   int64_t dim[3];    /* voxels, each > 0                                   */
+   double origin[3];  /* micro meter                                        */
+   double voxel_size; /* micro meter, > 0                                   */
+} fpl_volume;
+
+/* Fiber bundles: PliGenerator::SetFiberBundles + fiber::Bundle(...),
+ * generator_module.cpp:36-66, fiber_bundle.cpp:76-92.  All fibers of all
+ * bundles are packed into one (n_points, 4) array (x, y, z, r) in
+ * bundle -> fiber -> point order. */
+typedef struct fpl_fiber_bundles {
+   int64_t n_bundles;
+   int64_t n_fibers;
+   int64_t n_points;
+   const double *points;        /* DEVICE (n_points, 4) float64             */
+   const int64_t *fiber_ptr;    /* DEVICE (n_fibers + 1) point offsets      */
+   const int64_t *bundle_ptr;   /* HOST (n_bundles + 1) fiber offsets       */
+   const int64_t *layer_ptr;    /* HOST (n_bundles + 1) offsets into layers */
+   const double *layer_scale;   /* HOST layer scale (any order per bundle)  */
+   const double *layer_dn;      /* HOST birefringence                       */
+   const double *layer_mu;      /* HOST absorption [1/mm]                   */
+   const uint8_t *layer_model;  /* HOST 'b', 'p' or 'r'                     */
+} fpl_fiber_bundles;
+
+/* Optical setup: setup::Setup, setup.hpp:47-82 (bound at
+ * simulator_module.cpp:23-54). */
+typedef struct fpl_setup {
+   double step_size;
+   double light_intensity;
+   double voxel_size;
+   double wavelength;          /* nano meter */
+   double tissue_refrection;
+   int32_t interpolate;        /* 0 NN, 1 Lerp, 2 Slerp */
+   int32_t untilt_sensor_view; /* bool */
+   int32_t flip_z;             /* bool */
+   int32_t n_rho;
+   const double *filter_rotations; /* HOST (n_rho) radian */
+} fpl_setup;
+
+/* ---------------------------------------------------------- entry points */
+
+/* Number of rows of the property table fpl_generate writes:
+ * n_bundles * max_layer + 1 (generator.cpp:496-500). */
+int64_t fpl_num_properties(const fpl_fiber_bundles *fbs);
+
+/* Tissue generation.  Replaces _Generator.set_volume + set_fiber_bundles +
+ * run_generation (generator_module.cpp:28-66,98-134) ->
+ * PliGenerator::RunTissueGeneration (generator.cpp:165-261).
+ *   label  DEVICE int32 (dim[0], dim[1], dim[2]), fully written
+ *   vector DEVICE float (dim..., 3), fully written; ignored if only_label
+ *   props  HOST float64 (fpl_num_properties(fbs), 2) = (dn, mu) per label
+ * Labels are bit-identical to the reference's FP64 voxelizer. */
+int fpl_generate(const fpl_volume *vol, const fpl_fiber_bundles *fbs,
+                 int only_label, int32_t *label, float *vector, double *props,
+                 uint32_t *warn, void *stream, char *msg, size_t msg_len);
+
+/* Ray march of one or more tilts in one call.  Replaces
+ * _Simulator.set_pli_setup + run_simulation (simulator_module.cpp:23-93) ->
+ * PliSimulator::RunSimulation (simulator.cpp:182-411), called once per tilt
+ * by the reference.
+ *   label  DEVICE int32 (dim...), vector DEVICE float (dim..., 3)
+ *   props  HOST float64 (n_props, 2)
+ *   tilts  HOST float64 (n_tilts, 2) = (theta, phi) radian
+ *   images DEVICE float64 (n_tilts, dim[0], dim[1], n_rho)
+ * Intensities are computed in FP32 with compensated accumulation. */
+int fpl_simulate(const fpl_setup *setup, const int64_t dim[3],
+                 const int32_t *label, const float *vector,
+                 const double *props, int64_t n_props, const double *tilts,
+                 int64_t n_tilts, double *images, uint32_t *warn, void *stream,
+                 char *msg, size_t msg_len);
+
+/* Exact count of Mueller voxel-steps of one tilt: n_rho x rays x z-march
+ * iterations (simulator.cpp:259-261).  Host only. */
+int64_t fpl_count_voxel_steps(const fpl_setup *setup, const int64_t dim[3],
+                              double theta, double phi);
+
+/* Kernel timing with HIP events recorded on each kernel's launch stream.
+ * fpl_profile_enable(1) clears and starts collection, (0) stops it.
+ * fpl_profile_read(name) returns the summed milliseconds and launch count of
+ * the kernel `name` ("voxelize", "march", ...) since the last enable; returns
+ * 0 when the kernel was seen, -1 otherwise.  No counterpart in the reference
+ * (it has no profiling); used by bench.py for the roofline figure. */
+void fpl_profile_enable(int enable);
+int fpl_profile_read(const char *name, double *total_ms, int64_t *launches);
+
+/* Release the library's cached device workspace of the current device. */
+void fpl_release_workspace(void);
+
+/* Library version string. */
+const char *fpl_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FASTPLI_AMD_H_ */

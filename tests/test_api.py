@@ -1,0 +1,145 @@
+"""CPU tests of the host-side API: Simpli parameters, containers, the C ABI library."""
+
+import ctypes
+import os
+import re
+import warnings
+
+import numpy as np
+import pytest
+
+from fastpli_amd import _lib, objects
+from fastpli_amd.io import fiber_bundles as fio
+from fastpli_amd.simulation import Simpli
+from tests.conftest import GOLDEN, ROOT
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, 'include', 'fastpli_amd.h')).read()
+    declared = set(re.findall(r'\b(fpl_[a-z_]+)\s*\(', header))
+    assert declared == set(_lib.EXPORTS)
+    L = _lib.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+    assert L.fpl_version().startswith(b'fastpli_amd')
+
+
+def test_count_voxel_steps_host_only():
+    s = Simpli()
+    s.voxel_size = 0.2
+    s.dim = [600, 600, 300]
+    s.light_intensity = 26000
+    s.wavelength = 525
+    s.filter_rotations = np.linspace(0, np.pi, 9, endpoint=False)
+    assert s.count_voxel_steps(0, 0) == 9 * 600 * 600 * 300
+    assert s.count_voxel_steps(np.deg2rad(5.5), 0) == 9 * 600 * 600 * 301
+
+
+def test_frozen_and_dict():
+    s = Simpli()
+    with pytest.raises(TypeError, match='frozen'):
+        s.not_a_member = 1
+    d = s.get_dict()
+    assert 'interpolate' in d and d['interpolate'] == 'Slerp'
+    assert d['step_size'] == 1.0 and d['untilt_sensor_view'] is True
+    assert not any(k.startswith('Simpli') or k.startswith('_') for k in d)
+    with pytest.warns(UserWarning):
+        s.set_dict(d)  # None values warn (simpli_test.py:27-29)
+    with pytest.raises(ValueError):
+        s.set_dict({'_dim': 1})
+
+
+def test_setters_validate():
+    s = Simpli()
+    with pytest.raises(TypeError, match='voxel_size is not set yet'):
+        s.set_voi([0, 0, 0], [12, 12, 12])
+    with pytest.raises(ValueError):
+        s.voxel_size = 0
+    with pytest.raises(ValueError, match='Only'):
+        s.interpolate = 'Cubic'
+    with pytest.raises(TypeError):
+        s.dim = [1.5, 2, 3]
+    with pytest.raises(TypeError):
+        s.dim = [1, 2]
+    with pytest.raises(TypeError):
+        s.filter_rotations = []
+    with pytest.raises(TypeError):
+        s.tilts = 5
+    with pytest.raises(TypeError):
+        s.tilts = [[1, 2, 3]]
+    with pytest.raises(TypeError):
+        s.omp_num_threads = 1.5
+    s.voxel_size = 0.5
+    s.set_voi([-1, -2, -3], [1, 2, 3])
+    assert np.array_equal(s.dim, [4, 8, 12])
+    v_min, v_max = s.get_voi()
+    assert np.allclose(v_min, [-1, -2, -3]) and np.allclose(v_max, [1, 2, 3])
+    s.voxel_size = 0.25  # recalculates dim for the same VOI
+    assert np.array_equal(s.dim, [8, 16, 24])
+    assert s.memory_usage('MB') == pytest.approx(8 * 16 * 24 * 20 / 1024**2)
+
+
+def test_crop_tilt():
+    # test_crop_tilt (simpli_test.py:300-325)
+    s = Simpli()
+    s.voxel_size = 0.51
+    s.pixel_size = 10
+    s.set_voi([-100] * 3, [100] * 3)
+    s.tilts = np.deg2rad([(15.5, 0)])
+    dim_org = s.dim.copy()
+    s.add_crop_tilt_halo()
+    dim_crop = s.dim.copy()
+    dim_crop[:2] -= 2 * s.crop_tilt_voxel()
+    assert np.array_equal(dim_org, dim_crop)
+    image_crop = s.rm_crop_tilt_halo(np.empty(s.dim))
+    assert np.array_equal(dim_org, image_crop.shape)
+
+
+def test_generation_input_checks():
+    s = Simpli()
+    s.voxel_size = 1
+    s.dim = [4, 4, 4]
+    with pytest.raises(ValueError, match='fiber_bundles is not set'):
+        s.generate_tissue()
+    s.fiber_bundles = [[[[0, 0, 0, 1], [1, 1, 1, 1]]]]
+    with pytest.raises(ValueError, match='None'):
+        s.generate_tissue()
+
+
+def test_layers_validation():
+    with pytest.raises(ValueError, match='same radii'):
+        objects.Layers([(0.5, 0, 1, 'b'), (0.5, -0.004, 1, 'p')])
+    with pytest.raises(ValueError, match='background'):
+        objects.Layers([(1.0, 0.1, 1, 'b')])
+    with pytest.warns(UserWarning, match='sorted'):
+        lys = objects.Layers([(1.0, -0.004, 1, 'p'), (0.5, 0, 1, 'b')])
+    assert [ly.scale for ly in lys] == [0.5, 1.0]
+    with pytest.raises(TypeError):
+        objects.Layer((1.0, 0, 1))
+
+
+def test_fiber_containers_and_flat():
+    with pytest.raises(TypeError):
+        objects.Fiber([[1, 2, 3]])
+    fbs = objects.FiberBundles([[[[0, 0, 0, 1], [1, 0, 0, 1]], [[0, 1, 0, 1]]],
+                                [[[0, 0, 1, 2], [0, 0, 2, 2], [0, 0, 3, 2]]]],
+                               [[(1.0, -0.004, 1, 'p')], [(0.5, 0, 1, 'b'), (1.0, 0.004, 1, 'r')]])
+    points, fiber_ptr, bundle_ptr, layer_ptr, table = fbs.to_flat()
+    assert points.shape == (6, 4) and points.flags.c_contiguous
+    assert list(fiber_ptr) == [0, 2, 3, 6]
+    assert list(bundle_ptr) == [0, 2, 3]
+    assert list(layer_ptr) == [0, 1, 3]
+    assert list(table[3]) == [ord('p'), ord('b'), ord('r')]
+    with pytest.raises(TypeError):
+        objects.FiberBundles([[[[0, 0, 0, 1]]]], [[(1, 0, 0, 'b')], [(1, 0, 0, 'b')]])
+
+
+def test_dat_loader_round_trip(tmp_path):
+    fbs = fio.load(os.path.join(GOLDEN, 'cube.dat'))
+    assert len(fbs) == 1 and len(fbs[0]) == 100
+    assert sum(f.shape[0] for f in fbs[0]) == 1994
+    p = tmp_path / 'x.dat'
+    fio.save(str(p), fbs)
+    again = fio.load(str(p))
+    for a, b in zip(fbs[0], again[0]):
+        np.testing.assert_allclose(a.as_array(), b.as_array(), atol=1e-6)

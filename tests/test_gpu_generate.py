@@ -1,0 +1,143 @@
+"""GPU parity of generate_tissue: labels and vectors bit-exact vs the oracle."""
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from fastpli_amd import objects, synthetic
+from fastpli_amd.simulation import Simpli
+from tests import bruteforce
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _simpli(fbs, dim, origin, voxel_size):
+    s = Simpli()
+    s.fiber_bundles = fbs
+    s.voxel_size = voxel_size
+    s.dim = np.array(dim, dtype=int)
+    s.dim_origin = origin
+    return s
+
+
+def _compare(fbs, dim, origin, voxel_size, only_tissue=False):
+    s = _simpli(fbs, dim, origin, voxel_size)
+    label, vector, props = s.generate_tissue(only_tissue=only_tissue)
+    ol, ov, op, ow = oracle.generate(fbs, dim, np.asarray(origin, float), voxel_size,
+                                     only_label=only_tissue, n_threads=8)
+    assert label.dtype == np.int32 and label.shape == tuple(dim)
+    assert np.array_equal(label, ol), f'{np.count_nonzero(label != ol)} labels differ'
+    if only_tissue:
+        assert vector.size == 0
+    else:
+        assert vector.dtype == np.float32 and vector.shape == tuple(dim) + (3,)
+        assert np.array_equal(vector, ov), f'{np.count_nonzero(vector != ov)} components differ'
+    op[0, 1] = s.background_absorption
+    assert np.array_equal(props, op)
+    return label, vector, ow
+
+
+def test_generate_golden_cube(golden, cube_bundles):
+    _say('generate: golden cube.dat 24^3')
+    label, vector, _ = _compare(cube_bundles, (24, 24, 24), [-60.0] * 3, 5.0)
+    assert np.array_equal(label, golden['tissue'])
+    assert np.array_equal(vector, golden['optical_axis'])
+
+
+def test_generate_only_tissue(cube_bundles):
+    _say('generate: only_tissue')
+    _compare(cube_bundles, (24, 24, 24), [-60.0] * 3, 5.0, only_tissue=True)
+
+
+def test_generate_parallel_fibers_config():
+    _say('generate: 64 parallel fibers 120^3')
+    _compare(synthetic.parallel_fibers(), (120, 120, 120), [0.0] * 3, 0.25)
+
+
+@pytest.mark.parametrize('seed', range(8))
+def test_generate_random_scenes(seed):
+    _say(f'generate: random scene {seed}')
+    fbs = bruteforce.random_scene(seed, n_fibers=4, n_points=6)
+    _compare(fbs, (13, 11, 37), [0.0] * 3, 1.0)
+    _compare(fbs, (9, 10, 70), [-1.5, 0.5, 0.25], 0.5)
+
+
+def test_generate_ragged_and_edges():
+    _say('generate: ragged sizes, single point fiber, empty volume')
+    # single point fiber: warning; a fiber completely outside: culled
+    fbs = objects.FiberBundles(
+        [[[[1, 1, 1, 1]], [[0, 1, 2, 1.5], [5, 4, 3, 1.0], [6, 6, 6, 2.0]]],
+         [[[100, 100, 100, 1], [101, 100, 100, 1]]]],
+        [[(0.5, 0, 1, 'b'), (1.0, 0.004, 1, 'r')], [(1.0, -0.004, 2, 'p')]])
+    with pytest.warns(UserWarning, match='single point'):
+        _compare(fbs, (1, 7, 3), [0.0] * 3, 1.0)
+    with pytest.warns(UserWarning):
+        _compare(fbs, (33, 1, 65), [0.0] * 3, 0.25)
+
+
+def test_generate_overlap_warning():
+    _say('generate: overlap warning')
+    fbs = objects.FiberBundles([[[[0, 3, 3, 2], [8, 3, 3, 2]]], [[[3, 0, 3, 2], [3, 8, 3, 2]]]],
+                               [[(1.0, -0.004, 1, 'p')], [(1.0, -0.004, 1, 'p')]])
+    s = _simpli(fbs, (8, 8, 8), [0.0] * 3, 1.0)
+    with pytest.warns(UserWarning, match='objects overlap'):
+        s.generate_tissue()
+    _, _, _, ow = oracle.generate(fbs, (8, 8, 8), np.zeros(3), 1.0)
+    assert ow & oracle.WARN_OVERLAP
+
+
+def test_generate_reference_closed_forms():
+    _say('generate: reference closed-form voxel tests')
+    fbs = objects.FiberBundles([[[[0, 0.25, -500, 1], [0, 0.25, 500, 1]]]],
+                               [[(0.3, -0.004, 1, 'p'), (0.6, 0, 1, 'b'), (1.0, 0.004, 1, 'r')]])
+    s = Simpli()
+    s.fiber_bundles = fbs
+    s.voxel_size = 0.1
+    s.set_voi([0, 0, 0], [1.5, 1.5, 1.5])
+    tissue, optical_axis, props = s.generate_tissue()
+    for i in range(tissue.shape[-1]):
+        assert np.array_equal(tissue[:, :, 0], tissue[:, :, i])
+    x, y, r = 0.0, 2.5, 10.0
+    for i in range(15):
+        for j in range(15):
+            rr = (x - (i + 0.5))**2 + (y - (j + 0.5))**2
+            if rr > r**2:
+                assert tissue[i, j, 0] == 0
+            elif np.sqrt(rr) < 0.3 * r:
+                assert tissue[i, j, 0] == 1
+                np.testing.assert_array_almost_equal(optical_axis[i, j, 0], [0, 0, 1])
+            elif np.sqrt(rr) < 0.6 * r:
+                assert tissue[i, j, 0] == 2
+                assert np.array_equal(optical_axis[i, j, 0], [0, 0, 0])
+            else:
+                assert tissue[i, j, 0] == 3
+                phi = np.arctan2((j + 0.5) - y, (i + 0.5) - x)
+                np.testing.assert_array_almost_equal(optical_axis[i, j, 0],
+                                                     [np.cos(phi), np.sin(phi), 0])
+    # test_dimension (simpli_test.py:31-46)
+    s.fiber_bundles = objects.FiberBundles([[[[1, 3, 0, 2], [1, 3, 7, 2]]]], [[(1, 0, 0, 'p')]])
+    s.dim = [3, 5, 7]
+    s.dim_origin = [0, 0, 0]
+    s.voxel_size = 1.0
+    tissue, optical_axis, props = s.generate_tissue()
+    assert tissue.shape == (3, 5, 7) and optical_axis.shape == (3, 5, 7, 3)
+    assert props.shape == (2, 2)
+    assert np.array_equal(tissue, np.array(optical_axis[:, :, :, 2], dtype=int))
+
+
+def test_generate_device_tensors(cube_bundles):
+    import torch
+    _say('generate: device tensors')
+    s = _simpli(cube_bundles, (24, 24, 24), [-60.0] * 3, 5.0)
+    label, vector, props = s.generate_tissue_device()
+    assert label.is_cuda and vector.is_cuda
+    ol, ov, _, _ = oracle.generate(cube_bundles, (24, 24, 24), np.array([-60.0] * 3), 5.0)
+    assert np.array_equal(label.cpu().numpy(), ol)
+    assert torch.equal(vector.cpu(), torch.from_numpy(ov))

@@ -1,0 +1,178 @@
+"""Pins of the FP64 CPU oracle (CPU only).
+
+Golden vectors: the reference's tests/simulation/simpli_rep.h5 (labels,
+optical axes and the images of 5 tilts with Slerp).  Closed forms: the voxel
+tests of tests/simulation/simpli_test.py:65-218, uniform columns, the
+rho-rotation identity and a single tilted voxel.  Brute force: every voxel
+against every segment on tiny random scenes.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from fastpli_amd import objects
+from tests import bruteforce
+from tests.conftest import GOLDEN_ROT, GOLDEN_TILTS, rel_err
+
+
+# ----------------------------------------------------------------- golden
+def test_oracle_generation_matches_golden(golden, cube_bundles):
+    label, vector, props, warn = oracle.generate(cube_bundles, (24, 24, 24),
+                                                 np.array([-60.0] * 3), 5.0)
+    assert np.array_equal(label, golden['tissue'])
+    assert np.array_equal(vector, golden['optical_axis'])
+    assert np.array_equal(props, golden['tissue_properties'])
+    assert warn == 0
+
+
+@pytest.mark.parametrize('t', range(5))
+def test_oracle_simulation_matches_golden(golden, t):
+    theta, phi = GOLDEN_TILTS[t]
+    img, warn = oracle.simulate(golden['tissue'], golden['optical_axis'],
+                                golden['tissue_properties'], theta, phi, voxel_size=5.0,
+                                filter_rotations=GOLDEN_ROT, interpolate='Slerp', n_threads=4)
+    # the reference's own bar: h5diff --relative=0.00001
+    assert rel_err(img, golden['data'][t]).max() < 1e-5
+    assert warn == 0
+
+
+def test_golden_background_pixel_closed_form(golden):
+    # pixel (1,1) of the untilted golden image crosses two 'b' voxels (mu = 5/mm,
+    # t = 5 um) and background only: I = I0/4 * exp(-2 * 5e3 * 5e-6)
+    assert np.allclose(golden['data'][0][1, 1, :], 26000 / 4 * np.exp(-0.05), rtol=1e-12)
+    col = golden['tissue'][1, 1, :]
+    assert np.count_nonzero(col == 2) == 2 and np.count_nonzero(col) == 2
+
+
+# ------------------------------------------------------------ closed forms
+def _single_fiber(layers):
+    fbs = objects.FiberBundles([[[[0, 0.25, -500, 1], [0, 0.25, 500, 1]]]], [layers])
+    return fbs
+
+
+def test_oracle_tissue_radial():
+    fbs = _single_fiber([(1.0, 0.004, 1, 'r')])
+    label, vector, _, _ = oracle.generate(fbs, (15, 15, 15), np.zeros(3), 0.1)
+    x, y, r = 0 / 0.1, 0.25 / 0.1, 1 / 0.1
+    for i in range(15):
+        for j in range(15):
+            t = 0 if (x - (i + 0.5))**2 + (y - (j + 0.5))**2 > r**2 else 1
+            assert np.all(label[i, j, :] == t)
+            if t:
+                phi = np.arctan2((j + 0.5) - y, (i + 0.5) - x)
+                np.testing.assert_array_almost_equal(vector[i, j, 0, :],
+                                                     [np.cos(phi), np.sin(phi), 0], decimal=6)
+
+
+def test_oracle_tissue_layered():
+    fbs = _single_fiber([(0.3, -0.004, 1, 'p'), (0.6, 0, 1, 'b'), (1.0, 0.004, 1, 'r')])
+    label, vector, props, _ = oracle.generate(fbs, (15, 15, 15), np.zeros(3), 0.1)
+    x, y, r = 0.0, 2.5, 10.0
+    for i in range(15):
+        for j in range(15):
+            rr = (x - (i + 0.5))**2 + (y - (j + 0.5))**2
+            if rr > r**2:
+                assert label[i, j, 0] == 0
+            elif np.sqrt(rr) < 0.3 * r:
+                assert label[i, j, 0] == 1
+                np.testing.assert_array_almost_equal(vector[i, j, 0], [0, 0, 1], decimal=6)
+            elif np.sqrt(rr) < 0.6 * r:
+                assert label[i, j, 0] == 2
+                assert np.array_equal(vector[i, j, 0], [0, 0, 0])
+            else:
+                assert label[i, j, 0] == 3
+    assert np.array_equal(props, [[0, 0], [-0.004, 1], [0, 1], [0.004, 1]])
+
+
+@pytest.mark.parametrize('seed', range(6))
+def test_oracle_matches_bruteforce(seed):
+    fbs = bruteforce.random_scene(seed)
+    dim = (7, 6, 8)
+    label, vector, _, _ = oracle.generate(fbs, dim, np.zeros(3), 1.0)
+    bl, bv = bruteforce.voxelize(fbs, dim, 1.0)
+    assert np.array_equal(label, bl)
+    assert np.array_equal(vector, bv)
+
+
+def _uniform_volume(dim, vec, dn, mu):
+    label = np.ones(dim, dtype=np.int32)
+    vector = np.empty(tuple(dim) + (3,), dtype=np.float32)
+    vector[:] = vec
+    props = np.array([[0.0, 0.0], [dn, mu]])
+    return label, vector, props
+
+
+@pytest.mark.parametrize('incl', [0.0, 0.3, 1.1])
+def test_oracle_uniform_column_closed_form(incl):
+    phi_v = 0.7
+    v = np.array([np.cos(incl) * np.cos(phi_v), np.cos(incl) * np.sin(phi_v), np.sin(incl)])
+    v = (v / np.linalg.norm(v.astype(np.float32))).astype(np.float32)
+    dim = (3, 2, 40)
+    dn, mu, vs, wl, I0 = -0.004, 3.0, 0.5, 525.0, 26000.0
+    label, vector, props = _uniform_volume(dim, v, dn, mu)
+    rho = np.linspace(0, np.pi, 9, endpoint=False)
+    img, _ = oracle.simulate(label, vector, props, 0.0, 0.0, voxel_size=vs, wavelength=wl,
+                             light_intensity=I0, filter_rotations=rho, interpolate='NN')
+    t = vs * 1e-6
+    vd = v.astype(np.float64)
+    cos2a = 1.0 - (vd[2] / np.linalg.norm(vd))**2
+    delta = np.pi / 2 * dn * 4 * t / (wl * 1e-9) * np.linalg.norm(vd) * cos2a
+    phi = np.arctan2(vd[1], vd[0])
+    expect = I0 / 4 * np.exp(-mu * 1e3 * t * dim[2]) * (
+        1 - np.sin(dim[2] * delta) * np.sin(2 * (rho - phi)))
+    assert rel_err(img, expect[None, None, :]).max() < 1e-10
+
+
+def test_oracle_rho_rotation_identity(golden):
+    # every pixel is T (1 + a cos 2rho + b sin 2rho): three free numbers
+    rho = np.deg2rad(np.arange(0, 180, 7.5))
+    img, _ = oracle.simulate(golden['tissue'], golden['optical_axis'],
+                             golden['tissue_properties'], GOLDEN_TILTS[1][0],
+                             GOLDEN_TILTS[1][1], voxel_size=5.0, filter_rotations=rho,
+                             interpolate='Slerp', n_threads=4)
+    A = np.stack([np.ones_like(rho), np.cos(2 * rho), np.sin(2 * rho)], axis=1)
+    y = img.reshape(-1, rho.size).T
+    coef, *_ = np.linalg.lstsq(A, y, rcond=None)
+    assert np.abs(A @ coef - y).max() < 1e-9 * np.abs(y).max()
+
+
+def test_oracle_single_tilted_voxel():
+    # one voxel, tilt (theta, phi), step_size = 1/cos(theta): the ray crosses it
+    # once; closed form with the tilt rotation built by scipy
+    from scipy.spatial.transform import Rotation
+    v = np.array([np.cos(np.pi / 4), 0, np.sin(np.pi / 4)], dtype=np.float32)
+    label, vector, props = _uniform_volume((1, 1, 1), v, -0.001, 2.0)
+    rho = np.linspace(0, np.pi, 9, endpoint=False)
+    for theta, phi in [(0.0, 0.0), (np.deg2rad(5.5), 0.0), (np.deg2rad(5.5), np.pi / 2),
+                       (np.deg2rad(5.5), np.pi), (np.deg2rad(5.5), 3 * np.pi / 2)]:
+        step = 1.0 / np.cos(theta)
+        img, _ = oracle.simulate(label, vector, props, theta, phi, step_size=step,
+                                 voxel_size=60.0, filter_rotations=rho, interpolate='NN')
+        R = Rotation.from_euler('ZYZ', [phi, theta, -phi]).as_matrix()
+        w = R @ v.astype(np.float64)
+        t = 60e-6 * step
+        alpha = np.arcsin(w[2] / np.linalg.norm(w))
+        delta = np.pi / 2 * (-0.001) * 4 * t / 525e-9 * np.linalg.norm(w) * np.cos(alpha)**2
+        ph = np.arctan2(w[1], w[0])
+        expect = 26000 / 4 * np.exp(-2e3 * t) * (1 - np.sin(delta) * np.sin(2 * (rho - ph)))
+        assert rel_err(img[0, 0], expect).max() < 1e-9, (theta, phi)
+
+
+def test_oracle_count_steps():
+    # untilted: nz steps; 5.5 deg at nz=300: 301 (floor((nz - cos/2)/cos) + 1)
+    assert oracle.count_steps((2, 3, 300), 1.0, 0.0, 0.0) == 2 * 3 * 300
+    assert oracle.count_steps((2, 3, 300), 1.0, np.deg2rad(5.5), 0.0) == 2 * 3 * 301
+
+
+def test_oracle_errors():
+    label = np.zeros((2, 2, 2), np.int32)
+    vec = np.zeros((2, 2, 2, 3), np.float32)
+    props = np.array([[0.0, 0.0], [0.1, 1.0]])
+    rho = [0.0]
+    label[0, 0, 0] = -1
+    with pytest.raises(ValueError):
+        oracle.simulate(label, vec, props, 0, 0, filter_rotations=rho)
+    label[0, 0, 0] = 2
+    with pytest.raises(ValueError):
+        oracle.simulate(label, vec, props, 0, 0, filter_rotations=rho)
