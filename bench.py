@@ -1,0 +1,206 @@
+"""Benchmark: Mueller voxel-steps/s of generate_tissue + run_simulation on one GPU.
+
+Workload (default): two fiber bundles crossing at 90 deg, 2 x 2000 fibers with
+0.5 um segments, in 120 x 120 x 60 um at 0.2 um voxels (600 x 600 x 300),
+layers [(0.75, 0, 5, 'b'), (1.0, -0.004, 1, 'p')], 5 tilts {(0), (5.5 deg,
+phi = 0/90/180/270)} x 9 polarizer rotations, Slerp interpolation.
+
+One step = generate_tissue (labels + optical axes) + run_simulation of all 5
+tilts, with the fiber model already resident in HBM.  The unit of work is one
+Mueller voxel-step = (tilt, ray, z-step, rotation); a step does
+9 * (360000 * 300 + 4 * 360000 * 301) = 4.873e9 of them.
+
+Prints one JSON line.  The CPU baseline is the FP64 OpenMP oracle (a port of
+the reference's C++ core) timed on a bounded y-slab of the same workload.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+ROT = np.linspace(0, np.pi, 9, endpoint=False)
+TILTS = np.deg2rad([(0, 0), (5.5, 0), (5.5, 90), (5.5, 180), (5.5, 270)])
+VOXEL = 0.2
+VOI = (120.0, 120.0, 60.0)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_simpli(dim, fbs):
+    from fastpli_amd.simulation import Simpli
+    s = Simpli()
+    s.voxel_size = VOXEL
+    s.dim = np.array(dim, dtype=int)
+    s.dim_origin = [0.0, 0.0, 0.0]
+    s.fiber_bundles = fbs
+    s.filter_rotations = ROT
+    s.light_intensity = 26000
+    s.wavelength = 525
+    s.interpolate = 'Slerp'
+    return s
+
+
+def voxel_steps(s):
+    return sum(s.count_voxel_steps(t, p) for t, p in TILTS)
+
+
+def cpu_baseline(fbs, fraction, threads):
+    """Oracle (FP64, OpenMP) on the y-slab [0, fraction * 120 um] of the workload."""
+    import oracle
+    dim = (int(round(VOI[0] / VOXEL)), max(1, int(round(VOI[1] * fraction / VOXEL))),
+           int(round(VOI[2] / VOXEL)))
+    s = make_simpli(dim, fbs)
+    oracle.lib()
+    t0 = time.perf_counter()
+    label, vec, props, _ = oracle.generate(fbs, dim, np.zeros(3), VOXEL, n_threads=threads)
+    for theta, phi in TILTS:
+        oracle.simulate(label, vec, props, theta, phi, voxel_size=VOXEL, filter_rotations=ROT,
+                        interpolate='Slerp', n_threads=threads)
+    dt = time.perf_counter() - t0
+    w = voxel_steps(s)
+    return {
+        'value': w / dt,
+        'unit': 'Mueller voxel-steps/s',
+        'cores': threads,
+        'kind': 'port',
+        'sample': (f'y-slab {dim[0]}x{dim[1]}x{dim[2]} voxels of the same workload '
+                   f'(generate + 5 tilts x 9 rotations, Slerp), {w:.3e} voxel-steps in '
+                   f'{dt:.2f} s, FP64 OpenMP oracle'),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--cpu-fraction', type=float, default=0.1,
+                    help='y fraction of the volume timed for the CPU baseline')
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f'bench.py runs on one GPU only (--gpus {args.gpus} requested): the single-GPU '
+              'design holds the whole volume on one device and has no multi-GPU path.',
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from fastpli_amd import _lib, synthetic
+
+    assert torch.cuda.is_available(), 'no HIP device'
+    dev = torch.device('cuda', 0)
+    torch.cuda.set_device(dev)
+
+    log('building fiber model')
+    fbs = synthetic.crossing_bundles(seed=0)
+    dim = [int(round(v / VOXEL)) for v in VOI]
+    s = make_simpli(dim, fbs)
+    work = voxel_steps(s)
+    nx, ny, nz = dim
+    n_vox = nx * ny * nz
+    n_rho = ROT.size
+
+    points, fiber_ptr, bundle_ptr, layer_ptr, table = fbs.to_flat()
+    points_d = torch.from_numpy(points).to(dev)
+    fiber_ptr_d = torch.from_numpy(fiber_ptr).to(dev)
+    label = torch.empty(dim, dtype=torch.int32, device=dev)
+    vector = torch.empty(dim + [3], dtype=torch.float32, device=dev)
+    images = torch.empty((len(TILTS), nx, ny, n_rho), dtype=torch.float64, device=dev)
+    setup, _keep = s._setup()
+
+    def step():
+        props, _ = _lib.generate(dim, s.dim_origin, VOXEL, points_d, fiber_ptr_d, bundle_ptr,
+                                 layer_ptr, table, False, label, vector)
+        props[0, 1] = s.background_absorption
+        _lib.simulate(setup, dim, label, vector, props, TILTS, images)
+
+    log(f'warmup {args.warmup}')
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    log(f'timing {args.steps} steps')
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    _lib.profile_enable(False)
+    ms_step = dt / args.steps * 1e3
+
+    kernels = {}
+    alg_bytes = {
+        # generation writes int32 label + 3 x f32 vector per voxel
+        'voxelize': 16.0 * n_vox,
+        # each tilt reads label + vector per voxel once, writes n_rho f64 per pixel
+        'march': len(TILTS) * (16.0 * n_vox + 8.0 * n_rho * nx * ny),
+    }
+    for name in alg_bytes:
+        ms, n = _lib.profile_read(name)
+        if n:
+            kernels[name] = {'avg_ms': ms / n, 'launches': n,
+                             'algorithmic_bytes': alg_bytes[name],
+                             'achieved_GBs': alg_bytes[name] / (ms / n * 1e-3) / 1e9}
+    dominant = max(kernels, key=lambda k: kernels[k]['avg_ms'])
+    kd = kernels[dominant]
+    roofline = {
+        'bound': 'hbm',
+        'achieved': kd['achieved_GBs'],
+        'peak': HBM_PEAK_GBS,
+        'unit': 'GB/s',
+        'frac': kd['achieved_GBs'] / HBM_PEAK_GBS,
+        'traffic': None,
+        'kernel': dominant,
+    }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f'cpu baseline: oracle, {threads} threads, fraction {args.cpu_fraction}')
+        cpu = cpu_baseline(fbs, args.cpu_fraction, threads)
+
+    out = {
+        'metric': 'Mueller voxel-steps/s (5 tilts x 9 rot), generate_tissue + run_simulation',
+        'value': work / (dt / args.steps),
+        'unit': 'Mueller voxel-steps/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': ms_step,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic',
+        'config': {
+            'workload': 'crossing bundles 2x2000 fibers, 600x600x300 voxels @0.2um, '
+                        '5 tilts x 9 rotations, Slerp',
+            'voxels': n_vox,
+            'voxel_steps_per_step': work,
+            'tilts': len(TILTS),
+            'rotations': n_rho,
+            'interpolate': 'Slerp',
+            'parallelism': 'single GPU',
+        },
+        'roofline': roofline,
+        'kernels': kernels,
+        'cpu_baseline': cpu,
+    }
+    if cpu:
+        out['speedup_vs_cpu_baseline'] = out['value'] / cpu['value']
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

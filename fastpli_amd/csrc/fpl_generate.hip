@@ -8,10 +8,10 @@
 // than the stored one (later segment wins ties).
 //
 // GPU structure (no atomics on the volume, every voxel written exactly once):
-//   k_points      one thread per point: translate/resize (fiber.cpp:128-150),
-//                 fiber and bundle AABBs (fiber.cpp:83-99, fiber_bundle.cpp:147-154)
-//                 reduced with ordered-integer atomic min/max (min/max is exact
-//                 and order independent)
+//   k_fiber_boxes one wave per fiber: translate/resize (fiber.cpp:128-150) and
+//                 the fiber AABB (fiber.cpp:83-99) as a wave min/max reduction
+//   k_bundle_boxes one workgroup per bundle: union of its fiber boxes
+//                 (fiber_bundle.cpp:147-154); min/max is exact in any order
 //   k_fibers      one thread per fiber: bundle/fiber culling (generator.cpp:208-221)
 //   k_segments    one thread per segment: FP64 record + integer bbox
 //                 (generator.cpp:293-305) + number of 8x8x32 voxel tiles it touches
@@ -70,25 +70,6 @@ struct Params {
    int32_t ntx, nty, ntz;
 };
 
-// ordered encoding: unsigned compare == double compare (no NaN)
-__device__ __forceinline__ unsigned long long ord_enc(double d) {
-   unsigned long long b = (unsigned long long)__double_as_longlong(d);
-   return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-__device__ __forceinline__ double ord_dec(unsigned long long e) {
-   unsigned long long b = (e >> 63) ? (e & 0x7fffffffffffffffull) : ~e;
-   return __longlong_as_double((long long)b);
-}
-
-// box layout per object: lo x,y,z then hi x,y,z (6 ordered u64)
-__device__ __forceinline__ void box_unite(unsigned long long *box, const double lo[3],
-                                          const double hi[3]) {
-   for (int i = 0; i < 3; i++) {
-      atomicMin(&box[i], ord_enc(lo[i]));
-      atomicMax(&box[3 + i], ord_enc(hi[i]));
-   }
-}
-
 // aabb::AABB(p, q) unsorted constructor (aabb.hpp:54-71)
 __device__ __forceinline__ void box_of(const double p[3], const double q[3], double lo[3],
                                        double hi[3]) {
@@ -103,18 +84,31 @@ __device__ __forceinline__ void box_of(const double p[3], const double q[3], dou
    }
 }
 
-__device__ __forceinline__ int64_t find_fiber(const int64_t *fiber_ptr, int64_t n_fibers,
-                                              int64_t i) {
-   // largest f with fiber_ptr[f] <= i
-   int64_t lo = 0, hi = n_fibers; // answer in [lo, hi)
-   while (hi - lo > 1) {
-      int64_t mid = (lo + hi) >> 1;
-      if (fiber_ptr[mid] <= i)
-         lo = mid;
-      else
-         hi = mid;
+// aabb::AABB::Unite (aabb.hpp:90-97); min/max is exact and order independent,
+// so the union can be reduced in any order
+__device__ __forceinline__ void box_unite(double lo[3], double hi[3], const double a[3],
+                                          const double b[3]) {
+   for (int k = 0; k < 3; k++) {
+      if (a[k] < lo[k]) lo[k] = a[k];
+      if (b[k] > hi[k]) hi[k] = b[k];
    }
-   return lo;
+}
+
+__device__ __forceinline__ void box_empty(double lo[3], double hi[3]) {
+   for (int k = 0; k < 3; k++) {
+      lo[k] = INFINITY;
+      hi[k] = -INFINITY;
+   }
+}
+
+__device__ __forceinline__ void wave_box_reduce(double lo[3], double hi[3]) {
+   for (int off = 32; off > 0; off >>= 1) {
+      for (int k = 0; k < 3; k++) {
+         const double a = __shfl_xor(lo[k], off), b = __shfl_xor(hi[k], off);
+         if (a < lo[k]) lo[k] = a;
+         if (b > hi[k]) hi[k] = b;
+      }
+   }
 }
 
 __device__ __forceinline__ int find_bundle(const BundleDev *bd, int64_t nb, int64_t f) {
@@ -122,6 +116,20 @@ __device__ __forceinline__ int find_bundle(const BundleDev *bd, int64_t nb, int6
    while (hi - lo > 1) {
       int mid = (lo + hi) >> 1;
       if (bd[mid].fiber_begin <= f)
+         lo = mid;
+      else
+         hi = mid;
+   }
+   return lo;
+}
+
+__device__ __forceinline__ int64_t find_fiber(const int64_t *fiber_ptr, int64_t n_fibers,
+                                              int64_t i) {
+   // largest f with fiber_ptr[f] <= i
+   int64_t lo = 0, hi = n_fibers; // answer in [lo, hi)
+   while (hi - lo > 1) {
+      int64_t mid = (lo + hi) >> 1;
+      if (fiber_ptr[mid] <= i)
          lo = mid;
       else
          hi = mid;
@@ -145,62 +153,84 @@ __device__ __forceinline__ void load_point(const double *P, int64_t i, const Par
    r = r * pr.scale;
 }
 
-__global__ void k_init_boxes(unsigned long long *boxes, int64_t n_boxes, uint32_t *flags) {
-   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (i < n_boxes * 6) {
-      const int c = (int)(i % 6);
-      boxes[i] = c < 3 ? ord_enc(INFINITY) : ord_enc(-INFINITY);
-   }
-   if (i == 0) flags[0] = 0;
-}
-
-__global__ void k_points(Params pr, const double *__restrict__ P,
-                         const int64_t *__restrict__ fiber_ptr, const BundleDev *__restrict__ bd,
-                         unsigned long long *fiber_box, unsigned long long *bundle_box) {
-   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (i >= pr.n_points) return;
-   const int64_t f = find_fiber(fiber_ptr, pr.n_fibers, i);
-   const int b = find_bundle(bd, pr.n_bundles, f);
-   const int64_t end = fiber_ptr[f + 1];
-   double p[3], r;
-   load_point(P, i, pr, p, r);
-   double lo[3], hi[3];
-   if (i == fiber_ptr[f]) {
-      // first point: AABB(p - r, p + r) (fiber.cpp:90-91)
-      double a[3] = {p[0] - r, p[1] - r, p[2] - r};
-      double c[3] = {p[0] + r, p[1] + r, p[2] + r};
-      box_of(a, c, lo, hi);
-      box_unite(fiber_box + 6 * f, lo, hi);
-      box_unite(bundle_box + 6 * b, lo, hi);
-   }
-   if (i + 1 < end) {
-      double q[3], rq;
-      load_point(P, i + 1, pr, q, rq);
-      const double rm = (r < rq) ? rq : r; // std::max
-      box_of(p, q, lo, hi);
-      for (int k = 0; k < 3; k++) {
-         lo[k] -= rm;
-         hi[k] += rm;
+// Fiber::CalculateVoi (fiber.cpp:83-99) after translate/resize: one wave per
+// fiber; box layout lo x,y,z then hi x,y,z.  A fiber without points keeps the
+// default (+inf, -inf) box.
+__global__ void k_fiber_boxes(Params pr, const double *__restrict__ P,
+                              const int64_t *__restrict__ fiber_ptr, double *fiber_box) {
+   const int64_t f = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
+   if (f >= pr.n_fibers) return; // uniform per wave
+   const int64_t beg = fiber_ptr[f], end = fiber_ptr[f + 1];
+   double lo[3], hi[3], tl[3], th[3];
+   box_empty(lo, hi);
+   for (int64_t i = beg + lane; i < end; i += 64) {
+      double p[3], r;
+      load_point(P, i, pr, p, r);
+      if (i == beg) { // AABB(p0 - r0, p0 + r0) (fiber.cpp:90-91)
+         const double a[3] = {p[0] - r, p[1] - r, p[2] - r};
+         const double c[3] = {p[0] + r, p[1] + r, p[2] + r};
+         box_of(a, c, tl, th);
+         box_unite(lo, hi, tl, th);
       }
-      box_unite(fiber_box + 6 * f, lo, hi);
-      box_unite(bundle_box + 6 * b, lo, hi);
+      if (i + 1 < end) { // segment box grown by max(r_i, r_i+1) (fiber.cpp:92-98)
+         double q[3], rq;
+         load_point(P, i + 1, pr, q, rq);
+         const double rm = (r < rq) ? rq : r; // std::max
+         box_of(p, q, tl, th);
+         for (int k = 0; k < 3; k++) {
+            tl[k] -= rm;
+            th[k] += rm;
+         }
+         box_unite(lo, hi, tl, th);
+      }
+   }
+   wave_box_reduce(lo, hi);
+   if (lane < 6) fiber_box[6 * f + lane] = lane < 3 ? lo[lane] : hi[lane - 3];
+}
+
+// Bundle::CalculateVoi (fiber_bundle.cpp:147-154): union of its fiber boxes,
+// one workgroup per bundle
+__global__ void k_bundle_boxes(Params pr, const BundleDev *__restrict__ bd,
+                               const double *__restrict__ fiber_box, double *bundle_box,
+                               uint32_t *flags) {
+   __shared__ double red[6][4];
+   const int b = blockIdx.x;
+   if (b == 0 && threadIdx.x == 0) flags[0] = 0;
+   double lo[3], hi[3];
+   box_empty(lo, hi);
+   for (int64_t f = bd[b].fiber_begin + threadIdx.x; f < bd[b].fiber_end; f += blockDim.x)
+      box_unite(lo, hi, fiber_box + 6 * f, fiber_box + 6 * f + 3);
+   wave_box_reduce(lo, hi);
+   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   if (lane == 0)
+      for (int k = 0; k < 3; k++) {
+         red[k][w] = lo[k];
+         red[3 + k][w] = hi[k];
+      }
+   __syncthreads();
+   if (threadIdx.x < 6) {
+      double v = red[threadIdx.x][0];
+      for (int i = 1; i < (int)(blockDim.x >> 6); i++) {
+         const double o = red[threadIdx.x][i];
+         if (threadIdx.x < 3 ? (o < v) : (o > v)) v = o;
+      }
+      bundle_box[6 * b + threadIdx.x] = v;
    }
 }
 
-__device__ __forceinline__ bool overlaps_volume(const unsigned long long *box, const Params &pr) {
+__device__ __forceinline__ bool overlaps_volume(const double *box, const Params &pr) {
    // aabb::Overlap(volume_bb, box), strict (aabb.hpp:128-137)
    const double vhi[3] = {(double)pr.nx, (double)pr.ny, (double)pr.nz};
-   for (int k = 0; k < 3; k++) {
-      const double lo = ord_dec(box[k]), hi = ord_dec(box[3 + k]);
-      if (0.0 >= hi || vhi[k] <= lo) return false;
-   }
+   for (int k = 0; k < 3; k++)
+      if (0.0 >= box[3 + k] || vhi[k] <= box[k]) return false;
    return true;
 }
 
 __global__ void k_fibers(Params pr, const int64_t *__restrict__ fiber_ptr,
                          const BundleDev *__restrict__ bd,
-                         const unsigned long long *__restrict__ fiber_box,
-                         const unsigned long long *__restrict__ bundle_box, uint8_t *fiber_ok,
+                         const double *__restrict__ fiber_box,
+                         const double *__restrict__ bundle_box, uint8_t *fiber_ok,
                          uint32_t *flags) {
    int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
    if (f >= pr.n_fibers) return;
@@ -584,23 +614,22 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
    // ---- scratch
    const int64_t np = fbs->n_points, nf = fbs->n_fibers;
    void *boxes, *fok, *segs, *cnt, *off, *flags;
-   FPL_TRY(ws_get(st, WS_BOXES, sizeof(unsigned long long) * 6 * (nf + nb + 1), &boxes));
+   FPL_TRY(ws_get(st, WS_BOXES, sizeof(double) * 6 * (nf + nb + 1), &boxes));
    FPL_TRY(ws_get(st, WS_FIBER_OK, nf + 1, &fok));
    FPL_TRY(ws_get(st, WS_SEGS, sizeof(SegRec) * (np + 1), &segs));
    FPL_TRY(ws_get(st, WS_SEG_COUNT, sizeof(int64_t) * (np + 1), &cnt));
    FPL_TRY(ws_get(st, WS_SEG_OFF, sizeof(int64_t) * (np + 2), &off));
    FPL_TRY(ws_get(st, WS_FLAGS, 64, &flags));
-   unsigned long long *fiber_box = (unsigned long long *)boxes;
-   unsigned long long *bundle_box = fiber_box + 6 * nf;
+   double *fiber_box = (double *)boxes;
+   double *bundle_box = fiber_box + 6 * nf;
    uint32_t *d_flags = (uint32_t *)flags;
 
    const int B = 256;
-   hipLaunchKernelGGL(k_init_boxes, dim3(grid_for(6 * (nf + nb) + 1, B)), dim3(B), 0, st,
-                      fiber_box, nf + nb, d_flags);
-   if (np > 0) {
-      hipLaunchKernelGGL(k_points, dim3(grid_for(np, B)), dim3(B), 0, st, pr, fbs->points,
-                         fbs->fiber_ptr, d_bd, fiber_box, bundle_box);
-   }
+   if (nf > 0)
+      hipLaunchKernelGGL(k_fiber_boxes, dim3(grid_for(nf, B / 64)), dim3(B), 0, st, pr,
+                         fbs->points, fbs->fiber_ptr, fiber_box);
+   hipLaunchKernelGGL(k_bundle_boxes, dim3((unsigned)(nb > 0 ? nb : 1)), dim3(B), 0, st, pr,
+                      d_bd, fiber_box, bundle_box, d_flags);
    if (nf > 0) {
       hipLaunchKernelGGL(k_fibers, dim3(grid_for(nf, B)), dim3(B), 0, st, pr, fbs->fiber_ptr, d_bd,
                          fiber_box, bundle_box, (uint8_t *)fok, d_flags);

@@ -498,7 +498,14 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
       prm.s2r[k] = std::sin(2.0 * S.filter_rotations[k]);
    }
    std::vector<TiltDev> tdev(n_tilts);
-   for (int64_t t = 0; t < n_tilts; t++) tdev[t] = make_tilt(S, dim, tilts[2 * t], tilts[2 * t + 1]);
+   for (int64_t t = 0; t < n_tilts; t++) {
+      tdev[t] = make_tilt(S, dim, tilts[2 * t], tilts[2 * t + 1]);
+      // a light direction parallel to the tissue plane never leaves the volume
+      // (the reference loops forever here); reject it
+      if (tdev[t].step[2] == 0.0)
+         return make_status(FPL_E_INVALID, "tilt theta = %f gives no progress along z",
+                            tilts[2 * t]);
+   }
 
    const size_t sz_p = sizeof(Params);
    const size_t off_t = (sz_p + 255) & ~size_t(255);

@@ -122,6 +122,7 @@ def test_generate_reference_closed_forms():
                 np.testing.assert_array_almost_equal(optical_axis[i, j, 0],
                                                      [np.cos(phi), np.sin(phi), 0])
     # test_dimension (simpli_test.py:31-46)
+    s = Simpli()
     s.fiber_bundles = objects.FiberBundles([[[[1, 3, 0, 2], [1, 3, 7, 2]]]], [[(1, 0, 0, 'p')]])
     s.dim = [3, 5, 7]
     s.dim_origin = [0, 0, 0]
