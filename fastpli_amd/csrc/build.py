@@ -33,7 +33,9 @@ COMMON = [
 SOURCES = {
     'fpl_api.hip': [],
     'fpl_generate.hip': ['-ffp-contract=off'],
-    'fpl_simulate.hip': [],
+    # tolerance-based FP32 chain: hardware reciprocal / rsqrt instead of the
+    # correctly rounded division and square root sequences
+    'fpl_simulate.hip': ['-fno-hip-fp32-correctly-rounded-divide-sqrt'],
 }
 
 

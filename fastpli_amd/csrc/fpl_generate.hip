@@ -501,7 +501,9 @@ __global__ void __launch_bounds__(WG)
          vector[3 * ind + 2] = vo[2];
       }
    }
-   if (__ballot(overlap != 0) != 0 && lane == 0) atomicOr(&flags[0], FPL_WARN_OVERLAP);
+   if (__ballot(overlap != 0) != 0 && lane == 0 &&
+       !(__atomic_load_n(&flags[0], __ATOMIC_RELAXED) & FPL_WARN_OVERLAP))
+      atomicOr(&flags[0], FPL_WARN_OVERLAP);
 }
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }

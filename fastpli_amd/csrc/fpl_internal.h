@@ -72,6 +72,7 @@ enum Slot : int {
    WS_CUB_TEMP,    // hipcub temporary storage
    WS_SMALL,       // small per-call tables (bundles, layers, tilts, props)
    WS_FLAGS,       // warning / error words
+   WS_PACKED,      // simulation: 16-byte (label, axis) records in [x][z][y] order
    WS_NUM_SLOTS
 };
 

@@ -1,8 +1,8 @@
 // Ray march (Stokes vector propagation through per-voxel retarders) for gfx950.
 //
 // Semantics: fastpli PliSimulator::RunSimulation (src/simulation/simulator.cpp:182-411).
-// One thread per ray (CCD pixel); all tilts of a call run in one launch
-// (blockIdx.y = tilt).
+// One thread per ray (CCD pixel); all tilts of a call run in one launch, the
+// tilts of one ray tile in consecutive workgroups.
 //
 // Formulation (exact in exact arithmetic, derived from simulator.cpp:207-214,
 // 322-343, 355-360):
@@ -16,7 +16,7 @@
 //   * cos^2(alpha) = (vx^2 + vy^2) / |v|^2 and (cos 2phi, sin 2phi) =
 //     (vx^2 - vy^2, 2 vx vy) / (vx^2 + vy^2) replace asin/atan2/cos/pow.
 // Precision: u is accumulated in float-float (TwoSum) from FP32 increments
-// (Rodrigues form, 1 - cos d = 2 sin^2(d/2)), sum mu t likewise; ray positions
+// (Rodrigues form, 1 - cos d without cancellation), sum mu t likewise; ray positions
 // are stepped in FP64 exactly as the reference so that the visited voxels are
 // identical.
 #include <cmath>
@@ -70,25 +70,29 @@ __device__ __forceinline__ void two_sum(float &hi, float &lo, float b) {
    lo += err;
 }
 
+// The march reads a packed copy of the two fields: one 16-byte record
+// (label, axis x, y, z) per voxel in [x][z][y] order.  Adjacent lanes hold
+// adjacent rays in y, so every record load of a wave is one contiguous 1 KiB
+// access instead of 64 scattered ones.
 struct Field {
-   const int32_t *__restrict__ label;
-   const float *__restrict__ vec;
+   const int4 *__restrict__ rec;
    int64_t nx, ny, nz;
    int32_t flip_z;
 
-   // PliSimulator::GetLabel (simulator.cpp:425-443)
-   __device__ __forceinline__ int32_t label_at(int64_t x, int64_t y, int64_t z) const {
-      if (flip_z) z = nz - 1 - z;
-      if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) return 0;
-      return label[(x * ny + y) * nz + z];
+   __device__ __forceinline__ bool inside(int64_t x, int64_t y, int64_t z) const {
+      return x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
    }
-   // PliSimulator::GetVec(int...) (simulator.cpp:557-579): callers guarantee bounds
-   __device__ __forceinline__ float3 vec_at(int64_t x, int64_t y, int64_t z) const {
+   // record of an in-range voxel; flip_z mirrors z as GetLabel / GetVec do
+   // (simulator.cpp:427-428, 559-560)
+   __device__ __forceinline__ int4 at(int64_t x, int64_t y, int64_t z) const {
       if (flip_z) z = nz - 1 - z;
-      const int64_t i = 3 * ((x * ny + y) * nz + z);
-      return make_float3(vec[i], vec[i + 1], vec[i + 2]);
+      return rec[(x * nz + z) * ny + y];
    }
 };
+
+__device__ __forceinline__ float3 axis_of(const int4 &r) {
+   return make_float3(__int_as_float(r.y), __int_as_float(r.z), __int_as_float(r.w));
+}
 
 __device__ __forceinline__ bool is_zero(const float3 &v) {
    return v.x == 0.0f && v.y == 0.0f && v.z == 0.0f;
@@ -107,17 +111,44 @@ __device__ __forceinline__ float3 lerp_orient(const float3 &v, const float3 &u, 
    return make_float3(a.x - b.x, a.y - b.y, a.z - b.z);
 }
 
-// sin(a theta) / sin(theta) = a 2F1(1 - a, 1 + a; 3/2; sin^2(theta/2)), theta in [0, pi/2]
+// sin(a theta) / sin(theta) = a 2F1(1 - a, 1 + a; 3/2; z), z = sin^2(theta/2),
+// theta in [0, pi/2].  Term ratio: (k^2 - a^2) z / ((k + 1/2) k).  For
+// z <= 0.1 (theta <= 37 deg) eight terms in Horner form leave < 1e-9; beyond
+// that the series is summed until a term drops below 1e-8 (at most 27 terms).
 __device__ __forceinline__ float slerp_weight(float a, float z) {
-   float term = 1.0f, sum = 1.0f;
    const float a2 = a * a;
-   for (int k = 0; k < 32; k++) {
-      const float k1 = (float)(k + 1);
-      term *= (k1 * k1 - a2) * z / ((k1 + 0.5f) * k1);
+   if (z <= 0.1f) {
+      float acc = 1.0f;
+#pragma unroll
+      for (int k = 8; k >= 1; k--) {
+         const float r = ((float)(k * k) - a2) * (1.0f / ((k + 0.5f) * (float)k));
+         acc = fmaf(z * r, acc, 1.0f);
+      }
+      return a * acc;
+   }
+   float term = 1.0f, sum = 1.0f;
+   for (int k = 1; k <= 40; k++) {
+      const float kf = (float)k;
+      term *= (kf * kf - a2) * z / ((kf + 0.5f) * kf);
       sum += term;
       if (fabsf(term) < 1e-8f) break;
    }
    return a * sum;
+}
+
+// sin x and 1 - cos x for the per-voxel retardance; odd / even Taylor
+// polynomials for |x| <= 0.25 (truncation < 1e-11), sincosf beyond
+__device__ __forceinline__ void sin_omc(float x, float &s, float &omc) {
+   if (fabsf(x) <= 0.25f) {
+      const float x2 = x * x;
+      s = x * fmaf(x2, fmaf(x2, fmaf(x2, -1.0f / 5040.0f, 1.0f / 120.0f), -1.0f / 6.0f), 1.0f);
+      omc = x2 * fmaf(x2, fmaf(x2, fmaf(x2, -1.0f / 40320.0f, 1.0f / 720.0f), -1.0f / 24.0f), 0.5f);
+   } else {
+      float sh, ch;
+      sincosf(0.5f * x, &sh, &ch);
+      s = 2.0f * sh * ch;
+      omc = 2.0f * sh * sh; // 1 - cos x without cancellation
+   }
 }
 
 // SphericalInterpolation (simulator.cpp:473-494) without trigonometric calls
@@ -143,24 +174,28 @@ __device__ __forceinline__ int64_t clampi(int64_t v, int64_t n) {
    return v < 0 ? 0 : (v > n - 1 ? n - 1 : v);
 }
 
-// PliSimulator::GetVec Lerp/Slerp (simulator.cpp:496-555).  Levels whose two
-// corner indices coincide are skipped: with weight 0 and equal arguments both
-// interpolations return their first argument exactly.
+// PliSimulator::GetVec Lerp/Slerp (simulator.cpp:496-555).  `own` is the
+// record of the ray's (nearest) voxel, label `lab` != 0.  When all corner
+// indices coincide they are that voxel.  Levels whose two corner indices
+// coincide are skipped: with weight 0 and equal arguments both interpolations
+// return their first argument exactly.
 __device__ float3 get_vec_interp(const Field &F, double px, double py, double pz, int32_t lab,
-                                 int slerp) {
+                                 const int4 &own, int slerp) {
    const int64_t x0 = clampi((int64_t)floor(px - 0.5), F.nx);
    const int64_t y0 = clampi((int64_t)floor(py - 0.5), F.ny);
    const int64_t z0 = clampi((int64_t)floor(pz - 0.5), F.nz);
    const int64_t x1 = clampi((int64_t)ceil(px - 0.5), F.nx);
    const int64_t y1 = clampi((int64_t)ceil(py - 0.5), F.ny);
    const int64_t z1 = clampi((int64_t)ceil(pz - 0.5), F.nz);
-   if (x0 == x1 && y0 == y1 && z0 == z1) return F.vec_at(x0, y0, z0);
+   if (x0 == x1 && y0 == y1 && z0 == z1) return axis_of(own);
    const float xd = (x0 == x1) ? 0.0f : (float)((px - x0 - 0.5) / (double)(x1 - x0));
    const float yd = (y0 == y1) ? 0.0f : (float)((py - y0 - 0.5) / (double)(y1 - y0));
    const float zd = (z0 == z1) ? 0.0f : (float)((pz - z0 - 0.5) / (double)(z1 - z0));
+   // corners with another label are zeroed (simulator.cpp:533-540)
    auto corner = [&](int64_t x, int64_t y, int64_t z) {
-      if (F.label_at(x, y, z) != lab) return make_float3(0.0f, 0.0f, 0.0f);
-      return F.vec_at(x, y, z);
+      const int4 r = F.at(x, y, z);
+      if (r.x != lab) return make_float3(0.0f, 0.0f, 0.0f);
+      return axis_of(r);
    };
    auto ip = [&](const float3 &a, const float3 &b, float t) {
       return slerp ? slerp_orient(a, b, t) : lerp_orient(a, b, t);
@@ -218,22 +253,31 @@ __device__ void tilted_start(const TiltDev &T, int64_t i, int64_t j, double &tx,
 }
 
 // ------------------------------------------------------------------ kernel
+constexpr int RAYS_X = 4;  // ray tile of one workgroup: 4 (x) x 64 (y), one wave per x row
+constexpr int RAYS_Y = 64;
+
 __global__ void __launch_bounds__(WG)
     k_march(const Params *__restrict__ prm, const TiltDev *__restrict__ tilts,
-            const PropDev *__restrict__ props, const int32_t *__restrict__ label,
-            const float *__restrict__ vec, double *__restrict__ images, uint32_t *flags) {
+            const PropDev *__restrict__ props, const int4 *__restrict__ rec,
+            double *__restrict__ images, uint32_t *flags) {
    __shared__ PropDev sp[MAX_LDS_PROPS];
    const Params &P = *prm;
    const int np_lds = P.n_props < MAX_LDS_PROPS ? (int)P.n_props : MAX_LDS_PROPS;
    for (int k = threadIdx.x; k < np_lds; k += WG) sp[k] = props[k];
    __syncthreads();
 
-   const TiltDev &T = tilts[blockIdx.y];
-   const int64_t ray = (int64_t)blockIdx.x * WG + threadIdx.x;
+   // all tilts of one ray tile are consecutive workgroups: they march through
+   // overlapping columns at the same time, so re-reads hit L2 / MALL
+   const int tilt = (int)(blockIdx.x % (unsigned)P.n_tilts);
+   const int64_t tile = blockIdx.x / (unsigned)P.n_tilts;
+   const int64_t nty = (P.ny + RAYS_Y - 1) / RAYS_Y;
+   const int64_t i = (tile / nty) * RAYS_X + (threadIdx.x >> 6);
+   const int64_t j = (tile % nty) * RAYS_Y + (threadIdx.x & 63);
+   if (i >= P.nx || j >= P.ny) return;
+   const TiltDev &T = tilts[tilt];
    const int64_t nxy = P.nx * P.ny;
-   if (ray >= nxy) return;
-   const int64_t i = ray / P.ny, j = ray % P.ny;
-   Field F{label, vec, P.nx, P.ny, P.nz, P.flip_z};
+   const int64_t ray = i * P.ny + j;
+   Field F{rec, P.nx, P.ny, P.nz, P.flip_z};
 
    double px, py, pz = 0.0;
    if (P.untilt) {
@@ -253,11 +297,22 @@ __global__ void __launch_bounds__(WG)
    float mh = 0.0f, ml = 0.0f; // sum of mu * t
    uint32_t warn = 0, err = 0;
 
-   for (;;) {
-      const double fz = floor(pz);
-      if (!(fz >= 0.0 && fz < nzd)) break;
-      const int64_t ix = (int64_t)floor(px), iy = (int64_t)floor(py), iz = (int64_t)fz;
-      int32_t lab = F.label_at(ix, iy, iz);
+   // software pipelined: the record of step k+1 is requested before step k is
+   // processed; the loop condition is the reference's (simulator.cpp:259-261)
+   double fz = floor(pz);
+   bool live = fz >= 0.0 && fz < nzd;
+   int64_t ix = (int64_t)floor(px), iy = (int64_t)floor(py), iz = (int64_t)fz;
+   int4 r = make_int4(0, 0, 0, 0); // outside of the tissue: background (simulator.cpp:430-432)
+   if (live && F.inside(ix, iy, iz)) r = F.at(ix, iy, iz);
+   while (live) {
+      const double qx = px + sx, qy = py + sy, qz = pz + sz;
+      const double qfz = floor(qz);
+      const bool qlive = qfz >= 0.0 && qfz < nzd;
+      const int64_t jx = (int64_t)floor(qx), jy = (int64_t)floor(qy), jz = (int64_t)qfz;
+      int4 rn = make_int4(0, 0, 0, 0);
+      if (qlive && F.inside(jx, jy, jz)) rn = F.at(jx, jy, jz);
+
+      int32_t lab = r.x;
       if (lab < 0 || lab >= P.n_props) {
          err = 1;
          lab = 0;
@@ -266,8 +321,8 @@ __global__ void __launch_bounds__(WG)
       if (pd.bg) {
          if (pd.mu_t != 0.0f) two_sum(mh, ml, pd.mu_t);
       } else {
-         float3 v = (P.interp == 0) ? F.vec_at(ix, iy, iz)
-                                    : get_vec_interp(F, px, py, pz, lab, P.interp == 2);
+         float3 v = (P.interp == 0) ? axis_of(r)
+                                    : get_vec_interp(F, px, py, pz, lab, r, P.interp == 2);
          if (is_zero(v)) {
             warn |= FPL_WARN_ZERO_VEC; // skipped without attenuation (simulator.cpp:292-297)
          } else {
@@ -290,10 +345,8 @@ __global__ void __launch_bounds__(WG)
                C = (xx - yy) * inv;
                S = 2.0f * v.x * v.y * inv;
             }
-            float sh, ch;
-            sincosf(0.5f * delta, &sh, &ch);
-            const float sd = 2.0f * sh * ch;  // sin(delta)
-            const float omc = 2.0f * sh * sh; // 1 - cos(delta)
+            float sd, omc; // sin(delta), 1 - cos(delta)
+            sin_omc(delta, sd, omc);
             const float nv = C * h1 - S * h2;
             const float d1 = sd * S * h3 + omc * (C * nv - h1);
             const float d2 = sd * C * h3 + omc * (-S * nv - h2);
@@ -304,38 +357,62 @@ __global__ void __launch_bounds__(WG)
             if (pd.mu_t != 0.0f) two_sum(mh, ml, pd.mu_t);
          }
       }
-      px = px + sx;
-      py = py + sy;
-      pz = pz + sz;
+      px = qx;
+      py = qy;
+      pz = qz;
+      live = qlive;
+      r = rn;
    }
 
    const double A = exp(-((double)mh + (double)ml));
    const double u1 = (double)h1 + (double)l1, u2 = (double)h2 + (double)l2;
    const double base = P.quarter_I0 * A;
-   double *out = images + ((int64_t)blockIdx.y * nxy + ray) * P.n_rho;
+   double *out = images + ((int64_t)tilt * nxy + ray) * P.n_rho;
    for (int k = 0; k < P.n_rho; k++) out[k] = base * (1.0 - (P.c2r[k] * u1 - P.s2r[k] * u2));
 
    const uint32_t f = warn | (err ? 0x80000000u : 0u);
    if (f) atomicOr(&flags[0], f);
 }
 
-// min / max of the label field (CheckInput, simulator.cpp:149-167)
-__global__ void k_label_range(const int32_t *__restrict__ label, int64_t n, int32_t *mm) {
+// Packs label[x][y][z] + vector[x][y][z][3] into rec[x][z][y] (16-byte records)
+// through a 32 x 32 (y, z) LDS tile, and reduces the label min / max of
+// CheckInput (simulator.cpp:149-167) on the way.
+__global__ void __launch_bounds__(256)
+    k_pack(const int32_t *__restrict__ label, const float *__restrict__ vec, int64_t nx,
+           int64_t ny, int64_t nz, int4 *__restrict__ rec, int32_t *mm) {
+   __shared__ int4 tile[32][33];
+   const int64_t nyb = (ny + 31) / 32;
+   const int64_t x = blockIdx.x / nyb;
+   const int64_t y0 = (blockIdx.x % nyb) * 32, z0 = (int64_t)blockIdx.y * 32;
+   const int t = threadIdx.x;
    int32_t lo = 0x7fffffff, hi = (int32_t)0x80000000;
-   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-        i += (int64_t)gridDim.x * blockDim.x) {
-      const int32_t v = label[i];
-      lo = v < lo ? v : lo;
-      hi = v > hi ? v : hi;
+   for (int r = 0; r < 4; r++) {
+      const int yy = (t >> 5) + 8 * r, zz = t & 31;
+      const int64_t y = y0 + yy, z = z0 + zz;
+      if (y < ny && z < nz) {
+         const int64_t i = (x * ny + y) * nz + z;
+         const int32_t l = label[i];
+         lo = l < lo ? l : lo;
+         hi = l > hi ? l : hi;
+         tile[zz][yy] = make_int4(l, __float_as_int(vec[3 * i]), __float_as_int(vec[3 * i + 1]),
+                                  __float_as_int(vec[3 * i + 2]));
+      }
+   }
+   __syncthreads();
+   for (int r = 0; r < 4; r++) {
+      const int yy = t & 31, zz = (t >> 5) + 8 * r;
+      const int64_t y = y0 + yy, z = z0 + zz;
+      if (y < ny && z < nz) rec[(x * nz + z) * ny + y] = tile[zz][yy];
    }
    for (int o = 32; o > 0; o >>= 1) {
       const int32_t a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
       lo = a < lo ? a : lo;
       hi = b > hi ? b : hi;
    }
-   if ((threadIdx.x & 63) == 0) {
-      atomicMin(&mm[0], lo);
-      atomicMax(&mm[1], hi);
+   // only waves that improve the running min / max touch the shared words
+   if ((t & 63) == 0) {
+      if (lo < __atomic_load_n(&mm[0], __ATOMIC_RELAXED)) atomicMin(&mm[0], lo);
+      if (hi > __atomic_load_n(&mm[1], __ATOMIC_RELAXED)) atomicMax(&mm[1], hi);
    }
 }
 
@@ -524,22 +601,31 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
    int32_t *d_mm = (int32_t *)((char *)flags + 16);
 
    const int64_t n_vox = dim[0] * dim[1] * dim[2];
+   void *packed;
+   FPL_TRY(ws_get(st, WS_PACKED, sizeof(int4) * (size_t)n_vox, &packed));
    hipLaunchKernelGGL(k_init_flags, dim3(1), dim3(1), 0, st, d_flags, d_mm);
    {
-      int64_t g = grid_for(n_vox, 256);
-      if (g > 4096) g = 4096;
-      hipLaunchKernelGGL(k_label_range, dim3((unsigned)g), dim3(256), 0, st, label, n_vox, d_mm);
+      const int64_t nyb = (dim[1] + 31) / 32, nzb = (dim[2] + 31) / 32;
+      if (dim[0] * nyb >= ((int64_t)1 << 31) || nzb > 65535)
+         return make_status(FPL_E_INVALID, "volume too large");
    }
-   const int64_t nxy = dim[0] * dim[1];
+   KernelTimer kp("pack", st);
+   hipLaunchKernelGGL(k_pack, dim3((unsigned)(dim[0] * ((dim[1] + 31) / 32)), (unsigned)((dim[2] + 31) / 32)),
+                      dim3(256), 0, st, label, vector, dim[0], dim[1], dim[2], (int4 *)packed, d_mm);
+   kp.stop();
+   const int64_t n_tiles = ((dim[0] + RAYS_X - 1) / RAYS_X) * ((dim[1] + RAYS_Y - 1) / RAYS_Y);
+   if (n_tiles * n_tilts >= ((int64_t)1 << 31)) return make_status(FPL_E_INVALID, "too many rays");
    KernelTimer kt("march", st);
-   hipLaunchKernelGGL(k_march, dim3(grid_for(nxy, WG), (unsigned)n_tilts), dim3(WG), 0, st,
+   hipLaunchKernelGGL(k_march, dim3((unsigned)(n_tiles * n_tilts)), dim3(WG), 0, st,
                       (const Params *)small, (const TiltDev *)((char *)small + off_t),
-                      (const PropDev *)((char *)small + off_pr), label, vector, images, d_flags);
+                      (const PropDev *)((char *)small + off_pr), (const int4 *)packed, images,
+                      d_flags);
    kt.stop();
    FPL_HIP(hipGetLastError());
    FPL_HIP(hipMemcpyAsync(stage, flags, 32, hipMemcpyDeviceToHost, st));
    FPL_HIP(hipStreamSynchronize(st));
    kt.collect();
+   kp.collect();
    const uint32_t f = *(uint32_t *)stage;
    const int32_t lmin = ((int32_t *)((char *)stage + 16))[0];
    const int32_t lmax = ((int32_t *)((char *)stage + 16))[1];
