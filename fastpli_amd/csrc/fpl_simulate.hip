@@ -74,21 +74,27 @@ __device__ __forceinline__ void two_sum(float &hi, float &lo, float b) {
 // (label, axis x, y, z) per voxel in [x][z][y] order.  Adjacent lanes hold
 // adjacent rays in y, so every record load of a wave is one contiguous 1 KiB
 // access instead of 64 scattered ones.
-struct Field {
+template <typename Idx> struct Field {
    const int4 *__restrict__ rec;
-   int64_t nx, ny, nz;
+   int32_t nx, ny, nz;
    int32_t flip_z;
 
-   __device__ __forceinline__ bool inside(int64_t x, int64_t y, int64_t z) const {
+   __device__ __forceinline__ bool inside(int32_t x, int32_t y, int32_t z) const {
       return x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
    }
    // record of an in-range voxel; flip_z mirrors z as GetLabel / GetVec do
    // (simulator.cpp:427-428, 559-560)
-   __device__ __forceinline__ int4 at(int64_t x, int64_t y, int64_t z) const {
+   __device__ __forceinline__ int4 at(int32_t x, int32_t y, int32_t z) const {
       if (flip_z) z = nz - 1 - z;
-      return rec[(x * nz + z) * ny + y];
+      return rec[((Idx)x * (Idx)nz + (Idx)z) * (Idx)ny + (Idx)y];
    }
 };
+
+// floor of a ray coordinate as int32; values far outside the volume saturate
+// (they only ever select "outside -> background")
+__device__ __forceinline__ int32_t ifloor(double v) {
+   return (int32_t)fmin(fmax(floor(v), -1073741824.0), 1073741824.0);
+}
 
 __device__ __forceinline__ float3 axis_of(const int4 &r) {
    return make_float3(__int_as_float(r.y), __int_as_float(r.z), __int_as_float(r.w));
@@ -113,10 +119,20 @@ __device__ __forceinline__ float3 lerp_orient(const float3 &v, const float3 &u, 
 
 // sin(a theta) / sin(theta) = a 2F1(1 - a, 1 + a; 3/2; z), z = sin^2(theta/2),
 // theta in [0, pi/2].  Term ratio: (k^2 - a^2) z / ((k + 1/2) k).  For
-// z <= 0.1 (theta <= 37 deg) eight terms in Horner form leave < 1e-9; beyond
+// z <= 1e-3 three terms, for z <= 0.1 (theta <= 37 deg) eight terms in Horner
+// form leave < 1e-9; beyond
 // that the series is summed until a term drops below 1e-8 (at most 27 terms).
 __device__ __forceinline__ float slerp_weight(float a, float z) {
    const float a2 = a * a;
+   if (z <= 1e-3f) { // truncation < 1e-12
+      float acc = 1.0f;
+#pragma unroll
+      for (int k = 3; k >= 1; k--) {
+         const float r = ((float)(k * k) - a2) * (1.0f / ((k + 0.5f) * (float)k));
+         acc = fmaf(z * r, acc, 1.0f);
+      }
+      return a * acc;
+   }
    if (z <= 0.1f) {
       float acc = 1.0f;
 #pragma unroll
@@ -170,29 +186,31 @@ __device__ __forceinline__ float3 slerp_orient(const float3 &v, float3 u, float 
    return make_float3(v.x * wa + u.x * wb, v.y * wa + u.y * wb, v.z * wa + u.z * wb);
 }
 
-__device__ __forceinline__ int64_t clampi(int64_t v, int64_t n) {
+__device__ __forceinline__ int32_t clampi(int32_t v, int32_t n) {
    return v < 0 ? 0 : (v > n - 1 ? n - 1 : v);
 }
 
 // PliSimulator::GetVec Lerp/Slerp (simulator.cpp:496-555).  `own` is the
-// record of the ray's (nearest) voxel, label `lab` != 0.  When all corner
-// indices coincide they are that voxel.  Levels whose two corner indices
-// coincide are skipped: with weight 0 and equal arguments both interpolations
-// return their first argument exactly.
-__device__ float3 get_vec_interp(const Field &F, double px, double py, double pz, int32_t lab,
-                                 const int4 &own, int slerp) {
-   const int64_t x0 = clampi((int64_t)floor(px - 0.5), F.nx);
-   const int64_t y0 = clampi((int64_t)floor(py - 0.5), F.ny);
-   const int64_t z0 = clampi((int64_t)floor(pz - 0.5), F.nz);
-   const int64_t x1 = clampi((int64_t)ceil(px - 0.5), F.nx);
-   const int64_t y1 = clampi((int64_t)ceil(py - 0.5), F.ny);
-   const int64_t z1 = clampi((int64_t)ceil(pz - 0.5), F.nz);
+// record of the ray's (nearest) voxel, label `lab` != 0, so the position lies
+// inside the volume.  When all corner indices coincide they are that voxel.
+// x1 - x0 is 1 whenever the indices differ, so the weight
+// (x - x0 - 0.5) / (x1 - x0) is (x - x0) - 0.5 exactly.  Levels whose two
+// corner indices coincide are skipped: with weight 0 and equal arguments both
+// interpolations return their first argument exactly.
+template <typename Idx>
+__device__ float3 get_vec_interp(const Field<Idx> &F, double px, double py, double pz,
+                                 int32_t lab, const int4 &own, int slerp) {
+   const double fx = px - 0.5, fy = py - 0.5, fz = pz - 0.5;
+   const int32_t x0 = clampi((int32_t)floor(fx), F.nx), x1 = clampi((int32_t)ceil(fx), F.nx);
+   const int32_t y0 = clampi((int32_t)floor(fy), F.ny), y1 = clampi((int32_t)ceil(fy), F.ny);
+   const int32_t z0 = clampi((int32_t)floor(fz), F.nz), z1 = clampi((int32_t)ceil(fz), F.nz);
    if (x0 == x1 && y0 == y1 && z0 == z1) return axis_of(own);
-   const float xd = (x0 == x1) ? 0.0f : (float)((px - x0 - 0.5) / (double)(x1 - x0));
-   const float yd = (y0 == y1) ? 0.0f : (float)((py - y0 - 0.5) / (double)(y1 - y0));
-   const float zd = (z0 == z1) ? 0.0f : (float)((pz - z0 - 0.5) / (double)(z1 - z0));
+   const bool dx = x0 != x1, dy = y0 != y1, dz = z0 != z1;
+   const float xd = dx ? (float)((px - (double)x0) - 0.5) : 0.0f;
+   const float yd = dy ? (float)((py - (double)y0) - 0.5) : 0.0f;
+   const float zd = dz ? (float)((pz - (double)z0) - 0.5) : 0.0f;
    // corners with another label are zeroed (simulator.cpp:533-540)
-   auto corner = [&](int64_t x, int64_t y, int64_t z) {
+   auto corner = [&](int32_t x, int32_t y, int32_t z) {
       const int4 r = F.at(x, y, z);
       if (r.x != lab) return make_float3(0.0f, 0.0f, 0.0f);
       return axis_of(r);
@@ -200,7 +218,6 @@ __device__ float3 get_vec_interp(const Field &F, double px, double py, double pz
    auto ip = [&](const float3 &a, const float3 &b, float t) {
       return slerp ? slerp_orient(a, b, t) : lerp_orient(a, b, t);
    };
-   const bool dx = x0 != x1, dy = y0 != y1, dz = z0 != z1;
    // level x
    float3 c00 = corner(x0, y0, z0);
    if (dx) c00 = ip(c00, corner(x1, y0, z0), xd);
@@ -256,6 +273,7 @@ __device__ void tilted_start(const TiltDev &T, int64_t i, int64_t j, double &tx,
 constexpr int RAYS_X = 4;  // ray tile of one workgroup: 4 (x) x 64 (y), one wave per x row
 constexpr int RAYS_Y = 64;
 
+template <typename Idx>
 __global__ void __launch_bounds__(WG)
     k_march(const Params *__restrict__ prm, const TiltDev *__restrict__ tilts,
             const PropDev *__restrict__ props, const int4 *__restrict__ rec,
@@ -277,7 +295,7 @@ __global__ void __launch_bounds__(WG)
    const TiltDev &T = tilts[tilt];
    const int64_t nxy = P.nx * P.ny;
    const int64_t ray = i * P.ny + j;
-   Field F{rec, P.nx, P.ny, P.nz, P.flip_z};
+   const Field<Idx> F{rec, (int32_t)P.nx, (int32_t)P.ny, (int32_t)P.nz, P.flip_z};
 
    double px, py, pz = 0.0;
    if (P.untilt) {
@@ -301,14 +319,14 @@ __global__ void __launch_bounds__(WG)
    // processed; the loop condition is the reference's (simulator.cpp:259-261)
    double fz = floor(pz);
    bool live = fz >= 0.0 && fz < nzd;
-   int64_t ix = (int64_t)floor(px), iy = (int64_t)floor(py), iz = (int64_t)fz;
+   int32_t ix = ifloor(px), iy = ifloor(py), iz = ifloor(fz);
    int4 r = make_int4(0, 0, 0, 0); // outside of the tissue: background (simulator.cpp:430-432)
    if (live && F.inside(ix, iy, iz)) r = F.at(ix, iy, iz);
    while (live) {
       const double qx = px + sx, qy = py + sy, qz = pz + sz;
       const double qfz = floor(qz);
       const bool qlive = qfz >= 0.0 && qfz < nzd;
-      const int64_t jx = (int64_t)floor(qx), jy = (int64_t)floor(qy), jz = (int64_t)qfz;
+      const int32_t jx = ifloor(qx), jy = ifloor(qy), jz = ifloor(qfz);
       int4 rn = make_int4(0, 0, 0, 0);
       if (qlive && F.inside(jx, jy, jz)) rn = F.at(jx, jy, jz);
 
@@ -537,6 +555,8 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
    if (dim[0] <= 0 || dim[1] <= 0 || dim[2] <= 0)
       return make_status(FPL_E_INVALID, "global_dim[any] <= 0: [%lld,%lld,%lld]",
                          (long long)dim[0], (long long)dim[1], (long long)dim[2]);
+   if (dim[0] >= (1 << 30) || dim[1] >= (1 << 30) || dim[2] >= (1 << 30))
+      return make_status(FPL_E_INVALID, "volume dimension too large");
    if (n_props <= 0) return make_status(FPL_E_INVALID, "properties are empty");
    if (n_tilts <= 0 || n_tilts > 65535) return make_status(FPL_E_INVALID, "bad number of tilts");
    if (!label || !vector || !images) return make_status(FPL_E_INVALID, "null array");
@@ -616,10 +636,15 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
    const int64_t n_tiles = ((dim[0] + RAYS_X - 1) / RAYS_X) * ((dim[1] + RAYS_Y - 1) / RAYS_Y);
    if (n_tiles * n_tilts >= ((int64_t)1 << 31)) return make_status(FPL_E_INVALID, "too many rays");
    KernelTimer kt("march", st);
-   hipLaunchKernelGGL(k_march, dim3((unsigned)(n_tiles * n_tilts)), dim3(WG), 0, st,
-                      (const Params *)small, (const TiltDev *)((char *)small + off_t),
-                      (const PropDev *)((char *)small + off_pr), (const int4 *)packed, images,
-                      d_flags);
+   const Params *d_p = (const Params *)small;
+   const TiltDev *d_t = (const TiltDev *)((char *)small + off_t);
+   const PropDev *d_pr = (const PropDev *)((char *)small + off_pr);
+   if (n_vox < ((int64_t)1 << 32))
+      hipLaunchKernelGGL(k_march<uint32_t>, dim3((unsigned)(n_tiles * n_tilts)), dim3(WG), 0, st,
+                         d_p, d_t, d_pr, (const int4 *)packed, images, d_flags);
+   else
+      hipLaunchKernelGGL(k_march<int64_t>, dim3((unsigned)(n_tiles * n_tilts)), dim3(WG), 0, st,
+                         d_p, d_t, d_pr, (const int4 *)packed, images, d_flags);
    kt.stop();
    FPL_HIP(hipGetLastError());
    FPL_HIP(hipMemcpyAsync(stage, flags, 32, hipMemcpyDeviceToHost, st));
