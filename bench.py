@@ -87,6 +87,8 @@ def main():
     ap.add_argument('--cpu-fraction', type=float, default=0.1,
                     help='y fraction of the volume timed for the CPU baseline')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--field-storage', default='f32', choices=['f32', 'f16', 'angle16'],
+                    help='storage of the fields inside the march (precision modes)')
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -106,6 +108,7 @@ def main():
     fbs = synthetic.crossing_bundles(seed=0)
     dim = [int(round(v / VOXEL)) for v in VOI]
     s = make_simpli(dim, fbs)
+    s.field_storage = args.field_storage
     work = voxel_steps(s)
     nx, ny, nz = dim
     n_vox = nx * ny * nz
@@ -146,13 +149,17 @@ def main():
         # each tilt reads label + vector per voxel once, writes n_rho f64 per pixel
         'march': len(TILTS) * (16.0 * n_vox + 8.0 * n_rho * nx * ny),
     }
-    for name in alg_bytes:
+    for name in ('voxelize', 'march', 'pack'):
         ms, n = _lib.profile_read(name)
+        if name == 'pack':
+            if n:
+                kernels[name] = {'avg_ms': ms / n, 'launches': n}
+            continue
         if n:
             kernels[name] = {'avg_ms': ms / n, 'launches': n,
                              'algorithmic_bytes': alg_bytes[name],
                              'achieved_GBs': alg_bytes[name] / (ms / n * 1e-3) / 1e9}
-    dominant = max(kernels, key=lambda k: kernels[k]['avg_ms'])
+    dominant = max(alg_bytes, key=lambda k: kernels[k]['avg_ms'])
     kd = kernels[dominant]
     roofline = {
         'bound': 'hbm',
@@ -170,6 +177,10 @@ def main():
         log(f'cpu baseline: oracle, {threads} threads, fraction {args.cpu_fraction}')
         cpu = cpu_baseline(fbs, args.cpu_fraction, threads)
 
+    if args.field_storage != 'f32':
+        # the march reads the packed storage: 8 (f16) or 6 (angle16) B/voxel
+        kernels['march']['storage_bytes'] = len(TILTS) * (
+            {'f16': 8.0, 'angle16': 6.0}[args.field_storage] * n_vox + 8.0 * n_rho * nx * ny)
     out = {
         'metric': 'Mueller voxel-steps/s (5 tilts x 9 rot), generate_tissue + run_simulation',
         'value': work / (dt / args.steps),
@@ -192,6 +203,7 @@ def main():
             'rotations': n_rho,
             'interpolate': 'Slerp',
             'parallelism': 'single GPU',
+            'field_storage': args.field_storage,
         },
         'roofline': roofline,
         'kernels': kernels,

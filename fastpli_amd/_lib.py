@@ -37,6 +37,7 @@ WARN_MESSAGES = {
 }
 
 INTERP = {'NN': 0, 'Lerp': 1, 'Slerp': 2}
+STORAGE = {'f32': 0, 'f16': 1, 'angle16': 2}
 
 _c_i64 = ctypes.c_int64
 _c_p = ctypes.c_void_p
@@ -59,14 +60,16 @@ class FplSetup(ctypes.Structure):
                 ('voxel_size', ctypes.c_double), ('wavelength', ctypes.c_double),
                 ('tissue_refrection', ctypes.c_double), ('interpolate', ctypes.c_int32),
                 ('untilt_sensor_view', ctypes.c_int32), ('flip_z', ctypes.c_int32),
-                ('n_rho', ctypes.c_int32), ('filter_rotations', _c_p)]
+                ('n_rho', ctypes.c_int32), ('filter_rotations', _c_p),
+                ('field_storage', ctypes.c_int32)]
 
 
 _LIB = None
 
 # every symbol include/fastpli_amd.h declares
-EXPORTS = ('fpl_num_properties', 'fpl_generate', 'fpl_simulate', 'fpl_count_voxel_steps',
-           'fpl_profile_enable', 'fpl_profile_read', 'fpl_release_workspace', 'fpl_version')
+EXPORTS = ('fpl_num_properties', 'fpl_generate', 'fpl_simulate', 'fpl_optic_resample', 'fpl_epa',
+           'fpl_count_voxel_steps', 'fpl_profile_enable', 'fpl_profile_read',
+           'fpl_release_workspace', 'fpl_version')
 
 
 def lib():
@@ -88,6 +91,15 @@ def lib():
         L.fpl_simulate.argtypes = [
             ctypes.POINTER(FplSetup), _c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_i64, _c_p,
             ctypes.POINTER(ctypes.c_uint32), _c_p, ctypes.c_char_p, ctypes.c_size_t
+        ]
+        L.fpl_optic_resample.restype = ctypes.c_int
+        L.fpl_optic_resample.argtypes = [
+            _c_p, _c_i64, _c_i64, _c_i64, ctypes.c_double, ctypes.c_double, _c_p, _c_p,
+            ctypes.c_char_p, ctypes.c_size_t
+        ]
+        L.fpl_epa.restype = ctypes.c_int
+        L.fpl_epa.argtypes = [
+            _c_p, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, ctypes.c_char_p, ctypes.c_size_t
         ]
         L.fpl_count_voxel_steps.restype = _c_i64
         L.fpl_count_voxel_steps.argtypes = [
@@ -138,12 +150,13 @@ def _stream():
 
 
 def make_setup(step_size, light_intensity, voxel_size, wavelength, tissue_refrection,
-               interpolate, untilt_sensor_view, flip_z, filter_rotations):
+               interpolate, untilt_sensor_view, flip_z, filter_rotations, field_storage='f32'):
     """Returns (FplSetup, keep-alive rotations array)."""
     rho = np.ascontiguousarray(filter_rotations, dtype=np.float64).ravel()
     s = FplSetup(float(step_size), float(light_intensity), float(voxel_size), float(wavelength),
                  float(tissue_refrection), INTERP[interpolate], int(bool(untilt_sensor_view)),
-                 int(bool(flip_z)), int(rho.size), rho.ctypes.data if rho.size else None)
+                 int(bool(flip_z)), int(rho.size), rho.ctypes.data if rho.size else None,
+                 STORAGE[field_storage])
     return s, rho
 
 
@@ -204,3 +217,21 @@ def profile_read(name):
     n = ctypes.c_int64(0)
     lib().fpl_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(n))
     return ms.value, n.value
+
+
+def optic_resample(images, sigma, scale, out):
+    """fpl_optic_resample on float64 device tensors (nx, ny, n_rho) -> out."""
+    nx, ny, nr = images.shape
+    msg = ctypes.create_string_buffer(512)
+    rc = lib().fpl_optic_resample(_ptr(images), nx, ny, nr, float(sigma), float(scale),
+                                  _ptr(out), _stream(), msg, 512)
+    check(rc, msg)
+
+
+def epa(images, t, d, r):
+    """fpl_epa on a float64 device tensor (..., n_rho) -> t, d, r (...)."""
+    nr = images.shape[-1]
+    n_pix = images.numel() // nr
+    msg = ctypes.create_string_buffer(512)
+    rc = lib().fpl_epa(_ptr(images), n_pix, nr, _ptr(t), _ptr(d), _ptr(r), _stream(), msg, 512)
+    check(rc, msg)

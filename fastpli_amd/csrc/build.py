@@ -33,6 +33,7 @@ COMMON = [
 SOURCES = {
     'fpl_api.hip': [],
     'fpl_generate.hip': ['-ffp-contract=off'],
+    'fpl_analysis.hip': ['-ffp-contract=off'],
     # tolerance-based FP32 chain: hardware reciprocal / rsqrt instead of the
     # correctly rounded division and square root sequences
     'fpl_simulate.hip': ['-fno-hip-fp32-correctly-rounded-divide-sqrt'],

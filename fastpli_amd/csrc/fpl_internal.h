@@ -72,7 +72,9 @@ enum Slot : int {
    WS_CUB_TEMP,    // hipcub temporary storage
    WS_SMALL,       // small per-call tables (bundles, layers, tilts, props)
    WS_FLAGS,       // warning / error words
-   WS_PACKED,      // simulation: 16-byte (label, axis) records in [x][z][y] order
+   WS_PACKED,      // simulation: packed (label, axis) records in [x][z][y] order
+   WS_PACKED_B,    // simulation: second array of the angle storage
+   WS_ANALYSIS,    // optic: two image-sized temporaries
    WS_NUM_SLOTS
 };
 

@@ -22,6 +22,8 @@
 #include <cmath>
 #include <vector>
 
+#include <hip/hip_fp16.h>
+
 #include "fpl_internal.h"
 
 namespace fpl {
@@ -70,12 +72,32 @@ __device__ __forceinline__ void two_sum(float &hi, float &lo, float b) {
    lo += err;
 }
 
-// The march reads a packed copy of the two fields: one 16-byte record
-// (label, axis x, y, z) per voxel in [x][z][y] order.  Adjacent lanes hold
-// adjacent rays in y, so every record load of a wave is one contiguous 1 KiB
-// access instead of 64 scattered ones.
-template <typename Idx> struct Field {
-   const int4 *__restrict__ rec;
+// The march reads a packed copy of the two fields in [x][z][y] order, so that
+// adjacent lanes (adjacent rays in y) read adjacent records: every load of a
+// wave is one contiguous access instead of 64 scattered ones.  Three storages:
+//   ST_F32      16-byte record: int32 label, float axis x, y, z
+//   ST_F16       8-byte record: uint16 label, fp16 axis x, y, z; the axis is
+//                canonicalised to z >= 0 (the retardance and the interpolation
+//                are invariant under v -> -v) and the sign bit of z carries the
+//                reference's "not normalized" test of the source axis; axes that
+//                pass it are renormalised after decoding
+//   ST_ANGLE16   6 bytes per voxel in two arrays: uint16 label (bit 15 = "not
+//                normalized") and the axis as 16-bit polar / azimuth codes on the
+//                upper hemisphere, polar code 0xffff = zero axis
+enum Storage { ST_F32 = 0, ST_F16 = 1, ST_ANGLE16 = 2 };
+
+struct Vox {
+   int32_t lab;
+   float3 v;
+   bool nn; // source axis failed |v|^2 - 1 <= 1e-5 (precision storages only)
+};
+
+constexpr double ANG_THETA = 1.5707963267948966 / 65534.0; // polar step
+constexpr double ANG_PHI = 6.283185307179586 / 65536.0;    // azimuth step
+
+template <typename Idx, int ST> struct Field {
+   const void *a; // ST_F32: int4[], ST_F16: uint2[], ST_ANGLE16: uint16 labels
+   const uint32_t *b; // ST_ANGLE16: packed angle codes
    int32_t nx, ny, nz;
    int32_t flip_z;
 
@@ -84,9 +106,42 @@ template <typename Idx> struct Field {
    }
    // record of an in-range voxel; flip_z mirrors z as GetLabel / GetVec do
    // (simulator.cpp:427-428, 559-560)
-   __device__ __forceinline__ int4 at(int32_t x, int32_t y, int32_t z) const {
+   __device__ __forceinline__ Vox at(int32_t x, int32_t y, int32_t z) const {
       if (flip_z) z = nz - 1 - z;
-      return rec[((Idx)x * (Idx)nz + (Idx)z) * (Idx)ny + (Idx)y];
+      const Idx i = ((Idx)x * (Idx)nz + (Idx)z) * (Idx)ny + (Idx)y;
+      Vox o;
+      if (ST == ST_F32) {
+         const int4 r = ((const int4 *)a)[i];
+         o.lab = r.x;
+         o.v = make_float3(__int_as_float(r.y), __int_as_float(r.z), __int_as_float(r.w));
+         o.nn = false;
+      } else if (ST == ST_F16) {
+         const uint2 r = ((const uint2 *)a)[i];
+         o.lab = (int32_t)(r.x & 0xffffu);
+         const uint32_t hz = r.y >> 16;
+         o.nn = (hz & 0x8000u) != 0;
+         o.v = make_float3(__half2float(__ushort_as_half((unsigned short)(r.x >> 16))),
+                           __half2float(__ushort_as_half((unsigned short)(r.y & 0xffffu))),
+                           __half2float(__ushort_as_half((unsigned short)(hz & 0x7fffu))));
+         if (!o.nn && !(o.v.x == 0.0f && o.v.y == 0.0f && o.v.z == 0.0f)) {
+            const float inv = rsqrtf(o.v.x * o.v.x + o.v.y * o.v.y + o.v.z * o.v.z);
+            o.v = make_float3(o.v.x * inv, o.v.y * inv, o.v.z * inv);
+         }
+      } else {
+         const uint32_t l = ((const uint16_t *)a)[i];
+         o.lab = (int32_t)(l & 0x7fffu);
+         o.nn = (l & 0x8000u) != 0;
+         const uint32_t w = b[i];
+         const uint32_t tc = w & 0xffffu, pc = w >> 16;
+         if (tc == 0xffffu) {
+            o.v = make_float3(0.0f, 0.0f, 0.0f);
+         } else {
+            const float th = (float)(tc * ANG_THETA), ph = (float)(pc * ANG_PHI) - 3.14159265f;
+            const float st = __sinf(th);
+            o.v = make_float3(st * __cosf(ph), st * __sinf(ph), __cosf(th));
+         }
+      }
+      return o;
    }
 };
 
@@ -94,10 +149,6 @@ template <typename Idx> struct Field {
 // (they only ever select "outside -> background")
 __device__ __forceinline__ int32_t ifloor(double v) {
    return (int32_t)fmin(fmax(floor(v), -1073741824.0), 1073741824.0);
-}
-
-__device__ __forceinline__ float3 axis_of(const int4 &r) {
-   return make_float3(__int_as_float(r.y), __int_as_float(r.z), __int_as_float(r.w));
 }
 
 __device__ __forceinline__ bool is_zero(const float3 &v) {
@@ -197,23 +248,23 @@ __device__ __forceinline__ int32_t clampi(int32_t v, int32_t n) {
 // (x - x0 - 0.5) / (x1 - x0) is (x - x0) - 0.5 exactly.  Levels whose two
 // corner indices coincide are skipped: with weight 0 and equal arguments both
 // interpolations return their first argument exactly.
-template <typename Idx>
-__device__ float3 get_vec_interp(const Field<Idx> &F, double px, double py, double pz,
-                                 int32_t lab, const int4 &own, int slerp) {
+template <typename Idx, int ST>
+__device__ float3 get_vec_interp(const Field<Idx, ST> &F, double px, double py, double pz,
+                                 int32_t lab, const float3 &own, int slerp) {
    const double fx = px - 0.5, fy = py - 0.5, fz = pz - 0.5;
    const int32_t x0 = clampi((int32_t)floor(fx), F.nx), x1 = clampi((int32_t)ceil(fx), F.nx);
    const int32_t y0 = clampi((int32_t)floor(fy), F.ny), y1 = clampi((int32_t)ceil(fy), F.ny);
    const int32_t z0 = clampi((int32_t)floor(fz), F.nz), z1 = clampi((int32_t)ceil(fz), F.nz);
-   if (x0 == x1 && y0 == y1 && z0 == z1) return axis_of(own);
+   if (x0 == x1 && y0 == y1 && z0 == z1) return own;
    const bool dx = x0 != x1, dy = y0 != y1, dz = z0 != z1;
    const float xd = dx ? (float)((px - (double)x0) - 0.5) : 0.0f;
    const float yd = dy ? (float)((py - (double)y0) - 0.5) : 0.0f;
    const float zd = dz ? (float)((pz - (double)z0) - 0.5) : 0.0f;
    // corners with another label are zeroed (simulator.cpp:533-540)
    auto corner = [&](int32_t x, int32_t y, int32_t z) {
-      const int4 r = F.at(x, y, z);
-      if (r.x != lab) return make_float3(0.0f, 0.0f, 0.0f);
-      return axis_of(r);
+      const Vox r = F.at(x, y, z);
+      if (r.lab != lab) return make_float3(0.0f, 0.0f, 0.0f);
+      return r.v;
    };
    auto ip = [&](const float3 &a, const float3 &b, float t) {
       return slerp ? slerp_orient(a, b, t) : lerp_orient(a, b, t);
@@ -273,11 +324,11 @@ __device__ void tilted_start(const TiltDev &T, int64_t i, int64_t j, double &tx,
 constexpr int RAYS_X = 4;  // ray tile of one workgroup: 4 (x) x 64 (y), one wave per x row
 constexpr int RAYS_Y = 64;
 
-template <typename Idx>
+template <typename Idx, int ST>
 __global__ void __launch_bounds__(WG)
     k_march(const Params *__restrict__ prm, const TiltDev *__restrict__ tilts,
-            const PropDev *__restrict__ props, const int4 *__restrict__ rec,
-            double *__restrict__ images, uint32_t *flags) {
+            const PropDev *__restrict__ props, const void *__restrict__ rec_a,
+            const uint32_t *__restrict__ rec_b, double *__restrict__ images, uint32_t *flags) {
    __shared__ PropDev sp[MAX_LDS_PROPS];
    const Params &P = *prm;
    const int np_lds = P.n_props < MAX_LDS_PROPS ? (int)P.n_props : MAX_LDS_PROPS;
@@ -295,7 +346,7 @@ __global__ void __launch_bounds__(WG)
    const TiltDev &T = tilts[tilt];
    const int64_t nxy = P.nx * P.ny;
    const int64_t ray = i * P.ny + j;
-   const Field<Idx> F{rec, (int32_t)P.nx, (int32_t)P.ny, (int32_t)P.nz, P.flip_z};
+   const Field<Idx, ST> F{rec_a, rec_b, (int32_t)P.nx, (int32_t)P.ny, (int32_t)P.nz, P.flip_z};
 
    double px, py, pz = 0.0;
    if (P.untilt) {
@@ -320,17 +371,17 @@ __global__ void __launch_bounds__(WG)
    double fz = floor(pz);
    bool live = fz >= 0.0 && fz < nzd;
    int32_t ix = ifloor(px), iy = ifloor(py), iz = ifloor(fz);
-   int4 r = make_int4(0, 0, 0, 0); // outside of the tissue: background (simulator.cpp:430-432)
+   Vox r{0, make_float3(0.0f, 0.0f, 0.0f), false}; // outside: background (simulator.cpp:430-432)
    if (live && F.inside(ix, iy, iz)) r = F.at(ix, iy, iz);
    while (live) {
       const double qx = px + sx, qy = py + sy, qz = pz + sz;
       const double qfz = floor(qz);
       const bool qlive = qfz >= 0.0 && qfz < nzd;
       const int32_t jx = ifloor(qx), jy = ifloor(qy), jz = ifloor(qfz);
-      int4 rn = make_int4(0, 0, 0, 0);
+      Vox rn{0, make_float3(0.0f, 0.0f, 0.0f), false};
       if (qlive && F.inside(jx, jy, jz)) rn = F.at(jx, jy, jz);
 
-      int32_t lab = r.x;
+      int32_t lab = r.lab;
       if (lab < 0 || lab >= P.n_props) {
          err = 1;
          lab = 0;
@@ -339,13 +390,13 @@ __global__ void __launch_bounds__(WG)
       if (pd.bg) {
          if (pd.mu_t != 0.0f) two_sum(mh, ml, pd.mu_t);
       } else {
-         float3 v = (P.interp == 0) ? axis_of(r)
-                                    : get_vec_interp(F, px, py, pz, lab, r, P.interp == 2);
+         float3 v = (P.interp == 0) ? r.v
+                                    : get_vec_interp(F, px, py, pz, lab, r.v, P.interp == 2);
          if (is_zero(v)) {
             warn |= FPL_WARN_ZERO_VEC; // skipped without attenuation (simulator.cpp:292-297)
          } else {
             const float n2 = dot3(v, v);
-            if (n2 - 1.0f > 1e-5f) warn |= FPL_WARN_NOT_NORM;
+            if (ST == ST_F32 ? (n2 - 1.0f > 1e-5f) : r.nn) warn |= FPL_WARN_NOT_NORM;
             const float dr = pd.K * sqrtf(n2);
             if (T.rotate) {
                const float3 w = v;
@@ -392,12 +443,50 @@ __global__ void __launch_bounds__(WG)
    if (f) atomicOr(&flags[0], f);
 }
 
-// Packs label[x][y][z] + vector[x][y][z][3] into rec[x][z][y] (16-byte records)
-// through a 32 x 32 (y, z) LDS tile, and reduces the label min / max of
-// CheckInput (simulator.cpp:149-167) on the way.
+// Packs label[x][y][z] + vector[x][y][z][3] into the march's [x][z][y]
+// storage through a 32 x 32 (y, z) LDS tile, and reduces the label min / max
+// of CheckInput (simulator.cpp:149-167) on the way.
+__device__ __forceinline__ void encode(int ST, int32_t l, float x, float y, float z, int4 &out) {
+   if (ST == ST_F32) {
+      out = make_int4(l, __float_as_int(x), __float_as_int(y), __float_as_int(z));
+      return;
+   }
+   // "optical_axis not normalized" test of the reference, in double
+   // (simulator.cpp:298): length2 of the float components minus one > 1e-5
+   const double n2 = (double)x * x + (double)y * y + (double)z * z;
+   const bool nn = n2 - 1 > 1e-5;
+   if (z < 0.0f) { // canonical orientation: z >= 0
+      x = -x;
+      y = -y;
+      z = -z;
+   }
+   z = fabsf(z);
+   if (ST == ST_F16) {
+      const uint32_t hx = __half_as_ushort(__float2half_rn(x));
+      const uint32_t hy = __half_as_ushort(__float2half_rn(y));
+      const uint32_t hz = __half_as_ushort(__float2half_rn(z)) | (nn ? 0x8000u : 0u);
+      out = make_int4((int)(((uint32_t)l & 0xffffu) | (hx << 16)), (int)(hy | (hz << 16)), 0, 0);
+      return;
+   }
+   uint32_t code;
+   if (x == 0.0f && y == 0.0f && z == 0.0f) {
+      code = 0xffffu;
+   } else {
+      const double th = atan2(sqrt((double)x * x + (double)y * y), (double)z); // [0, pi/2]
+      const double ph = atan2((double)y, (double)x);                          // [-pi, pi]
+      uint32_t tc = (uint32_t)rint(th / ANG_THETA);
+      tc = tc > 65534u ? 65534u : tc;
+      const uint32_t pcode = ((uint32_t)rint((ph + 3.141592653589793) / ANG_PHI)) & 0xffffu;
+      code = tc | (pcode << 16);
+   }
+   out = make_int4((int)(((uint32_t)l & 0x7fffu) | (nn ? 0x8000u : 0u)), (int)code, 0, 0);
+}
+
+template <int ST>
 __global__ void __launch_bounds__(256)
     k_pack(const int32_t *__restrict__ label, const float *__restrict__ vec, int64_t nx,
-           int64_t ny, int64_t nz, int4 *__restrict__ rec, int32_t *mm) {
+           int64_t ny, int64_t nz, void *__restrict__ rec_a, uint32_t *__restrict__ rec_b,
+           int32_t *mm) {
    __shared__ int4 tile[32][33];
    const int64_t nyb = (ny + 31) / 32;
    const int64_t x = blockIdx.x / nyb;
@@ -412,15 +501,25 @@ __global__ void __launch_bounds__(256)
          const int32_t l = label[i];
          lo = l < lo ? l : lo;
          hi = l > hi ? l : hi;
-         tile[zz][yy] = make_int4(l, __float_as_int(vec[3 * i]), __float_as_int(vec[3 * i + 1]),
-                                  __float_as_int(vec[3 * i + 2]));
+         encode(ST, l, vec[3 * i], vec[3 * i + 1], vec[3 * i + 2], tile[zz][yy]);
       }
    }
    __syncthreads();
    for (int r = 0; r < 4; r++) {
       const int yy = t & 31, zz = (t >> 5) + 8 * r;
       const int64_t y = y0 + yy, z = z0 + zz;
-      if (y < ny && z < nz) rec[(x * nz + z) * ny + y] = tile[zz][yy];
+      if (y < ny && z < nz) {
+         const int64_t o = (x * nz + z) * ny + y;
+         const int4 e = tile[zz][yy];
+         if (ST == ST_F32) {
+            ((int4 *)rec_a)[o] = e;
+         } else if (ST == ST_F16) {
+            ((uint2 *)rec_a)[o] = make_uint2((uint32_t)e.x, (uint32_t)e.y);
+         } else {
+            ((uint16_t *)rec_a)[o] = (uint16_t)e.x;
+            rec_b[o] = (uint32_t)e.y;
+         }
+      }
    }
    for (int o = 32; o > 0; o >>= 1) {
       const int32_t a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
@@ -552,6 +651,8 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
       return make_status(FPL_E_INVALID, "at most %d filter_rotations are supported", MAX_RHO);
    if (S.interpolate < 0 || S.interpolate > 2)
       return make_status(FPL_E_INVALID, "Only NN, Lerp or Slerp are supported");
+   if (S.field_storage < 0 || S.field_storage > 2)
+      return make_status(FPL_E_INVALID, "field_storage must be 0 (f32), 1 (f16) or 2 (angle16)");
    if (dim[0] <= 0 || dim[1] <= 0 || dim[2] <= 0)
       return make_status(FPL_E_INVALID, "global_dim[any] <= 0: [%lld,%lld,%lld]",
                          (long long)dim[0], (long long)dim[1], (long long)dim[2]);
@@ -621,8 +722,11 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
    int32_t *d_mm = (int32_t *)((char *)flags + 16);
 
    const int64_t n_vox = dim[0] * dim[1] * dim[2];
-   void *packed;
-   FPL_TRY(ws_get(st, WS_PACKED, sizeof(int4) * (size_t)n_vox, &packed));
+   const int storage = S.field_storage;
+   const size_t rec_bytes = storage == ST_F32 ? 16 : (storage == ST_F16 ? 8 : 2);
+   void *packed, *packed_b = nullptr;
+   FPL_TRY(ws_get(st, WS_PACKED, rec_bytes * (size_t)n_vox, &packed));
+   if (storage == ST_ANGLE16) FPL_TRY(ws_get(st, WS_PACKED_B, 4 * (size_t)n_vox, &packed_b));
    hipLaunchKernelGGL(k_init_flags, dim3(1), dim3(1), 0, st, d_flags, d_mm);
    {
       const int64_t nyb = (dim[1] + 31) / 32, nzb = (dim[2] + 31) / 32;
@@ -630,8 +734,18 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
          return make_status(FPL_E_INVALID, "volume too large");
    }
    KernelTimer kp("pack", st);
-   hipLaunchKernelGGL(k_pack, dim3((unsigned)(dim[0] * ((dim[1] + 31) / 32)), (unsigned)((dim[2] + 31) / 32)),
-                      dim3(256), 0, st, label, vector, dim[0], dim[1], dim[2], (int4 *)packed, d_mm);
+   {
+      const dim3 grid((unsigned)(dim[0] * ((dim[1] + 31) / 32)), (unsigned)((dim[2] + 31) / 32));
+      if (storage == ST_F32)
+         hipLaunchKernelGGL(k_pack<ST_F32>, grid, dim3(256), 0, st, label, vector, dim[0], dim[1],
+                            dim[2], packed, (uint32_t *)packed_b, d_mm);
+      else if (storage == ST_F16)
+         hipLaunchKernelGGL(k_pack<ST_F16>, grid, dim3(256), 0, st, label, vector, dim[0], dim[1],
+                            dim[2], packed, (uint32_t *)packed_b, d_mm);
+      else
+         hipLaunchKernelGGL(k_pack<ST_ANGLE16>, grid, dim3(256), 0, st, label, vector, dim[0],
+                            dim[1], dim[2], packed, (uint32_t *)packed_b, d_mm);
+   }
    kp.stop();
    const int64_t n_tiles = ((dim[0] + RAYS_X - 1) / RAYS_X) * ((dim[1] + RAYS_Y - 1) / RAYS_Y);
    if (n_tiles * n_tilts >= ((int64_t)1 << 31)) return make_status(FPL_E_INVALID, "too many rays");
@@ -639,12 +753,19 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
    const Params *d_p = (const Params *)small;
    const TiltDev *d_t = (const TiltDev *)((char *)small + off_t);
    const PropDev *d_pr = (const PropDev *)((char *)small + off_pr);
-   if (n_vox < ((int64_t)1 << 32))
-      hipLaunchKernelGGL(k_march<uint32_t>, dim3((unsigned)(n_tiles * n_tilts)), dim3(WG), 0, st,
-                         d_p, d_t, d_pr, (const int4 *)packed, images, d_flags);
-   else
-      hipLaunchKernelGGL(k_march<int64_t>, dim3((unsigned)(n_tiles * n_tilts)), dim3(WG), 0, st,
-                         d_p, d_t, d_pr, (const int4 *)packed, images, d_flags);
+   const dim3 mgrid((unsigned)(n_tiles * n_tilts));
+   const bool small_idx = n_vox < ((int64_t)1 << 32);
+#define FPL_MARCH(IDX, STV)                                                                       \
+   hipLaunchKernelGGL((k_march<IDX, STV>), mgrid, dim3(WG), 0, st, d_p, d_t, d_pr, packed,       \
+                      (const uint32_t *)packed_b, images, d_flags)
+   if (storage == ST_F32) {
+      if (small_idx) FPL_MARCH(uint32_t, ST_F32); else FPL_MARCH(int64_t, ST_F32);
+   } else if (storage == ST_F16) {
+      if (small_idx) FPL_MARCH(uint32_t, ST_F16); else FPL_MARCH(int64_t, ST_F16);
+   } else {
+      if (small_idx) FPL_MARCH(uint32_t, ST_ANGLE16); else FPL_MARCH(int64_t, ST_ANGLE16);
+   }
+#undef FPL_MARCH
    kt.stop();
    FPL_HIP(hipGetLastError());
    FPL_HIP(hipMemcpyAsync(stage, flags, 32, hipMemcpyDeviceToHost, st));
@@ -656,6 +777,9 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
    const int32_t lmax = ((int32_t *)((char *)stage + 16))[1];
    if (lmin < 0) return make_status(FPL_E_LABEL_NEG, "label < 0 detected");
    if ((int64_t)lmax >= n_props) return make_status(FPL_E_LABEL_MAX, "label exceed properties.size()");
+   if ((storage == ST_F16 && lmax > 0xffff) || (storage == ST_ANGLE16 && lmax > 0x7fff))
+      return make_status(FPL_E_INVALID, "label %d does not fit the %s field storage", lmax,
+                         storage == ST_F16 ? "16-bit" : "15-bit");
    if (warn) *warn = f & 0x7fffffffu;
    return Status();
 }

@@ -47,6 +47,8 @@ class _SimpliBase:
         self._background_absorption = 0
         self._untilt_sensor_view = True
         self._wavelength = None
+        # storage of the fields inside the march: 'f32' (exact), 'f16', 'angle16'
+        self._field_storage = 'f32'
         # ANALYSIS
         self._noise_model = None
         # OTHER
@@ -262,6 +264,20 @@ class _SimpliBase:
     @step_size.setter
     def step_size(self, step_size):
         self._step_size = float(step_size)
+
+    @property
+    def field_storage(self):
+        """storage of label / optical axis inside the ray march (this package's
+        precision modes, no counterpart in the reference): 'f32' (int32 +
+        float32, exact input), 'f16' (uint16 + fp16, 8 B/voxel) or 'angle16'
+        (uint16 + 16-bit polar/azimuth codes, 6 B/voxel)"""
+        return self._field_storage
+
+    @field_storage.setter
+    def field_storage(self, field_storage):
+        if field_storage not in ('f32', 'f16', 'angle16'):
+            raise ValueError('field_storage must be \'f32\', \'f16\' or \'angle16\'')
+        self._field_storage = field_storage
 
     @property
     def verbose(self):

@@ -130,7 +130,7 @@ class Simpli(_SimpliBase):
         return _lib.make_setup(self._step_size, self._light_intensity, self._voxel_size,
                                self._wavelength, self._tissue_refrection, self._interpolate,
                                self._untilt_sensor_view, self._flip_z_beam,
-                               self._filter_rotations)
+                               self._filter_rotations, self._field_storage)
 
     def _prepare_fields(self, tissue, optical_axis, tissue_properties):
         """Validation of simpli.py:154-180; returns device tensors + props."""
@@ -219,6 +219,104 @@ class Simpli(_SimpliBase):
         self._check_volume_input()
         setup, _keep = self._setup()
         return _lib.count_voxel_steps(setup, self._dim_checked(), theta, phi)
+
+    # ------------------------------------------------------------ optic / analysis
+    def apply_optic_resample(self, data, shift=(0, 0), mp_pool=None):
+        """Gaussian blur + CCD resampling of an image stack (__simpli.py:491-549).
+
+        The blur has standard deviation optical_sigma / scale voxels with
+        scale = voxel_size / pixel_size.  Runs on the GPU; returns a numpy array
+        (squeezed as the reference) or a torch tensor for tensor input.
+        ``mp_pool`` is accepted for API compatibility and ignored.
+        """
+        self._print('Apply optic resample')
+        if self._optical_sigma is None:
+            raise ValueError('optical_sigma is None')
+        if self._voxel_size is None:
+            raise ValueError('voxel_size not set')
+        if self._pixel_size is None:
+            raise ValueError('pixel_size not set')
+        as_tensor = _is_tensor(data)
+        torch = _torch()
+        dev = torch.device('cuda', torch.cuda.current_device())
+        if as_tensor:
+            x = data.to(device=dev)
+            while x.dim() < 3:
+                x = x.unsqueeze(-1)
+        else:
+            x = torch.from_numpy(np.ascontiguousarray(np.atleast_3d(np.array(data)))).to(dev)
+        if x.dim() > 3:
+            raise TypeError('data can be 1d, 2d or 3d')
+        shift = np.array(shift, int)
+        if shift.size != 2 or shift.ndim != 1:
+            raise TypeError('shift has to be (x,y)')
+        if shift[0] > 0 and shift[1] > 0:
+            x = x[shift[0]:, shift[1]:, ...]
+        scale = self._voxel_size / self._pixel_size
+        size = np.array(np.round(np.array(x.shape[0:2]) * scale), dtype=int)
+        if np.amin(size) == 0:
+            raise ValueError(f'voxel_size {self._voxel_size}, ' +
+                             f'pixel_size {self._pixel_size} ' +
+                             f'and data shape {tuple(x.shape[0:2])} ' +
+                             f'result in optical image size of {size}')
+        x = x.to(torch.float64).contiguous()
+        out = torch.empty((int(size[0]), int(size[1]), x.shape[2]), dtype=torch.float64,
+                          device=dev)
+        _lib.optic_resample(x, self._optical_sigma / scale, scale, out)
+        if as_tensor:
+            return out.squeeze()
+        return np.squeeze(out.cpu().numpy())
+
+    def apply_optic(self, data, mp_pool=None):
+        """Optical resampling then the noise model (__simpli.py:462-489).
+
+        The noise model is the user's Python callable and runs on the host, as
+        in the reference.  Returns (resampled, noisy) numpy arrays.
+        """
+        self._print('Apply optic')
+        if self._noise_model is None:
+            raise ValueError('noise_model not set')
+        resampled = self.apply_optic_resample(data, mp_pool=mp_pool)
+        if _is_tensor(resampled):
+            resampled = resampled.cpu().numpy()
+        if np.amin(resampled) < 0:
+            raise AssertionError('intensity < 0 detected')
+        noisy = self._noise_model(np.array(resampled, copy=False))
+        return resampled, noisy
+
+    def apply_epa(self, data, mask=None):
+        """Transmittance, direction and retardation (__simpli.py:597-621, epa.py:9-45).
+
+        Computed on the GPU in float64; float32 input gives float32 results
+        (the reference computes those in float32).
+        """
+        self._print('Apply epa')
+        torch = _torch()
+        dev = torch.device('cuda', torch.cuda.current_device())
+        as_tensor = _is_tensor(data)
+        if as_tensor:
+            x = data.to(device=dev)
+            itemsize = x.element_size()
+        else:
+            arr = np.asarray(data)
+            itemsize = arr.dtype.itemsize
+            x = torch.from_numpy(np.ascontiguousarray(arr)).to(dev)
+        x = x.to(torch.float64).contiguous()
+        shape = tuple(x.shape[:-1])
+        t = torch.empty(shape, dtype=torch.float64, device=dev)
+        d = torch.empty_like(t)
+        r = torch.empty_like(t)
+        _lib.epa(x, t, d, r)
+        out = [t, d, r]
+        if itemsize <= 4:
+            out = [o.to(torch.float32) for o in out]
+        if not as_tensor:
+            out = [o.cpu().numpy() for o in out]
+        if mask is not None:
+            m = np.asarray(mask) if not as_tensor else mask
+            for o in out:
+                o[~m] = float('nan')
+        return tuple(out)
 
     # ------------------------------------------------------------ helpers
     def crop_tilt_pixel(self):

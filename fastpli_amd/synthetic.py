@@ -26,19 +26,22 @@ MYELIN = [(0.75, 0.0, 5.0, 'b'), (1.0, -0.004, 1.0, 'p')]
 
 
 def parallel_fibers(seed=0, n_side=8, spacing=2.2, radius=1.0, size=30.0, jitter=0.05,
-                    layers=MYELIN):
-    """64 straight fibers along x on a jittered hexagonal grid (config 1)."""
+                    layers=MYELIN, size_z=None, n_rows=None):
+    """Straight fibers along x on a jittered hexagonal grid (config 1: 8 x 8 in a
+    30 um cube).  ``size_z`` / ``n_rows`` give a taller stack of rows."""
     rng = np.random.default_rng(seed)
     pitch = spacing * np.sqrt(3.0) / 2.0
+    size_z = size if size_z is None else size_z
+    n_rows = n_side if n_rows is None else n_rows
     ys, zs = [], []
-    for row in range(n_side):
+    for row in range(n_rows):
         for col in range(n_side):
             ys.append(col * spacing + (spacing / 2.0 if row % 2 else 0.0))
             zs.append(row * pitch)
     ys = np.array(ys)
     zs = np.array(zs)
     ys += size / 2.0 - (ys.max() + ys.min()) / 2.0
-    zs += size / 2.0 - (zs.max() + zs.min()) / 2.0
+    zs += size_z / 2.0 - (zs.max() + zs.min()) / 2.0
     fibers = []
     for y, z in zip(ys, zs):
         j = rng.uniform(-jitter, jitter, size=(2, 3))

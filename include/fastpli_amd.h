@@ -88,6 +88,14 @@ typedef struct fpl_setup {
    int32_t flip_z;             /* bool */
    int32_t n_rho;
    const double *filter_rotations; /* HOST (n_rho) radian */
+   /* Storage of the fields inside the march (no counterpart in the
+    * reference, which reads int32 / float32):
+    *   0 f32      int32 label + float32 axis, 16 B/voxel (exact input)
+    *   1 f16      uint16 label + fp16 axis, 8 B/voxel
+    *   2 angle16  uint16 label + 16-bit polar/azimuth codes, 6 B/voxel
+    * Modes 1 and 2 need labels < 65536 / 32768 and treat axes that pass the
+    * reference's normalisation test as unit vectors. */
+   int32_t field_storage;
 } fpl_setup;
 
 /* ---------------------------------------------------------- entry points */
@@ -121,6 +129,24 @@ int fpl_simulate(const fpl_setup *setup, const int64_t dim[3],
                  const double *props, int64_t n_props, const double *tilts,
                  int64_t n_tilts, double *images, uint32_t *warn, void *stream,
                  char *msg, size_t msg_len);
+
+/* Optical resampling of an image stack, the step after run_simulation in the
+ * reference's pipeline: optic.filter_resample applied per rotation image
+ * (src/fastpli/simulation/optic.py:169-187, called from
+ * Simpli.apply_optic_resample, src/fastpli/simulation/__simpli.py:491-549).
+ * Gaussian blur with standard deviation `sigma` in source pixels (scipy
+ * gaussian_filter: radius int(4 sigma + 1/2), reflect boundary; sigma = 0 skips
+ * it), then box resampling by `scale` (optic.py:85-120).
+ *   in  DEVICE float64 (nx, ny, n_rho)
+ *   out DEVICE float64 (round(nx * scale), round(ny * scale), n_rho) */
+int fpl_optic_resample(const double *in, int64_t nx, int64_t ny, int64_t n_rho, double sigma,
+                       double scale, double *out, void *stream, char *msg, size_t msg_len);
+
+/* EPA: transmittance, direction and retardation of n_pix pixels with n_rho
+ * equidistant rotations in [0, pi) (src/fastpli/analysis/epa.py:9-45, float64
+ * branch).  in DEVICE float64 (n_pix, n_rho); outputs DEVICE float64 (n_pix). */
+int fpl_epa(const double *in, int64_t n_pix, int64_t n_rho, double *transmittance,
+            double *direction, double *retardation, void *stream, char *msg, size_t msg_len);
 
 /* Exact count of Mueller voxel-steps of one tilt: n_rho x rays x z-march
  * iterations (simulator.cpp:259-261).  Host only. */

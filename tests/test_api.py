@@ -143,3 +143,12 @@ def test_dat_loader_round_trip(tmp_path):
     again = fio.load(str(p))
     for a, b in zip(fbs[0], again[0]):
         np.testing.assert_allclose(a.as_array(), b.as_array(), atol=1e-6)
+
+
+def test_field_storage_property():
+    s = Simpli()
+    assert s.field_storage == 'f32'
+    s.field_storage = 'angle16'
+    assert s.get_dict()['field_storage'] == 'angle16'
+    with pytest.raises(ValueError):
+        s.field_storage = 'bf16'
