@@ -54,6 +54,26 @@ def voxel_steps(s):
     return sum(s.count_voxel_steps(t, p) for t, p in TILTS)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'hbm_traffic_crossing_600x600x300.txt')
+
+
+def measured_traffic(kernel, storage):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE + WRITE_SIZE summary.
+
+    The summary holds lines `traffic <kernel> <storage> <bytes>` (counter KiB scaled by
+    the factors a 1 GiB device copy gives); None when no line matches.
+    """
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 4 and w[0] == 'traffic' and w[1] == kernel and w[2] == storage:
+                    return float(w[3])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(fbs, fraction, threads):
     """Oracle (FP64, OpenMP) on the y-slab [0, fraction * 120 um] of the workload."""
     import oracle
@@ -84,11 +104,14 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--cpu-fraction', type=float, default=0.1,
+    ap.add_argument('--cpu-fraction', type=float, default=1.0,
                     help='y fraction of the volume timed for the CPU baseline')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--field-storage', default='f32', choices=['f32', 'f16', 'angle16'],
                     help='storage of the fields inside the march (precision modes)')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='after timing, copy 1 GiB device to device once (the reference kernel '
+                         'that scales FETCH_SIZE / WRITE_SIZE counters to bytes)')
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -167,9 +190,16 @@ def main():
         'peak': HBM_PEAK_GBS,
         'unit': 'GB/s',
         'frac': kd['achieved_GBs'] / HBM_PEAK_GBS,
-        'traffic': None,
+        'traffic': measured_traffic(dominant, args.field_storage),
         'kernel': dominant,
     }
+
+    if args.copy_calibration:
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     cpu = None
     if not args.no_cpu_baseline:
