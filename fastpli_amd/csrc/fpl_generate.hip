@@ -18,9 +18,12 @@
 //   scan          hipcub exclusive sum of per-segment tile counts
 //   k_emit        (tile, segment) pairs in segment order
 //   radix sort    stable by tile -> per tile list in global segment order
-//   k_voxelize    one workgroup per tile; each wave owns a 4x4x32 column
-//                 block, each lane 8 voxels; candidate records are wave-uniform
-//                 loads; per voxel the reference's update rule runs in order.
+//   k_voxelize    one workgroup per tile, the tile's segment records staged in
+//                 LDS; each wave owns an 8x8x8 slab whose state lives in LDS.
+//                 A float lower-bound test (conservative by construction)
+//                 queues candidates; the exact FP64 test runs on full batches of
+//                 64 queued candidates; per voxel the reference's update rule is
+//                 applied in segment order.
 //
 // This file must be compiled with -ffp-contract=off: the distance test must
 // round every FP64 operation exactly as the reference's scalar code.
@@ -36,18 +39,24 @@ namespace fpl {
 namespace gen {
 
 constexpr int TX = 8, TY = 8, TZ = 32; // voxel tile of one workgroup
-constexpr int WG = 256;                // 4 waves; wave w owns the 4x4x32 block (w/2, w%2)
+constexpr int WG = 256;                // 4 waves; wave w owns the 8x8x8 slab z in [8w, 8w + 8)
+constexpr int SZ = 8;                  // slab height
+constexpr int SLOTS = 64 * SZ;         // voxels of a slab: slot = (z - z_slab) * 64 + column
+constexpr int CHUNK = 64;              // segment records staged in LDS at a time
+constexpr int QN = 128;                // candidate ring of a wave (never holds 128 entries)
 
-struct SegRec {      // 128 bytes, one per segment (indexed by its first point)
-   double p[3];      // start point, voxel units
-   double q[3];      // end point
-   double v[3];      // q - p
-   double c2;        // dot(v, v)
-   double rp, rq;    // radii at p and q
-   int32_t lo[3];    // integer bbox [lo, hi) after intersection with the volume
+struct SegRec {        // 128 bytes, one per segment (indexed by its first point)
+   double p[3];        // start point, voxel units
+   double q[3];        // end point
+   double rp, rq;      // radii at p and q
+   int32_t lo[3];      // integer bbox [lo, hi) after intersection with the volume
    int32_t hi[3];
+   float bl[3];        // min(p, q) - 0.5 rounded down and max(p, q) - 0.5 rounded up: the
+   float bh[3];        // segment's box in the frame where voxel i has its centre at i
+   float thr;          // f_last * float(max(rp, rq)^2): outer-layer test at the larger radius
+   float f_last;       // float(last scale^2) of the bundle
+   int32_t label_base; // 1 + bundle * max_layer
    int32_t bundle;
-   int32_t pad;
 };
 static_assert(sizeof(SegRec) == 128, "SegRec layout");
 
@@ -263,16 +272,15 @@ __global__ void k_segments(Params pr, const double *__restrict__ P,
       load_point(P, i + 1, pr, R.q, rq);
       R.rp = rp;
       R.rq = rq;
-      for (int k = 0; k < 3; k++) R.v[k] = R.q[k] - R.p[k];
-      double c2 = 0.0;
-      c2 = c2 + R.v[0] * R.v[0];
-      c2 = c2 + R.v[1] * R.v[1];
-      c2 = c2 + R.v[2] * R.v[2];
-      R.c2 = c2;
       // generator.cpp:295-305
-      const double max_r = ((rp < rq) ? rq : rp) * bd[b].sqrt_last;
+      const double rmax = (rp < rq) ? rq : rp; // std::max
+      const double max_r = rmax * bd[b].sqrt_last;
       double lo[3], hi[3];
       box_of(R.p, R.q, lo, hi);
+      for (int k = 0; k < 3; k++) {
+         R.bl[k] = __double2float_rd(lo[k] - 0.5);
+         R.bh[k] = __double2float_ru(hi[k] - 0.5);
+      }
       const double vhi[3] = {(double)pr.nx, (double)pr.ny, (double)pr.nz};
       bool empty = false;
       int64_t ilo[3], ihi[3];
@@ -290,8 +298,10 @@ __global__ void k_segments(Params pr, const double *__restrict__ P,
             R.lo[k] = (int32_t)ilo[k];
             R.hi[k] = (int32_t)ihi[k];
          }
+         R.f_last = bd[b].f_last_sq;
+         R.thr = R.f_last * (float)(rmax * rmax);
+         R.label_base = bd[b].label_base;
          R.bundle = b;
-         R.pad = 0;
          segs[i] = R;
          const int64_t tx0 = ilo[0] / TX, tx1 = (ihi[0] - 1) / TX;
          const int64_t ty0 = ilo[1] / TY, ty1 = (ihi[1] - 1) / TY;
@@ -352,13 +362,33 @@ __device__ __forceinline__ void unit_to_float(double x, double y, double z, floa
    out[2] = (float)z;
 }
 
+struct SegExact { // what the FP64 distance test reads of a segment
+   double p[3], v[3], q[3], c2;
+};
+
+// v = q - p and c2 = dot(v, v) rounded as generator.cpp:283-289 computes them
+__device__ __forceinline__ SegExact seg_exact(const double p[3], const double q[3]) {
+   SegExact E;
+   for (int k = 0; k < 3; k++) {
+      E.p[k] = p[k];
+      E.q[k] = q[k];
+      E.v[k] = q[k] - p[k];
+   }
+   double c2 = 0.0;
+   c2 = c2 + E.v[0] * E.v[0];
+   c2 = c2 + E.v[1] * E.v[1];
+   c2 = c2 + E.v[2] * E.v[2];
+   E.c2 = c2;
+   return E;
+}
+
 struct Closest {
    double c[3];
    double t;
 };
 
 // ShortestPointToLineSegmentVecCalculation (generator.cpp:475-494)
-__device__ __forceinline__ Closest closest(const double pt[3], const SegRec &R) {
+__device__ __forceinline__ Closest closest(const double pt[3], const SegExact &R) {
    const double w0 = pt[0] - R.p[0], w1 = pt[1] - R.p[1], w2 = pt[2] - R.p[2];
    double c1 = 0.0;
    c1 = c1 + w0 * R.v[0];
@@ -394,106 +424,319 @@ __device__ __forceinline__ int tile_of_block(int bid, int n) {
    return (bid % 8) * per + bid / 8;
 }
 
+// orders the LDS accesses of one wave: its lanes exchange data through LDS
+__device__ __forceinline__ void wave_sync() {
+   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+   __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+   for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+   return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+   for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+   return v;
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniform(float v) {
+   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// A voxel's state is one 64-bit key: the float bits of its stored squared
+// distance (high word; non-negative floats order as integers) and
+// 0x80000000 | label (low word).  A candidate of a batch enters with the low
+// word 63 - lane, so the minimum key over the stored state and the batch is
+// the smallest distance with ties going to the latest candidate -- the state
+// the reference's sequential rule "update if stored >= new" ends in.
+constexpr unsigned long long KEY_EMPTY = (0x7F800000ull << 32) | 0x80000000ull; // +inf, label 0
+
+struct WaveState {
+   unsigned long long key[SLOTS];
+   int32_t seg[SLOTS];  // global index of the stored segment, -1 none
+   uint32_t queue[QN];  // candidates (chunk index << 9 | slot), ring
+};
+
+struct alignas(16) LdsSeg { // 144-byte stride spreads records over the LDS banks
+   SegRec r;
+   int32_t pad[4];
+};
+
+// Exact FP64 test of up to 64 queued candidates (one per lane) and their
+// ordered application to the slab state.
+__device__ __forceinline__ void run_batch(WaveState &W, const LdsSeg *chunk,
+                                          const uint32_t *chunk_idx, int n, int q_head,
+                                          int lane, int xt, int yt, int zw, int max_layer,
+                                          const BundleDev *__restrict__ bd,
+                                          const double *__restrict__ layer_sq, uint32_t *flags,
+                                          bool &ov) {
+   const bool valid = lane < n;
+   const uint32_t e = valid ? W.queue[(q_head + lane) & (QN - 1)] : 0u;
+   const int c = (int)(e >> 9), slot = (int)(e & (SLOTS - 1));
+   bool acc = false;
+   float dist_sq = 0.0f;
+   int32_t new_label = 0;
+   if (valid) {
+      const SegRec &R = chunk[c].r;
+      const double p[3] = {R.p[0], R.p[1], R.p[2]};
+      const double q[3] = {R.q[0], R.q[1], R.q[2]};
+      const double rp = R.rp, rq = R.rq;
+      const float f_last = R.f_last;
+      const SegExact E = seg_exact(p, q);
+      const int col = slot & 63;
+      const double pt[3] = {(double)(xt + (col >> 3)) + 0.5, (double)(yt + (col & 7)) + 0.5,
+                            (double)(zw + (slot >> 6)) + 0.5};
+      // FillVoxelsAroundFiberSegment inner loop (generator.cpp:322-349)
+      const Closest cp = closest(pt, E);
+      const double d0 = cp.c[0] - pt[0], d1 = cp.c[1] - pt[1], d2v = cp.c[2] - pt[2];
+      double dd = 0.0;
+      dd = dd + d0 * d0;
+      dd = dd + d1 * d1;
+      dd = dd + d2v * d2v;
+      dist_sq = (float)dd;
+      const double rr = rp * (1 - cp.t) + rq * cp.t;
+      const float r_sq = (float)(rr * rr);
+      acc = !(dist_sq > f_last * r_sq);
+      if (acc) {
+         // std::lower_bound over the bundle's double scale^2 list with the
+         // float ratio (generator.cpp:345-349)
+         const BundleDev &B = bd[R.bundle];
+         const double ratio = (double)(dist_sq / r_sq);
+         const int off = B.layer_off, nl = B.n_layers;
+         int ly = 0;
+         while (ly < nl && layer_sq[off + ly] < ratio) ly++;
+         new_label = ly + R.label_base;
+      }
+   }
+   unsigned long long old = KEY_EMPTY;
+   if (acc) old = W.key[slot];
+   // The overlap warning (generator.cpp:353-359) depends on the order of the
+   // updates.  Unless it is already raised, a batch whose labels (new ones and
+   // the ones they may replace) all lie within max_layer of each other cannot
+   // raise it; any other batch is applied one candidate at a time.
+   bool serial = false;
+   if (!ov) {
+      int mn = INT_MAX, mx = INT_MIN;
+      if (acc) {
+         mn = mx = new_label;
+         const int ol = (int)(uint32_t)(old & 0x7FFFFFFFull);
+         if (ol != 0) {
+            mn = min(mn, ol);
+            mx = max(mx, ol);
+         }
+      }
+      mn = wave_min(mn);
+      mx = wave_max(mx);
+      serial = uniform((int)(mx != INT_MIN && (long long)mx - mn >= max_layer)) != 0;
+   }
+   wave_sync();
+   const unsigned long long rest =
+       ((unsigned long long)__float_as_uint(dist_sq) << 32) | 0x80000000ull | (uint32_t)new_label;
+   if (!serial) {
+      const unsigned long long k =
+          ((unsigned long long)__float_as_uint(dist_sq) << 32) | (unsigned long long)(63 - lane);
+      if (acc) atomicMin(&W.key[slot], k);
+      wave_sync();
+      if (acc && W.key[slot] == k) {
+         W.key[slot] = rest;
+         W.seg[slot] = (int32_t)chunk_idx[c];
+      }
+   } else {
+      uint32_t overlap = 0;
+      for (int i = 0; i < n; i++) {
+         if (lane == i && acc) {
+            const unsigned long long cur = W.key[slot];
+            if (__uint_as_float((uint32_t)(cur >> 32)) >= dist_sq) {
+               const int ol = (int)(uint32_t)(cur & 0x7FFFFFFFull);
+               if (ol != 0 && abs(ol - new_label) >= max_layer) overlap = 1;
+               W.key[slot] = rest;
+               W.seg[slot] = (int32_t)chunk_idx[c];
+            }
+         }
+         wave_sync();
+      }
+      if (__ballot(overlap != 0) != 0) {
+         if (lane == 0) atomicOr(&flags[0], FPL_WARN_OVERLAP);
+         ov = true;
+      }
+   }
+   wave_sync();
+}
+
+// One workgroup per 8x8x32 tile, wave w owning the slab z in [8w, 8w + 8),
+// lane l the voxel column (l / 8, l % 8).  The tile's segments (global order)
+// are staged in LDS in chunks.  Stage A tests every (segment, slab voxel) of
+// the segment's integer bbox with a float lower bound of the squared distance
+// (distance to the segment's box) against the outer-layer threshold and the
+// voxel's stored distance, with margins that keep every candidate the exact
+// test could accept; survivors enter the wave's queue in order.  Stage B
+// (run_batch) runs the exact FP64 test on 64 queued candidates at a time.
 __global__ void __launch_bounds__(WG)
     k_voxelize(Params pr, const SegRec *__restrict__ segs, const uint32_t *__restrict__ list,
                const uint32_t *__restrict__ range, const BundleDev *__restrict__ bd,
                const double *__restrict__ layer_sq, const uint8_t *__restrict__ layer_model,
                int only_label, int32_t *__restrict__ label, float *__restrict__ vector,
                uint32_t *flags) {
+   __shared__ WaveState wstate[WG / 64];
+   __shared__ LdsSeg chunk[CHUNK];
+   __shared__ uint32_t chunk_idx[CHUNK];
+
    const int n_tiles = pr.ntx * pr.nty * pr.ntz;
    const int t = tile_of_block((int)blockIdx.x, n_tiles);
    const int tz = t % pr.ntz;
    const int ty = (t / pr.ntz) % pr.nty;
    const int tx = t / (pr.ntz * pr.nty);
-   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-   const int xw = tx * TX + 4 * (wave >> 1);
-   const int yw = ty * TY + 4 * (wave & 1);
-   const int zw = tz * TZ;
-   const int x = xw + (lane >> 4);
-   const int y = yw + ((lane >> 2) & 3);
-   const int lz = lane & 3;
-
-   float best_d[8];
-   int32_t best_seg[8];
-   int32_t best_label[8];
-#pragma unroll
-   for (int k = 0; k < 8; k++) {
-      best_d[k] = INFINITY;
-      best_seg[k] = -1;
-      best_label[k] = 0;
-   }
-   uint32_t overlap = 0;
+   // the wave index is uniform: keep it (and all loop bounds derived from it) scalar
+   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+   const int lane = threadIdx.x & 63;
+   const int xt = tx * TX, yt = ty * TY;
+   const int zw = tz * TZ + wave * SZ;
+   const int x = xt + (lane >> 3), y = yt + (lane & 7);
+   const float xf = (float)x, yf = (float)y;
    const int max_layer = pr.max_layer;
+   // float margins of the stage A test: relative 2^-12, absolute 2^-6
+   const float rel = 1.0f + 0x1p-12f, absm = 0x1p-6f;
 
+   WaveState &W = wstate[wave];
+   for (int s = lane; s < SLOTS; s += 64) {
+      W.key[s] = KEY_EMPTY;
+      W.seg[s] = -1;
+   }
+   bool ov = (__atomic_load_n(&flags[0], __ATOMIC_RELAXED) & FPL_WARN_OVERLAP) != 0;
+   ov = uniform((int)ov) != 0;
+   int q_head = 0, q_tail = 0;
+   wave_sync();
+
+   __shared__ unsigned long long slab_mask[WG / 64]; // chunk records touching each slab
    const uint32_t beg = range[2 * t], end = range[2 * t + 1];
-   for (uint32_t it = beg; it < end; it++) {
-      const uint32_t s = list[it];
-      const SegRec &R = segs[s];
-      const int lo0 = R.lo[0], lo1 = R.lo[1], lo2 = R.lo[2];
-      const int hi0 = R.hi[0], hi1 = R.hi[1], hi2 = R.hi[2];
-      // wave-uniform rejection of the 4x4x32 block
-      if (hi0 <= xw || lo0 >= xw + 4 || hi1 <= yw || lo1 >= yw + 4 || hi2 <= zw ||
-          lo2 >= zw + TZ)
-         continue;
-      if (x < lo0 || x >= hi0 || y < lo1 || y >= hi1) continue;
-      const BundleDev B = bd[R.bundle];
-      const float f_last = B.f_last_sq;
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-         const int zb = zw + 4 * k;
-         if (hi2 <= zb || lo2 >= zb + 4) continue;
-         const int z = zb + lz;
-         if (z < lo2 || z >= hi2) continue;
-         const double pt[3] = {x + 0.5, y + 0.5, z + 0.5};
-         const Closest cp = closest(pt, R);
-         const double d0 = cp.c[0] - pt[0], d1 = cp.c[1] - pt[1], d2v = cp.c[2] - pt[2];
-         double dd = 0.0;
-         dd = dd + d0 * d0;
-         dd = dd + d1 * d1;
-         dd = dd + d2v * d2v;
-         const float dist_sq = (float)dd;
-         const double rr = R.rp * (1 - cp.t) + R.rq * cp.t;
-         const float r_sq = (float)(rr * rr);
-         if (dist_sq > f_last * r_sq) continue;
-         if (best_d[k] >= dist_sq) {
-            // std::lower_bound over the bundle's double scale^2 list with the
-            // float ratio (generator.cpp:345-349)
-            const double ratio = (double)(dist_sq / r_sq);
-            int ly = 0;
-            while (ly < B.n_layers && layer_sq[B.layer_off + ly] < ratio) ly++;
-            const int32_t new_label = ly + B.label_base;
-            const int32_t old = best_label[k];
-            if (old != 0 && abs(old - new_label) >= max_layer) overlap = 1;
-            best_d[k] = dist_sq;
-            best_seg[k] = (int32_t)s;
-            best_label[k] = new_label;
+   for (uint32_t base = beg; base < end; base += CHUNK) {
+      const int n = (int)((end - base) < (uint32_t)CHUNK ? (end - base) : (uint32_t)CHUNK);
+      __syncthreads(); // every wave is done with the previous chunk
+      for (int e = threadIdx.x; e < n * 8; e += WG) {
+         const int c = e >> 3, piece = e & 7;
+         const uint32_t sidx = list[base + c];
+         ((int4 *)&chunk[c].r)[piece] = ((const int4 *)&segs[sidx])[piece];
+         if (piece == 0) chunk_idx[c] = sidx;
+      }
+      if (wave == 0) { // which records reach into which slab
+         int zlo = 0, zhi = 0;
+         if (lane < n) {
+            const SegRec &G = segs[list[base + lane]];
+            zlo = G.lo[2];
+            zhi = G.hi[2];
+         }
+         for (int w = 0; w < WG / 64; w++) {
+            const int s0 = tz * TZ + w * SZ;
+            const unsigned long long m = __ballot(lane < n && zlo < s0 + SZ && zhi > s0);
+            if (lane == 0) slab_mask[w] = m;
          }
       }
+      __syncthreads();
+      unsigned long long todo = slab_mask[wave];
+      todo = ((unsigned long long)uniform((int)(uint32_t)(todo >> 32)) << 32) |
+             (uint32_t)uniform((int)(uint32_t)todo);
+      while (todo) {
+         const int c = __builtin_ctzll(todo);
+         todo &= todo - 1;
+         const SegRec &R = chunk[c].r;
+         const int lo0 = uniform(R.lo[0]), hi0 = uniform(R.hi[0]);
+         const int lo1 = uniform(R.lo[1]), hi1 = uniform(R.hi[1]);
+         const int z0 = max(uniform(R.lo[2]), zw), z1 = min(uniform(R.hi[2]), zw + SZ);
+         const float bl0 = uniform(R.bl[0]), bh0 = uniform(R.bh[0]);
+         const float bl1 = uniform(R.bl[1]), bh1 = uniform(R.bh[1]);
+         const float bl2 = uniform(R.bl[2]), bh2 = uniform(R.bh[2]);
+         const float thr_m = fmaf(uniform(R.thr), rel, absm);
+         // Uniform z range.  The in-bbox columns nearest to the segment's box
+         // bound every lane's float ex and ey from below (float rounding is
+         // monotone), so a voxel that passes the float radius test
+         // RN(ez^2 + exy) <= thr_m has ez^2 <= rem + ulp(thr_m), rem =
+         // thr_m - exy_min >= 0.  h covers that with margin, and the range ends
+         // are computed in double, exact for any float box and voxel index.
+         const int cx0 = max(lo0, xt), cx1 = min(hi0, xt + TX) - 1;
+         const int cy0 = max(lo1, yt), cy1 = min(hi1, yt + TY) - 1;
+         const float exm = fmaxf(fmaxf(bl0 - (float)cx1, (float)cx0 - bh0), 0.0f);
+         const float eym = fmaxf(fmaxf(bl1 - (float)cy1, (float)cy0 - bh1), 0.0f);
+         const float rem = thr_m - fmaf(eym, eym, exm * exm);
+         if (!(rem >= 0.0f)) continue;
+         // hardware square root (1 ulp), covered by the relative margin
+         const float h =
+             fmaf(__builtin_amdgcn_sqrtf(fmaf(thr_m, 0x1p-20f, rem)), 1.0f + 0x1p-10f, 0x1p-10f);
+         const int za = max(z0, uniform((int)ceil((double)bl2 - (double)h)));
+         const int zb = min(z1, uniform((int)floor((double)bh2 + (double)h)) + 1);
+         const bool in_xy = (x >= lo0) & (x < hi0) & (y >= lo1) & (y < hi1);
+         const float ex = fmaxf(fmaxf(bl0 - xf, xf - bh0), 0.0f);
+         const float ey = fmaxf(fmaxf(bl1 - yf, yf - bh1), 0.0f);
+         const float exy = fmaf(ey, ey, ex * ex);
+         for (int z = za; z < zb; z++) {
+            const int slot = (z - zw) * 64 + lane;
+            const float zf = (float)z;
+            const float ez = fmaxf(fmaxf(bl2 - zf, zf - bh2), 0.0f);
+            const float lb = fmaf(ez, ez, exy);
+            const float stored = __uint_as_float((uint32_t)(W.key[slot] >> 32));
+            const bool pass = in_xy & (lb <= thr_m) & (lb <= fmaf(stored, rel, absm));
+            const unsigned long long m = __ballot(pass);
+            if (m) {
+               const int pos = q_tail + (int)__builtin_amdgcn_mbcnt_hi(
+                                            (uint32_t)(m >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+               if (pass) W.queue[pos & (QN - 1)] = ((uint32_t)c << 9) | (uint32_t)slot;
+               q_tail += __popcll(m);
+               if (q_tail - q_head >= 64) {
+                  wave_sync();
+                  run_batch(W, chunk, chunk_idx, 64, q_head, lane, xt, yt, zw, max_layer, bd,
+                            layer_sq, flags, ov);
+                  q_head += 64;
+               }
+            }
+         }
+      }
+      // drain: the queue refers to this chunk's records
+      wave_sync();
+      while (q_tail > q_head) {
+         const int nb = min(64, q_tail - q_head);
+         run_batch(W, chunk, chunk_idx, nb, q_head, lane, xt, yt, zw, max_layer, bd, layer_sq,
+                   flags, ov);
+         q_head += nb;
+      }
+      if (!ov)
+         ov = uniform((int)((__atomic_load_n(&flags[0], __ATOMIC_RELAXED) & FPL_WARN_OVERLAP) !=
+                            0)) != 0;
    }
 
-   const bool in_xy = x < pr.nx && y < pr.ny;
-#pragma unroll
-   for (int k = 0; k < 8; k++) {
-      const int z = zw + 4 * k + lz;
-      if (!in_xy || z >= pr.nz) continue;
-      const int64_t ind = ((int64_t)x * pr.ny + y) * pr.nz + z;
-      label[ind] = best_label[k];
+   // results: eight columns of eight voxels per pass, z fastest over the lanes
+   wave_sync();
+   for (int j = 0; j < 8; j++) {
+      const int col = j * 8 + (lane >> 3), zz = lane & 7;
+      const int slot = zz * 64 + col;
+      const int vx = xt + (col >> 3), vy = yt + (col & 7), vz = zw + zz;
+      if (vx >= pr.nx || vy >= pr.ny || vz >= pr.nz) continue;
+      const int64_t ind = ((int64_t)vx * pr.ny + vy) * pr.nz + vz;
+      const int32_t lab = (int32_t)(uint32_t)(W.key[slot] & 0x7FFFFFFFull);
+      const int32_t sg = W.seg[slot];
+      label[ind] = lab;
       if (!only_label) {
          float vo[3] = {0.0f, 0.0f, 0.0f};
-         if (best_seg[k] >= 0) {
-            const SegRec &R = segs[best_seg[k]];
+         if (sg >= 0) {
+            const SegRec &R = segs[sg];
             const BundleDev B = bd[R.bundle];
-            int ly = best_label[k] - B.label_base;
+            int ly = lab - R.label_base;
             // a float ratio just above the last scale^2 gives the past-the-end
             // layer index; its orientation is read as the outermost layer's
             if (ly > B.n_layers - 1) ly = B.n_layers - 1;
             const uint8_t m = layer_model[B.layer_off + ly];
-            if (m == 'p') {
-               unit_to_float(R.v[0], R.v[1], R.v[2], vo);
-            } else if (m == 'r') {
-               const double pt[3] = {x + 0.5, y + 0.5, z + 0.5};
-               const Closest cp = closest(pt, R);
-               unit_to_float(pt[0] - cp.c[0], pt[1] - cp.c[1], pt[2] - cp.c[2], vo);
+            if (m != 'b') {
+               const double p[3] = {R.p[0], R.p[1], R.p[2]};
+               const double q[3] = {R.q[0], R.q[1], R.q[2]};
+               const SegExact E = seg_exact(p, q);
+               if (m == 'p') {
+                  unit_to_float(E.v[0], E.v[1], E.v[2], vo);
+               } else {
+                  const double pt[3] = {vx + 0.5, vy + 0.5, vz + 0.5};
+                  const Closest cp = closest(pt, E);
+                  unit_to_float(pt[0] - cp.c[0], pt[1] - cp.c[1], pt[2] - cp.c[2], vo);
+               }
             }
          }
          vector[3 * ind + 0] = vo[0];
@@ -501,9 +744,6 @@ __global__ void __launch_bounds__(WG)
          vector[3 * ind + 2] = vo[2];
       }
    }
-   if (__ballot(overlap != 0) != 0 && lane == 0 &&
-       !(__atomic_load_n(&flags[0], __ATOMIC_RELAXED) & FPL_WARN_OVERLAP))
-      atomicOr(&flags[0], FPL_WARN_OVERLAP);
 }
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
@@ -515,6 +755,8 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
    if (nx <= 0 || ny <= 0 || nz <= 0)
       return make_status(FPL_E_INVALID, "dim.global[any] <= 0: [%lld,%lld,%lld]", (long long)nx,
                          (long long)ny, (long long)nz);
+   if (nx >= (1 << 24) || ny >= (1 << 24) || nz >= (1 << 24))
+      return make_status(FPL_E_INVALID, "volume dimension >= 2^24 is not supported");
    if (!(vol->voxel_size > 0))
       return make_status(FPL_E_INVALID, "voxel_size <= 0: %f", vol->voxel_size);
    if (!label || (!only_label && !vector))

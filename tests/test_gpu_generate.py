@@ -1,6 +1,7 @@
 """GPU parity of generate_tissue: labels and vectors bit-exact vs the oracle."""
 
 import sys
+import warnings
 
 import numpy as np
 import pytest
@@ -29,9 +30,16 @@ def _simpli(fbs, dim, origin, voxel_size):
 
 def _compare(fbs, dim, origin, voxel_size, only_tissue=False):
     s = _simpli(fbs, dim, origin, voxel_size)
-    label, vector, props = s.generate_tissue(only_tissue=only_tissue)
+    with warnings.catch_warnings(record=True) as caught:
+        warnings.simplefilter('always')
+        label, vector, props = s.generate_tissue(only_tissue=only_tissue)
     ol, ov, op, ow = oracle.generate(fbs, dim, np.asarray(origin, float), voxel_size,
                                      only_label=only_tissue, n_threads=8)
+    # the overlap warning depends on the order of the updates: same verdict as the oracle
+    got_overlap = any('objects overlap' in str(w.message) for w in caught)
+    assert got_overlap == bool(ow & oracle.WARN_OVERLAP)
+    for w in caught:
+        warnings.warn(w.message, w.category)
     assert label.dtype == np.int32 and label.shape == tuple(dim)
     assert np.array_equal(label, ol), f'{np.count_nonzero(label != ol)} labels differ'
     if only_tissue:
@@ -80,6 +88,27 @@ def test_generate_ragged_and_edges():
         _compare(fbs, (1, 7, 3), [0.0] * 3, 1.0)
     with pytest.warns(UserWarning):
         _compare(fbs, (33, 1, 65), [0.0] * 3, 0.25)
+
+
+def test_generate_crossing_bundles_small():
+    # two interpenetrating bundles, many segments per tile (several LDS chunks
+    # and full candidate batches per wave)
+    _say('generate: crossing bundles 2 x 300 fibers, 120 x 120 x 60')
+    fbs = synthetic.crossing_bundles(seed=3, n_fibers=300, size_xy=24.0, size_z=12.0)
+    with pytest.warns(UserWarning, match='objects overlap'):
+        _compare(fbs, (120, 120, 60), [0.0] * 3, 0.2)
+
+
+def test_generate_adjacent_bundles_without_overlap():
+    # fibers of different bundles side by side with a gap: batches mix labels of
+    # both bundles, and no voxel is claimed by both, so no overlap warning
+    _say('generate: adjacent bundles, no overlap')
+    ly = [(0.6, 0, 1, 'b'), (1.0, -0.004, 1, 'p')]
+    a = [[[-1, y, z, 1.0], [21, y, z, 1.0]] for y in (2.0, 6.6, 11.2) for z in (2.0, 6.6)]
+    b = [[[-1, y, z, 1.0], [21, y, z, 1.0]] for y in (4.3, 8.9) for z in (2.0, 6.6)]
+    fbs = objects.FiberBundles([a, b], [ly, ly])
+    _, _, ow = _compare(fbs, (40, 28, 18), [0.0] * 3, 0.5)
+    assert not ow & oracle.WARN_OVERLAP
 
 
 def test_generate_overlap_warning():
