@@ -19,6 +19,7 @@ import json
 import os
 import sys
 import time
+import warnings
 
 import numpy as np
 
@@ -36,13 +37,14 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_simpli(dim, fbs):
+def make_simpli(dim, fbs=None):
     from fastpli_amd.simulation import Simpli
     s = Simpli()
     s.voxel_size = VOXEL
     s.dim = np.array(dim, dtype=int)
     s.dim_origin = [0.0, 0.0, 0.0]
-    s.fiber_bundles = fbs
+    if fbs is not None:
+        s.fiber_bundles = fbs
     s.filter_rotations = ROT
     s.light_intensity = 26000
     s.wavelength = 525
@@ -72,6 +74,113 @@ def measured_traffic(kernel, storage):
     except OSError:
         pass
     return None
+
+
+SERVE_SIZES = [(64, 64, 60), (128, 128, 120), (256, 256, 200), (512, 512, 300)]
+
+
+def serving_bench(n_requests, load, seed=0):
+    """Mixed-size request stream through SimulationServer (p50 / p99 latency).
+
+    Four registered tissues (crossing bundles at 0.2 um, fiber count scaled to
+    the cross-section): 64^2x60, 128^2x120, 256^2x200, 512^2x300 voxels.  Each
+    request picks a size uniformly, the 5-tilt set or the untilted view alone
+    (1:1), Slerp or NN (1:1), and scales the layers' dn and mu by U(0.5, 1.5);
+    9 rotations.  Arrivals are Poisson at `load` x the rate one request at a
+    time would sustain (mean service time measured first, each shape alone).
+    Latency is submit -> completion as the server measures it.
+    """
+    import torch
+    from fastpli_amd import synthetic
+    from fastpli_amd.simulation import SimulationServer
+    rng = np.random.default_rng(seed)
+    srv = SimulationServer(max_inflight=4)
+    tissues = []
+    for (nx, ny, nz) in SERVE_SIZES:
+        L, Z = nx * VOXEL, nz * VOXEL
+        n_f = max(8, int(round(2000 / (120.0 * 60.0) * L * Z)))
+        s = make_simpli([nx, ny, nz], synthetic.crossing_bundles(seed=seed + nx, n_fibers=n_f,
+                                                                  size_xy=L, size_z=Z))
+        with warnings.catch_warnings():
+            warnings.simplefilter('ignore')
+            lab, vec, props = s.generate_tissue_device()
+        tissues.append((srv.add_tissue(lab, vec), props, s))
+        del lab, vec
+    simplis = {}
+    for interp in ('Slerp', 'NN'):
+        s = make_simpli([8, 8, 8])
+        s.interpolate = interp
+        simplis[interp] = s
+    work_of = {(k, nt): sum(tissues[k][2].count_voxel_steps(t, p) for t, p in TILTS[:nt])
+               for k in range(len(SERVE_SIZES)) for nt in (1, 5)}
+
+    def make_request():
+        k = int(rng.integers(0, len(SERVE_SIZES)))
+        tilts = TILTS if rng.uniform() < 0.5 else TILTS[:1]
+        interp = 'Slerp' if rng.uniform() < 0.5 else 'NN'
+        props = np.array(tissues[k][1], dtype=np.float64)
+        props[1:, 0] *= rng.uniform(0.5, 1.5)
+        props[1:, 1] *= rng.uniform(0.5, 1.5)
+        return k, tilts, interp, props
+
+    # service time of each shape alone (warms the graph cache of every slot)
+    service = {}
+    for k in range(len(SERVE_SIZES)):
+        for nt, tilts in ((5, TILTS), (1, TILTS[:1])):
+            for interp in ('Slerp', 'NN'):
+                times = []
+                for _ in range(6):
+                    t = srv.submit(tissues[k][0], simplis[interp], tissues[k][1], tilts)
+                    _, lat = srv.wait(t, return_latency=True)
+                    times.append(lat)
+                service[(k, nt, interp)] = float(np.median(times[2:]))
+    mean_service = float(np.mean(list(service.values())))
+    rate = load / mean_service
+    reqs = [make_request() for _ in range(n_requests)]
+    gaps = rng.exponential(1.0 / rate, size=n_requests)
+    arrivals = np.cumsum(gaps)
+    outstanding, lats, work = [], [], 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    i = 0
+    while i < n_requests or outstanding:
+        now = time.perf_counter() - t0
+        while i < n_requests and arrivals[i] <= now:
+            k, tilts, interp, props = reqs[i]
+            outstanding.append(srv.submit(tissues[k][0], simplis[interp], props, tilts))
+            work += work_of[(k, len(tilts))]
+            i += 1
+        still = []
+        for t in outstanding:
+            try:
+                _, lat = srv.wait(t, timeout=0.0, return_latency=True)
+                lats.append(lat)
+            except TimeoutError:
+                still.append(t)
+        outstanding = still
+        if i < n_requests and not outstanding:
+            time.sleep(max(0.0, min(arrivals[i] - (time.perf_counter() - t0), 0.001)))
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lats = np.array(lats) * 1e3
+    return {
+        'requests': n_requests,
+        'p50_ms': float(np.percentile(lats, 50)),
+        'p99_ms': float(np.percentile(lats, 99)),
+        'mean_ms': float(lats.mean()),
+        'max_ms': float(lats.max()),
+        'arrival_rate_per_s': rate,
+        'offered_load': load,
+        'mean_service_alone_ms': mean_service * 1e3,
+        'throughput_requests_per_s': n_requests / wall,
+        'throughput_voxel_steps_per_s': work / wall,
+        'max_inflight': 4,
+        'graphs_captured': st['graphs'],
+        'graph_replays': st['replays'],
+        'mix': 'sizes 64^2x60 / 128^2x120 / 256^2x200 / 512^2x300 uniform; 5 tilts or 1 (1:1); '
+               'Slerp or NN (1:1); dn, mu x U(0.5, 1.5); 9 rotations; Poisson arrivals',
+    }
 
 
 def cpu_baseline(fbs, fraction, threads):
@@ -109,6 +218,11 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--field-storage', default='f32', choices=['f32', 'f16', 'angle16'],
                     help='storage of the fields inside the march (precision modes)')
+    ap.add_argument('--no-serving', action='store_true',
+                    help='skip the mixed-size serving stream (p50 / p99 latency)')
+    ap.add_argument('--serve-requests', type=int, default=1500)
+    ap.add_argument('--serve-load', type=float, default=0.7,
+                    help='arrival rate as a fraction of the one-at-a-time service rate')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='after timing, copy 1 GiB device to device once (the reference kernel '
                          'that scales FETCH_SIZE / WRITE_SIZE counters to bytes)')
@@ -201,6 +315,11 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
+    serving = None
+    if not args.no_serving:
+        log(f'serving stream: {args.serve_requests} requests at load {args.serve_load}')
+        serving = serving_bench(args.serve_requests, args.serve_load)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -238,6 +357,7 @@ def main():
         'roofline': roofline,
         'kernels': kernels,
         'cpu_baseline': cpu,
+        'serving': serving,
     }
     if cpu:
         out['speedup_vs_cpu_baseline'] = out['value'] / cpu['value']

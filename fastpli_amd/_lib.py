@@ -20,6 +20,7 @@ FPL_E_LABEL_MAX = -3
 FPL_E_BG_DN = -4
 FPL_E_DEVICE = -5
 FPL_E_NOMEM = -6
+FPL_E_PENDING = -7
 
 WARN_OVERLAP = 1 << 0
 WARN_SINGLE_POINT = 1 << 1
@@ -64,12 +65,19 @@ class FplSetup(ctypes.Structure):
                 ('field_storage', ctypes.c_int32)]
 
 
+class FplRequest(ctypes.Structure):
+    _fields_ = [('tissue', _c_i64), ('setup', ctypes.POINTER(FplSetup)), ('props', _c_p),
+                ('n_props', _c_i64), ('tilts', _c_p), ('n_tilts', _c_i64), ('images', _c_p)]
+
+
 _LIB = None
 
 # every symbol include/fastpli_amd.h declares
 EXPORTS = ('fpl_num_properties', 'fpl_generate', 'fpl_simulate', 'fpl_optic_resample', 'fpl_epa',
            'fpl_count_voxel_steps', 'fpl_profile_enable', 'fpl_profile_read',
-           'fpl_release_workspace', 'fpl_version')
+           'fpl_release_workspace', 'fpl_version', 'fpl_server_create', 'fpl_server_add_tissue',
+           'fpl_server_remove_tissue', 'fpl_server_submit', 'fpl_server_wait',
+           'fpl_server_stats', 'fpl_server_destroy')
 
 
 def lib():
@@ -116,6 +124,26 @@ def lib():
         L.fpl_release_workspace.argtypes = []
         L.fpl_version.restype = ctypes.c_char_p
         L.fpl_version.argtypes = []
+        msgargs = [ctypes.c_char_p, ctypes.c_size_t]
+        L.fpl_server_create.restype = ctypes.c_int
+        L.fpl_server_create.argtypes = [ctypes.c_int, _c_i64, ctypes.c_int32,
+                                        ctypes.POINTER(_c_p)] + msgargs
+        L.fpl_server_add_tissue.restype = ctypes.c_int
+        L.fpl_server_add_tissue.argtypes = [_c_p, _c_p, _c_p, _c_p, ctypes.c_int32,
+                                            ctypes.POINTER(_c_i64)] + msgargs
+        L.fpl_server_remove_tissue.restype = ctypes.c_int
+        L.fpl_server_remove_tissue.argtypes = [_c_p, _c_i64] + msgargs
+        L.fpl_server_submit.restype = ctypes.c_int
+        L.fpl_server_submit.argtypes = [_c_p, ctypes.POINTER(FplRequest),
+                                        ctypes.POINTER(_c_i64)] + msgargs
+        L.fpl_server_wait.restype = ctypes.c_int
+        L.fpl_server_wait.argtypes = [_c_p, _c_i64, ctypes.c_double,
+                                      ctypes.POINTER(ctypes.c_uint32),
+                                      ctypes.POINTER(ctypes.c_double)] + msgargs
+        L.fpl_server_stats.restype = ctypes.c_int
+        L.fpl_server_stats.argtypes = [_c_p, ctypes.POINTER(_c_i64)]
+        L.fpl_server_destroy.restype = ctypes.c_int
+        L.fpl_server_destroy.argtypes = [_c_p]
         _LIB = L
     return _LIB
 

@@ -37,6 +37,7 @@ SOURCES = {
     # tolerance-based FP32 chain: hardware reciprocal / rsqrt instead of the
     # correctly rounded division and square root sequences
     'fpl_simulate.hip': ['-fno-hip-fp32-correctly-rounded-divide-sqrt'],
+    'fpl_serve.hip': [],
 }
 
 

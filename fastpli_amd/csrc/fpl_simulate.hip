@@ -25,6 +25,7 @@
 #include <hip/hip_fp16.h>
 
 #include "fpl_internal.h"
+#include "fpl_sim.h"
 
 namespace fpl {
 namespace sim {
@@ -632,9 +633,12 @@ TiltDev make_tilt(const fpl_setup &S, const int64_t dim[3], double theta, double
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
-Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, const float *vector,
-           const double *props, int64_t n_props, const double *tilts, int64_t n_tilts,
-           double *images, uint32_t *warn, hipStream_t st) {
+size_t record_bytes(int storage) {
+   return storage == ST_F32 ? 16 : (storage == ST_F16 ? 8 : 2);
+}
+
+Status prepare(const fpl_setup *setup, const int64_t dim[3], const double *props, int64_t n_props,
+               const double *tilts, int64_t n_tilts, Block &B) {
    if (!setup || !dim || !props || !tilts) return make_status(FPL_E_INVALID, "null argument");
    const fpl_setup &S = *setup;
    // setup::Setup::Check (setup.hpp:48-71)
@@ -660,12 +664,19 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
       return make_status(FPL_E_INVALID, "volume dimension too large");
    if (n_props <= 0) return make_status(FPL_E_INVALID, "properties are empty");
    if (n_tilts <= 0 || n_tilts > 65535) return make_status(FPL_E_INVALID, "bad number of tilts");
-   if (!label || !vector || !images) return make_status(FPL_E_INVALID, "null array");
    // CheckInput (simulator.cpp:169-175)
    if (props[0] != 0) return make_status(FPL_E_BG_DN, "background birefringence has to be 0");
    for (int64_t l = 0; l < n_props; l++)
       if (props[2 * l + 1] < 0)
          return make_status(FPL_E_INVALID, "mu < 0: %f", props[2 * l + 1]);
+   {
+      const int64_t nyb = (dim[1] + 31) / 32, nzb = (dim[2] + 31) / 32;
+      if (dim[0] * nyb >= ((int64_t)1 << 31) || nzb > 65535)
+         return make_status(FPL_E_INVALID, "volume too large");
+      const int64_t n_tiles = ((dim[0] + RAYS_X - 1) / RAYS_X) * ((dim[1] + RAYS_Y - 1) / RAYS_Y);
+      if (n_tiles * n_tilts >= ((int64_t)1 << 31))
+         return make_status(FPL_E_INVALID, "too many rays");
+   }
 
    const double lambda = S.wavelength * 1e-9;
    const double thickness = S.voxel_size * 1e-6 * S.step_size;
@@ -704,68 +715,98 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
          return make_status(FPL_E_INVALID, "tilt theta = %f gives no progress along z",
                             tilts[2 * t]);
    }
+   B.off_t = (sizeof(Params) + 255) & ~size_t(255);
+   B.off_pr = (B.off_t + sizeof(TiltDev) * n_tilts + 255) & ~size_t(255);
+   B.bytes.assign(B.off_pr + sizeof(PropDev) * n_props, 0);
+   memcpy(B.bytes.data(), &prm, sizeof(Params));
+   memcpy(B.bytes.data() + B.off_t, tdev.data(), sizeof(TiltDev) * n_tilts);
+   memcpy(B.bytes.data() + B.off_pr, pdev.data(), sizeof(PropDev) * n_props);
+   B.dim[0] = dim[0];
+   B.dim[1] = dim[1];
+   B.dim[2] = dim[2];
+   B.n_tilts = n_tilts;
+   B.n_props = n_props;
+   B.n_rho = S.n_rho;
+   B.storage = S.field_storage;
+   return Status();
+}
 
-   const size_t sz_p = sizeof(Params);
-   const size_t off_t = (sz_p + 255) & ~size_t(255);
-   const size_t off_pr = (off_t + sizeof(TiltDev) * n_tilts + 255) & ~size_t(255);
-   const size_t bytes = off_pr + sizeof(PropDev) * n_props;
-   void *small, *stage, *flags;
-   FPL_TRY(ws_get(st, WS_SMALL, bytes, &small));
-   FPL_TRY(ws_get(st, WS_FLAGS, 64, &flags));
-   FPL_TRY(host_stage(st, bytes + 64, &stage));
-   FPL_HIP(hipStreamSynchronize(st));
-   memcpy((char *)stage, &prm, sz_p);
-   memcpy((char *)stage + off_t, tdev.data(), sizeof(TiltDev) * n_tilts);
-   memcpy((char *)stage + off_pr, pdev.data(), sizeof(PropDev) * n_props);
-   FPL_HIP(hipMemcpyAsync(small, stage, bytes, hipMemcpyHostToDevice, st));
-   uint32_t *d_flags = (uint32_t *)flags;
-   int32_t *d_mm = (int32_t *)((char *)flags + 16);
-
-   const int64_t n_vox = dim[0] * dim[1] * dim[2];
-   const int storage = S.field_storage;
-   const size_t rec_bytes = storage == ST_F32 ? 16 : (storage == ST_F16 ? 8 : 2);
-   void *packed, *packed_b = nullptr;
-   FPL_TRY(ws_get(st, WS_PACKED, rec_bytes * (size_t)n_vox, &packed));
-   if (storage == ST_ANGLE16) FPL_TRY(ws_get(st, WS_PACKED_B, 4 * (size_t)n_vox, &packed_b));
+void clear_flags(uint32_t *d_flags, int32_t *d_mm, hipStream_t st) {
    hipLaunchKernelGGL(k_init_flags, dim3(1), dim3(1), 0, st, d_flags, d_mm);
-   {
-      const int64_t nyb = (dim[1] + 31) / 32, nzb = (dim[2] + 31) / 32;
-      if (dim[0] * nyb >= ((int64_t)1 << 31) || nzb > 65535)
-         return make_status(FPL_E_INVALID, "volume too large");
-   }
-   KernelTimer kp("pack", st);
-   {
-      const dim3 grid((unsigned)(dim[0] * ((dim[1] + 31) / 32)), (unsigned)((dim[2] + 31) / 32));
-      if (storage == ST_F32)
-         hipLaunchKernelGGL(k_pack<ST_F32>, grid, dim3(256), 0, st, label, vector, dim[0], dim[1],
-                            dim[2], packed, (uint32_t *)packed_b, d_mm);
-      else if (storage == ST_F16)
-         hipLaunchKernelGGL(k_pack<ST_F16>, grid, dim3(256), 0, st, label, vector, dim[0], dim[1],
-                            dim[2], packed, (uint32_t *)packed_b, d_mm);
-      else
-         hipLaunchKernelGGL(k_pack<ST_ANGLE16>, grid, dim3(256), 0, st, label, vector, dim[0],
-                            dim[1], dim[2], packed, (uint32_t *)packed_b, d_mm);
-   }
-   kp.stop();
-   const int64_t n_tiles = ((dim[0] + RAYS_X - 1) / RAYS_X) * ((dim[1] + RAYS_Y - 1) / RAYS_Y);
-   if (n_tiles * n_tilts >= ((int64_t)1 << 31)) return make_status(FPL_E_INVALID, "too many rays");
-   KernelTimer kt("march", st);
-   const Params *d_p = (const Params *)small;
-   const TiltDev *d_t = (const TiltDev *)((char *)small + off_t);
-   const PropDev *d_pr = (const PropDev *)((char *)small + off_pr);
-   const dim3 mgrid((unsigned)(n_tiles * n_tilts));
-   const bool small_idx = n_vox < ((int64_t)1 << 32);
+}
+
+void pack(int storage, const int64_t dim[3], const int32_t *label, const float *vector,
+          void *packed, uint32_t *packed_b, int32_t *d_mm, hipStream_t st) {
+   const dim3 grid((unsigned)(dim[0] * ((dim[1] + 31) / 32)), (unsigned)((dim[2] + 31) / 32));
+   if (storage == ST_F32)
+      hipLaunchKernelGGL(k_pack<ST_F32>, grid, dim3(256), 0, st, label, vector, dim[0], dim[1],
+                         dim[2], packed, packed_b, d_mm);
+   else if (storage == ST_F16)
+      hipLaunchKernelGGL(k_pack<ST_F16>, grid, dim3(256), 0, st, label, vector, dim[0], dim[1],
+                         dim[2], packed, packed_b, d_mm);
+   else
+      hipLaunchKernelGGL(k_pack<ST_ANGLE16>, grid, dim3(256), 0, st, label, vector, dim[0],
+                         dim[1], dim[2], packed, packed_b, d_mm);
+}
+
+void march(const Block &B, const void *d_block, const void *packed, const uint32_t *packed_b,
+           double *images, uint32_t *d_flags, hipStream_t st) {
+   const int64_t n_tiles =
+       ((B.dim[0] + RAYS_X - 1) / RAYS_X) * ((B.dim[1] + RAYS_Y - 1) / RAYS_Y);
+   const Params *d_p = (const Params *)d_block;
+   const TiltDev *d_t = (const TiltDev *)((const char *)d_block + B.off_t);
+   const PropDev *d_pr = (const PropDev *)((const char *)d_block + B.off_pr);
+   const dim3 mgrid((unsigned)(n_tiles * B.n_tilts));
+   const bool small_idx = B.dim[0] * B.dim[1] * B.dim[2] < ((int64_t)1 << 32);
 #define FPL_MARCH(IDX, STV)                                                                       \
    hipLaunchKernelGGL((k_march<IDX, STV>), mgrid, dim3(WG), 0, st, d_p, d_t, d_pr, packed,       \
-                      (const uint32_t *)packed_b, images, d_flags)
-   if (storage == ST_F32) {
+                      packed_b, images, d_flags)
+   if (B.storage == ST_F32) {
       if (small_idx) FPL_MARCH(uint32_t, ST_F32); else FPL_MARCH(int64_t, ST_F32);
-   } else if (storage == ST_F16) {
+   } else if (B.storage == ST_F16) {
       if (small_idx) FPL_MARCH(uint32_t, ST_F16); else FPL_MARCH(int64_t, ST_F16);
    } else {
       if (small_idx) FPL_MARCH(uint32_t, ST_ANGLE16); else FPL_MARCH(int64_t, ST_ANGLE16);
    }
 #undef FPL_MARCH
+}
+
+Status check_labels(int32_t lmin, int32_t lmax, int64_t n_props, int storage) {
+   if (lmin < 0) return make_status(FPL_E_LABEL_NEG, "label < 0 detected");
+   if ((int64_t)lmax >= n_props) return make_status(FPL_E_LABEL_MAX, "label exceed properties.size()");
+   if ((storage == ST_F16 && lmax > 0xffff) || (storage == ST_ANGLE16 && lmax > 0x7fff))
+      return make_status(FPL_E_INVALID, "label %d does not fit the %s field storage", lmax,
+                         storage == ST_F16 ? "16-bit" : "15-bit");
+   return Status();
+}
+
+Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, const float *vector,
+           const double *props, int64_t n_props, const double *tilts, int64_t n_tilts,
+           double *images, uint32_t *warn, hipStream_t st) {
+   Block B;
+   FPL_TRY(prepare(setup, dim, props, n_props, tilts, n_tilts, B));
+   if (!label || !vector || !images) return make_status(FPL_E_INVALID, "null array");
+
+   void *small, *stage, *flags;
+   FPL_TRY(ws_get(st, WS_SMALL, B.bytes.size(), &small));
+   FPL_TRY(ws_get(st, WS_FLAGS, 64, &flags));
+   FPL_TRY(host_stage(st, B.bytes.size() + 64, &stage));
+   FPL_HIP(hipStreamSynchronize(st));
+   memcpy(stage, B.bytes.data(), B.bytes.size());
+   FPL_HIP(hipMemcpyAsync(small, stage, B.bytes.size(), hipMemcpyHostToDevice, st));
+   uint32_t *d_flags = (uint32_t *)flags;
+   int32_t *d_mm = (int32_t *)((char *)flags + 16);
+
+   const int64_t n_vox = dim[0] * dim[1] * dim[2];
+   void *packed, *packed_b = nullptr;
+   FPL_TRY(ws_get(st, WS_PACKED, record_bytes(B.storage) * (size_t)n_vox, &packed));
+   if (B.storage == ST_ANGLE16) FPL_TRY(ws_get(st, WS_PACKED_B, 4 * (size_t)n_vox, &packed_b));
+   clear_flags(d_flags, d_mm, st);
+   KernelTimer kp("pack", st);
+   pack(B.storage, dim, label, vector, packed, (uint32_t *)packed_b, d_mm, st);
+   kp.stop();
+   KernelTimer kt("march", st);
+   march(B, small, packed, (const uint32_t *)packed_b, images, d_flags, st);
    kt.stop();
    FPL_HIP(hipGetLastError());
    FPL_HIP(hipMemcpyAsync(stage, flags, 32, hipMemcpyDeviceToHost, st));
@@ -775,11 +816,7 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
    const uint32_t f = *(uint32_t *)stage;
    const int32_t lmin = ((int32_t *)((char *)stage + 16))[0];
    const int32_t lmax = ((int32_t *)((char *)stage + 16))[1];
-   if (lmin < 0) return make_status(FPL_E_LABEL_NEG, "label < 0 detected");
-   if ((int64_t)lmax >= n_props) return make_status(FPL_E_LABEL_MAX, "label exceed properties.size()");
-   if ((storage == ST_F16 && lmax > 0xffff) || (storage == ST_ANGLE16 && lmax > 0x7fff))
-      return make_status(FPL_E_INVALID, "label %d does not fit the %s field storage", lmax,
-                         storage == ST_F16 ? "16-bit" : "15-bit");
+   FPL_TRY(check_labels(lmin, lmax, n_props, B.storage));
    if (warn) *warn = f & 0x7fffffffu;
    return Status();
 }

@@ -1,3 +1,4 @@
+from .serving import SimulationServer
 from .simpli import Simpli
 
-__all__ = ['Simpli']
+__all__ = ['Simpli', 'SimulationServer']

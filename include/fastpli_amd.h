@@ -39,6 +39,7 @@ extern "C" {
 #define FPL_E_BG_DN (-4)     /* "background birefringence has to be 0" (:169) */
 #define FPL_E_DEVICE (-5)    /* HIP runtime failure or no device              */
 #define FPL_E_NOMEM (-6)     /* device allocation failed                      */
+#define FPL_E_PENDING (-7)   /* fpl_server_wait: not finished within timeout */
 
 /* ----------------------------------------------------------- warnings --- */
 #define FPL_WARN_OVERLAP (1u << 0)      /* "objects overlap" generator.cpp:357      */
@@ -161,6 +162,65 @@ int64_t fpl_count_voxel_steps(const fpl_setup *setup, const int64_t dim[3],
  * (it has no profiling); used by bench.py for the roofline figure. */
 void fpl_profile_enable(int enable);
 int fpl_profile_read(const char *name, double *total_ms, int64_t *launches);
+
+/* ------------------------------------------------------------ serving
+ * A device-resident request loop for many simulations at once.  No
+ * counterpart in the reference (it runs one simulation per process call, MPI
+ * ranks sharing one volume); it serves callers that stream run_simulation
+ * requests of different volume sizes, tilt sets and layer parameters.
+ *
+ * Tissues are registered once: their label/axis fields are packed into the
+ * march's record layout on the device and stay resident.  A request names a
+ * tissue, a setup (fpl_setup; the tissue's field_storage applies), a property table
+ * and a tilt list, and a DEVICE output buffer.  A worker thread admits pending
+ * requests in arrival order whenever a stream slot is free and the resident
+ * tissues plus the device footprint of the requests in flight plus the new one
+ * fit the HBM budget; admitted requests run concurrently, each on its slot's
+ * stream as a captured hipGraph (parameter upload, flag reset, march, warning
+ * readback) cached per (tissue, tilt count, property count, rotation count)
+ * and replayed, followed by a copy of the slot's images into the request's
+ * buffer.  Results are identical to fpl_simulate's for the same inputs.
+ *
+ * The server owns its streams, slot buffers and graphs; the caller owns the
+ * request's output buffer, which must stay valid until fpl_server_wait has
+ * returned FPL_OK (or an error) for the ticket.  All functions are thread safe. */
+typedef struct fpl_server fpl_server;
+
+typedef struct fpl_request {
+   int64_t tissue;          /* id returned by fpl_server_add_tissue */
+   const fpl_setup *setup;  /* HOST; copied at submit */
+   const double *props;     /* HOST float64 (n_props, 2); copied at submit */
+   int64_t n_props;
+   const double *tilts;     /* HOST float64 (n_tilts, 2) = (theta, phi); copied */
+   int64_t n_tilts;
+   double *images;          /* DEVICE float64 (n_tilts, nx, ny, n_rho) */
+} fpl_request;
+
+/* hbm_budget in bytes (<= 0: 90% of the device's free memory at creation);
+ * max_inflight concurrent requests (stream slots, 1..16). */
+int fpl_server_create(int device, int64_t hbm_budget, int32_t max_inflight, fpl_server **server,
+                      char *msg, size_t msg_len);
+/* label DEVICE int32 (dim...), vector DEVICE float (dim..., 3); packed and
+ * range-checked before returning, after which the caller may free them. */
+int fpl_server_add_tissue(fpl_server *server, const int64_t dim[3], const int32_t *label,
+                          const float *vector, int32_t field_storage, int64_t *tissue,
+                          char *msg, size_t msg_len);
+/* Frees a tissue's resident fields; FPL_E_INVALID while requests use it. */
+int fpl_server_remove_tissue(fpl_server *server, int64_t tissue, char *msg, size_t msg_len);
+/* Validates the request (the checks of fpl_simulate) and queues it. */
+int fpl_server_submit(fpl_server *server, const fpl_request *request, int64_t *ticket,
+                      char *msg, size_t msg_len);
+/* Waits up to timeout_s (< 0: no limit) for a ticket.  FPL_OK: the images are
+ * written, warn receives the FPL_WARN_* mask and latency_s the time from
+ * submit to completion; the ticket is then retired.  FPL_E_PENDING: not
+ * finished yet.  Any other code: the request failed (retired). */
+int fpl_server_wait(fpl_server *server, int64_t ticket, double timeout_s, uint32_t *warn,
+                    double *latency_s, char *msg, size_t msg_len);
+/* stats[0..5] = submitted, completed, in flight, pending, graphs captured,
+ * graph replays. */
+int fpl_server_stats(fpl_server *server, int64_t stats[6]);
+/* Finishes the requests in flight, fails the pending ones, frees everything. */
+int fpl_server_destroy(fpl_server *server);
 
 /* Release the library's cached device workspace of the current device. */
 void fpl_release_workspace(void);
