@@ -44,6 +44,7 @@ constexpr int SZ = 8;                  // slab height
 constexpr int SLOTS = 64 * SZ;         // voxels of a slab: slot = (z - z_slab) * 64 + column
 constexpr int CHUNK = 64;              // segment records staged in LDS at a time
 constexpr int QN = 128;                // candidate ring of a wave (never holds 128 entries)
+constexpr int LSQ_LDS = 256;           // layer tables up to this many rows are read from LDS
 
 struct SegRec {        // 128 bytes, one per segment (indexed by its first point)
    double p[3];        // start point, voxel units
@@ -55,7 +56,7 @@ struct SegRec {        // 128 bytes, one per segment (indexed by its first point
    float bh[3];        // segment's box in the frame where voxel i has its centre at i
    float thr;          // f_last * float(max(rp, rq)^2): outer-layer test at the larger radius
    float f_last;       // float(last scale^2) of the bundle
-   int32_t label_base; // 1 + bundle * max_layer
+   int32_t layer_off;  // the bundle's rows in the (sentinel-padded) layer tables
    int32_t bundle;
 };
 static_assert(sizeof(SegRec) == 128, "SegRec layout");
@@ -77,6 +78,7 @@ struct Params {
    double scale;   // 1.0 / voxel_size
    int32_t max_layer;
    int32_t ntx, nty, ntz;
+   int32_t n_lsq; // rows of the layer tables
 };
 
 // aabb::AABB(p, q) unsorted constructor (aabb.hpp:54-71)
@@ -300,7 +302,7 @@ __global__ void k_segments(Params pr, const double *__restrict__ P,
          }
          R.f_last = bd[b].f_last_sq;
          R.thr = R.f_last * (float)(rmax * rmax);
-         R.label_base = bd[b].label_base;
+         R.layer_off = bd[b].layer_off;
          R.bundle = b;
          segs[i] = R;
          const int64_t tx0 = ilo[0] / TX, tx1 = (ihi[0] - 1) / TX;
@@ -457,6 +459,7 @@ struct WaveState {
    unsigned long long key[SLOTS];
    int32_t seg[SLOTS];  // global index of the stored segment, -1 none
    uint32_t queue[QN];  // candidates (chunk index << 9 | slot), ring
+   uint2 tab[64];       // per chunk record: candidate prefix | box << 16, division magics
 };
 
 struct alignas(16) LdsSeg { // 144-byte stride spreads records over the LDS banks
@@ -469,9 +472,7 @@ struct alignas(16) LdsSeg { // 144-byte stride spreads records over the LDS bank
 __device__ __forceinline__ void run_batch(WaveState &W, const LdsSeg *chunk,
                                           const uint32_t *chunk_idx, int n, int q_head,
                                           int lane, int xt, int yt, int zw, int max_layer,
-                                          const BundleDev *__restrict__ bd,
-                                          const double *__restrict__ layer_sq, uint32_t *flags,
-                                          bool &ov) {
+                                          const double *lsq, uint32_t *flags, bool &ov) {
    const bool valid = lane < n;
    const uint32_t e = valid ? W.queue[(q_head + lane) & (QN - 1)] : 0u;
    const int c = (int)(e >> 9), slot = (int)(e & (SLOTS - 1));
@@ -501,13 +502,13 @@ __device__ __forceinline__ void run_batch(WaveState &W, const LdsSeg *chunk,
       acc = !(dist_sq > f_last * r_sq);
       if (acc) {
          // std::lower_bound over the bundle's double scale^2 list with the
-         // float ratio (generator.cpp:345-349)
-         const BundleDev &B = bd[R.bundle];
+         // float ratio (generator.cpp:345-349); the list ends with a +inf
+         // sentinel, so the search stops at n_layers at the latest
          const double ratio = (double)(dist_sq / r_sq);
-         const int off = B.layer_off, nl = B.n_layers;
+         const double *L = lsq + R.layer_off;
          int ly = 0;
-         while (ly < nl && layer_sq[off + ly] < ratio) ly++;
-         new_label = ly + R.label_base;
+         while (L[ly] < ratio) ly++;
+         new_label = ly + 1 + R.bundle * max_layer;
       }
    }
    unsigned long long old = KEY_EMPTY;
@@ -565,13 +566,15 @@ __device__ __forceinline__ void run_batch(WaveState &W, const LdsSeg *chunk,
    wave_sync();
 }
 
-// One workgroup per 8x8x32 tile, wave w owning the slab z in [8w, 8w + 8),
-// lane l the voxel column (l / 8, l % 8).  The tile's segments (global order)
-// are staged in LDS in chunks.  Stage A tests every (segment, slab voxel) of
-// the segment's integer bbox with a float lower bound of the squared distance
-// (distance to the segment's box) against the outer-layer threshold and the
-// voxel's stored distance, with margins that keep every candidate the exact
-// test could accept; survivors enter the wave's queue in order.  Stage B
+// One workgroup per 8x8x32 tile, wave w owning the slab z in [8w, 8w + 8)
+// (slot = z * 64 + column, column = x * 8 + y within the tile).  The tile's
+// segments (global order) are staged in LDS in chunks of 64.  Stage A: one
+// lane per record computes the record's box within the slab; the candidate
+// voxels of all records are then enumerated 64 at a time in record order, and
+// each is tested with a float lower bound of its squared distance (distance
+// to the segment's box) against the outer-layer threshold and the voxel's
+// stored distance, with margins that keep every candidate the exact test
+// could accept; survivors enter the wave's queue in order.  Stage B
 // (run_batch) runs the exact FP64 test on 64 queued candidates at a time.
 __global__ void __launch_bounds__(WG)
     k_voxelize(Params pr, const SegRec *__restrict__ segs, const uint32_t *__restrict__ list,
@@ -593,11 +596,16 @@ __global__ void __launch_bounds__(WG)
    const int lane = threadIdx.x & 63;
    const int xt = tx * TX, yt = ty * TY;
    const int zw = tz * TZ + wave * SZ;
-   const int x = xt + (lane >> 3), y = yt + (lane & 7);
-   const float xf = (float)x, yf = (float)y;
    const int max_layer = pr.max_layer;
    // float margins of the stage A test: relative 2^-12, absolute 2^-6
    const float rel = 1.0f + 0x1p-12f, absm = 0x1p-6f;
+
+   // the layer tables in LDS when small (read through a generic pointer)
+   __shared__ double lsq_lds[LSQ_LDS];
+   const bool lsq_in_lds = pr.n_lsq <= LSQ_LDS;
+   if (lsq_in_lds)
+      for (int i = threadIdx.x; i < pr.n_lsq; i += WG) lsq_lds[i] = layer_sq[i];
+   const double *lsq = lsq_in_lds ? (const double *)lsq_lds : layer_sq;
 
    WaveState &W = wstate[wave];
    for (int s = lane; s < SLOTS; s += 64) {
@@ -609,7 +617,6 @@ __global__ void __launch_bounds__(WG)
    int q_head = 0, q_tail = 0;
    wave_sync();
 
-   __shared__ unsigned long long slab_mask[WG / 64]; // chunk records touching each slab
    const uint32_t beg = range[2 * t], end = range[2 * t + 1];
    for (uint32_t base = beg; base < end; base += CHUNK) {
       const int n = (int)((end - base) < (uint32_t)CHUNK ? (end - base) : (uint32_t)CHUNK);
@@ -620,75 +627,106 @@ __global__ void __launch_bounds__(WG)
          ((int4 *)&chunk[c].r)[piece] = ((const int4 *)&segs[sidx])[piece];
          if (piece == 0) chunk_idx[c] = sidx;
       }
-      if (wave == 0) { // which records reach into which slab
-         int zlo = 0, zhi = 0;
-         if (lane < n) {
-            const SegRec &G = segs[list[base + lane]];
-            zlo = G.lo[2];
-            zhi = G.hi[2];
-         }
-         for (int w = 0; w < WG / 64; w++) {
-            const int s0 = tz * TZ + w * SZ;
-            const unsigned long long m = __ballot(lane < n && zlo < s0 + SZ && zhi > s0);
-            if (lane == 0) slab_mask[w] = m;
+      __syncthreads();
+
+      // Stage A set-up, one lane per record of the chunk: the record's integer
+      // box within the slab and its z range narrowed by the nearest columns of
+      // the tile.  The in-box columns nearest to the segment's box bound every
+      // voxel's float ex and ey from below (float rounding is monotone), so a
+      // voxel that passes the float radius test RN(ez^2 + exy) <= thr_m has
+      // ez^2 <= rem + ulp(thr_m), rem = thr_m - exy_min >= 0; h covers that
+      // with margin (the hardware square root is within 1 ulp), and the range
+      // ends are computed in double, exact for any float box and voxel index.
+      int cnt = 0;
+      uint32_t geo = 0, magic = 0;
+      if (lane < n) {
+         const SegRec &R = chunk[lane].r;
+         const int z0 = max(R.lo[2], zw), z1 = min(R.hi[2], zw + SZ);
+         const int cx0 = max(R.lo[0], xt), cx1 = min(R.hi[0], xt + TX);
+         const int cy0 = max(R.lo[1], yt), cy1 = min(R.hi[1], yt + TY);
+         if (z0 < z1 && cx0 < cx1 && cy0 < cy1) {
+            const float thr_m = fmaf(R.thr, rel, absm);
+            const float exm =
+                fmaxf(fmaxf(R.bl[0] - (float)(cx1 - 1), (float)cx0 - R.bh[0]), 0.0f);
+            const float eym =
+                fmaxf(fmaxf(R.bl[1] - (float)(cy1 - 1), (float)cy0 - R.bh[1]), 0.0f);
+            const float rem = thr_m - fmaf(eym, eym, exm * exm);
+            if (rem >= 0.0f) {
+               const float h = fmaf(__builtin_amdgcn_sqrtf(fmaf(thr_m, 0x1p-20f, rem)),
+                                    1.0f + 0x1p-10f, 0x1p-10f);
+               const int za = max(z0, (int)ceil((double)R.bl[2] - (double)h));
+               const int zb = min(z1, (int)floor((double)R.bh[2] + (double)h) + 1);
+               if (za < zb) {
+                  const int ncy = cy1 - cy0, nzz = zb - za;
+                  cnt = (cx1 - cx0) * ncy * nzz;
+                  geo = (uint32_t)(cx0 - xt) | (uint32_t)(cy0 - yt) << 3 |
+                        (uint32_t)(za - zw) << 6 | (uint32_t)(ncy - 1) << 9 |
+                        (uint32_t)(nzz - 1) << 12;
+                  // floor(i / n) = (i * ceil(2^15 / n)) >> 15 for i < 512, n <= 8
+                  magic = (uint32_t)(32767 / nzz + 1) | (uint32_t)(32767 / ncy + 1) << 16;
+               }
+            }
          }
       }
-      __syncthreads();
-      unsigned long long todo = slab_mask[wave];
-      todo = ((unsigned long long)uniform((int)(uint32_t)(todo >> 32)) << 32) |
-             (uint32_t)uniform((int)(uint32_t)todo);
-      while (todo) {
-         const int c = __builtin_ctzll(todo);
-         todo &= todo - 1;
-         const SegRec &R = chunk[c].r;
-         const int lo0 = uniform(R.lo[0]), hi0 = uniform(R.hi[0]);
-         const int lo1 = uniform(R.lo[1]), hi1 = uniform(R.hi[1]);
-         const int z0 = max(uniform(R.lo[2]), zw), z1 = min(uniform(R.hi[2]), zw + SZ);
-         const float bl0 = uniform(R.bl[0]), bh0 = uniform(R.bh[0]);
-         const float bl1 = uniform(R.bl[1]), bh1 = uniform(R.bh[1]);
-         const float bl2 = uniform(R.bl[2]), bh2 = uniform(R.bh[2]);
-         const float thr_m = fmaf(uniform(R.thr), rel, absm);
-         // Uniform z range.  The in-bbox columns nearest to the segment's box
-         // bound every lane's float ex and ey from below (float rounding is
-         // monotone), so a voxel that passes the float radius test
-         // RN(ez^2 + exy) <= thr_m has ez^2 <= rem + ulp(thr_m), rem =
-         // thr_m - exy_min >= 0.  h covers that with margin, and the range ends
-         // are computed in double, exact for any float box and voxel index.
-         const int cx0 = max(lo0, xt), cx1 = min(hi0, xt + TX) - 1;
-         const int cy0 = max(lo1, yt), cy1 = min(hi1, yt + TY) - 1;
-         const float exm = fmaxf(fmaxf(bl0 - (float)cx1, (float)cx0 - bh0), 0.0f);
-         const float eym = fmaxf(fmaxf(bl1 - (float)cy1, (float)cy0 - bh1), 0.0f);
-         const float rem = thr_m - fmaf(eym, eym, exm * exm);
-         if (!(rem >= 0.0f)) continue;
-         // hardware square root (1 ulp), covered by the relative margin
-         const float h =
-             fmaf(__builtin_amdgcn_sqrtf(fmaf(thr_m, 0x1p-20f, rem)), 1.0f + 0x1p-10f, 0x1p-10f);
-         const int za = max(z0, uniform((int)ceil((double)bl2 - (double)h)));
-         const int zb = min(z1, uniform((int)floor((double)bh2 + (double)h)) + 1);
-         const bool in_xy = (x >= lo0) & (x < hi0) & (y >= lo1) & (y < hi1);
-         const float ex = fmaxf(fmaxf(bl0 - xf, xf - bh0), 0.0f);
-         const float ey = fmaxf(fmaxf(bl1 - yf, yf - bh1), 0.0f);
-         const float exy = fmaf(ey, ey, ex * ex);
-         for (int z = za; z < zb; z++) {
-            const int slot = (z - zw) * 64 + lane;
-            const float zf = (float)z;
-            const float ez = fmaxf(fmaxf(bl2 - zf, zf - bh2), 0.0f);
-            const float lb = fmaf(ez, ez, exy);
+      // candidates in record order: exclusive prefix of the counts
+      int incl = cnt;
+      for (int off = 1; off < 64; off <<= 1) {
+         const int v = __shfl_up(incl, off);
+         if (lane >= off) incl += v;
+      }
+      const int P = incl - cnt;
+      const int total = uniform(__shfl(incl, 63));
+      W.tab[lane] = make_uint2((uint32_t)P | geo << 16, magic);
+      wave_sync();
+      int cur = 0; // record holding the entry before the window
+      for (int wb = 0; wb < total; wb += 64) {
+         const int e = wb + lane;
+         // records whose candidates start inside the window, in order
+         unsigned long long st = __ballot((cnt > 0) & (P >= wb) & (P < wb + 64));
+         st = ((unsigned long long)(uint32_t)uniform((int)(uint32_t)(st >> 32)) << 32) |
+              (uint32_t)uniform((int)(uint32_t)st);
+         int c = cur;
+         while (st) {
+            const int s0 = __builtin_ctzll(st);
+            st &= st - 1;
+            if (e >= __builtin_amdgcn_readlane(P, s0)) c = s0;
+            cur = s0;
+         }
+         bool pass = false;
+         uint32_t entry = 0;
+         if (e < total) {
+            const uint2 tb = W.tab[c];
+            const uint32_t g = tb.x >> 16;
+            const int li = e - (int)(tb.x & 0xffffu);
+            const int q1 = (int)(((uint32_t)li * (tb.y & 0xffffu)) >> 15);
+            const int iz = li - q1 * (int)(((g >> 12) & 7) + 1);
+            const int ix = (int)(((uint32_t)q1 * (tb.y >> 16)) >> 15);
+            const int iy = q1 - ix * (int)(((g >> 9) & 7) + 1);
+            const int lx = (int)(g & 7) + ix, ly = (int)((g >> 3) & 7) + iy;
+            const int lz = (int)((g >> 6) & 7) + iz;
+            const int slot = lz * 64 + (lx << 3 | ly);
+            const SegRec &R = chunk[c].r;
+            const float xf = (float)(xt + lx), yf = (float)(yt + ly), zf = (float)(zw + lz);
+            const float ex = fmaxf(fmaxf(R.bl[0] - xf, xf - R.bh[0]), 0.0f);
+            const float ey = fmaxf(fmaxf(R.bl[1] - yf, yf - R.bh[1]), 0.0f);
+            const float ez = fmaxf(fmaxf(R.bl[2] - zf, zf - R.bh[2]), 0.0f);
+            const float lb = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
             const float stored = __uint_as_float((uint32_t)(W.key[slot] >> 32));
-            const bool pass = in_xy & (lb <= thr_m) & (lb <= fmaf(stored, rel, absm));
-            const unsigned long long m = __ballot(pass);
-            if (m) {
-               const int pos = q_tail + (int)__builtin_amdgcn_mbcnt_hi(
-                                            (uint32_t)(m >> 32),
-                                            __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-               if (pass) W.queue[pos & (QN - 1)] = ((uint32_t)c << 9) | (uint32_t)slot;
-               q_tail += __popcll(m);
-               if (q_tail - q_head >= 64) {
-                  wave_sync();
-                  run_batch(W, chunk, chunk_idx, 64, q_head, lane, xt, yt, zw, max_layer, bd,
-                            layer_sq, flags, ov);
-                  q_head += 64;
-               }
+            pass = (lb <= fmaf(R.thr, rel, absm)) & (lb <= fmaf(stored, rel, absm));
+            entry = ((uint32_t)c << 9) | (uint32_t)slot;
+         }
+         const unsigned long long m = __ballot(pass);
+         if (m) {
+            const int pos = q_tail + (int)__builtin_amdgcn_mbcnt_hi(
+                                         (uint32_t)(m >> 32),
+                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (pass) W.queue[pos & (QN - 1)] = entry;
+            q_tail += __popcll(m);
+            if (q_tail - q_head >= 64) {
+               wave_sync();
+               run_batch(W, chunk, chunk_idx, 64, q_head, lane, xt, yt, zw, max_layer, lsq,
+                         flags, ov);
+               q_head += 64;
             }
          }
       }
@@ -696,8 +734,8 @@ __global__ void __launch_bounds__(WG)
       wave_sync();
       while (q_tail > q_head) {
          const int nb = min(64, q_tail - q_head);
-         run_batch(W, chunk, chunk_idx, nb, q_head, lane, xt, yt, zw, max_layer, bd, layer_sq,
-                   flags, ov);
+         run_batch(W, chunk, chunk_idx, nb, q_head, lane, xt, yt, zw, max_layer, lsq, flags,
+                   ov);
          q_head += nb;
       }
       if (!ov)
@@ -720,12 +758,11 @@ __global__ void __launch_bounds__(WG)
          float vo[3] = {0.0f, 0.0f, 0.0f};
          if (sg >= 0) {
             const SegRec &R = segs[sg];
-            const BundleDev B = bd[R.bundle];
-            int ly = lab - R.label_base;
             // a float ratio just above the last scale^2 gives the past-the-end
-            // layer index; its orientation is read as the outermost layer's
-            if (ly > B.n_layers - 1) ly = B.n_layers - 1;
-            const uint8_t m = layer_model[B.layer_off + ly];
+            // layer index; the padded model table repeats the outermost
+            // layer's orientation there
+            const int ly = lab - (1 + R.bundle * max_layer);
+            const uint8_t m = layer_model[R.layer_off + ly];
             if (m != 'b') {
                const double p[3] = {R.p[0], R.p[1], R.p[2]};
                const double q[3] = {R.q[0], R.q[1], R.q[2]};
@@ -800,6 +837,12 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
       B.sqrt_last = std::sqrt(last);
       B.fiber_begin = fbs->bundle_ptr[b];
       B.fiber_end = fbs->bundle_ptr[b + 1];
+      // one padding row per bundle: a +inf scale^2 ends the layer search and
+      // the past-the-end layer index reads the outermost layer's orientation
+      lsq.push_back(INFINITY);
+      lmodel.push_back(B.n_layers ? lmodel.back() : (uint8_t)'b');
+      ldn.push_back(0.0);
+      lmu.push_back(0.0);
    }
    for (int64_t b = 0; b < nb; b++) bdev[b].label_base = (int32_t)(1 + b * max_layer);
    if (lsq.empty()) {
@@ -830,6 +873,7 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
    for (int k = 0; k < 3; k++) pr.off[k] = -vol->origin[k];
    pr.scale = 1.0 / vol->voxel_size;
    pr.max_layer = max_layer;
+   pr.n_lsq = (int32_t)lsq.size();
    pr.ntx = (int32_t)((nx + TX - 1) / TX);
    pr.nty = (int32_t)((ny + TY - 1) / TY);
    pr.ntz = (int32_t)((nz + TZ - 1) / TZ);
