@@ -77,7 +77,7 @@ EXPORTS = ('fpl_num_properties', 'fpl_generate', 'fpl_simulate', 'fpl_optic_resa
            'fpl_count_voxel_steps', 'fpl_profile_enable', 'fpl_profile_read',
            'fpl_release_workspace', 'fpl_version', 'fpl_server_create', 'fpl_server_add_tissue',
            'fpl_server_remove_tissue', 'fpl_server_submit', 'fpl_server_wait',
-           'fpl_server_stats', 'fpl_server_destroy')
+           'fpl_server_stats', 'fpl_server_destroy', 'fpl_rofl')
 
 
 def lib():
@@ -125,6 +125,9 @@ def lib():
         L.fpl_version.restype = ctypes.c_char_p
         L.fpl_version.argtypes = []
         msgargs = [ctypes.c_char_p, ctypes.c_size_t]
+        L.fpl_rofl.restype = ctypes.c_int
+        L.fpl_rofl.argtypes = [_c_p, _c_i64, _c_i64, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_double, _c_p, _c_p, _c_p] + msgargs
         L.fpl_server_create.restype = ctypes.c_int
         L.fpl_server_create.argtypes = [ctypes.c_int, _c_i64, ctypes.c_int32,
                                         ctypes.POINTER(_c_p)] + msgargs
@@ -262,4 +265,14 @@ def epa(images, t, d, r):
     n_pix = images.numel() // nr
     msg = ctypes.create_string_buffer(512)
     rc = lib().fpl_epa(_ptr(images), n_pix, nr, _ptr(t), _ptr(d), _ptr(r), _stream(), msg, 512)
+    check(rc, msg)
+
+
+def rofl(images, tilt_angle, gain, dir_offset, mask, out):
+    """fpl_rofl on a float64 device tensor (5, ..., n_rho) -> out (8, ...)."""
+    nr = images.shape[-1]
+    n_pix = images.numel() // (5 * nr)
+    msg = ctypes.create_string_buffer(512)
+    rc = lib().fpl_rofl(_ptr(images), n_pix, nr, float(tilt_angle), float(gain),
+                        float(dir_offset), _ptr(mask), _ptr(out), _stream(), msg, 512)
     check(rc, msg)

@@ -38,6 +38,7 @@ SOURCES = {
     # correctly rounded division and square root sequences
     'fpl_simulate.hip': ['-fno-hip-fp32-correctly-rounded-divide-sqrt'],
     'fpl_serve.hip': [],
+    'fpl_rofl.hip': ['-ffp-contract=off'],
 }
 
 

@@ -318,6 +318,58 @@ class Simpli(_SimpliBase):
                 o[~m] = float('nan')
         return tuple(out)
 
+    def apply_rofl(self, data, mask=None, mp_pool=None, grad_mode=False):
+        """Tilt analysis: direction, inclination and t_rel per pixel (__simpli.py:623-706).
+
+        data: (5, x, y, rho) images of the tilts (0, 0), (tau, 0), (tau, 90 deg),
+        (tau, 180 deg), (tau, 270 deg).  Returns (directionmap, inclmap, trelmap,
+        (dirdevmap, incldevmap, treldevmap, funcmap, itermap)).  Runs one GPU
+        thread per pixel in float64 (Levenberg-Marquardt with the exact
+        Jacobian); the results agree with the reference's MINPACK fit to its
+        convergence tolerance, and itermap counts this fit's model
+        evaluations.  ``mp_pool`` and ``grad_mode`` are accepted for API
+        compatibility (the fit always uses the exact Jacobian).
+        """
+        self._print('Apply rofl')
+        if self._tilts is None:
+            raise ValueError('tilts not set')
+        tilts = np.asarray(self._tilts)
+        if (tilts.shape != (5, 2) or np.any(tilts[:, 1] != np.deg2rad([0, 0, 90, 180, 270])) or
+                tilts[0, 0] != 0 or np.any(tilts[1:, 0] != tilts[1, 0])):
+            raise ValueError('tilts not suitable for ROFL')
+        tilt_angle = float(tilts[1, 0])
+        torch = _torch()
+        dev = torch.device('cuda', torch.cuda.current_device())
+        as_tensor = _is_tensor(data)
+        if as_tensor:
+            x = data.to(device=dev)
+            itemsize = x.element_size()
+        else:
+            arr = np.asarray(data)
+            itemsize = arr.dtype.itemsize
+            x = torch.from_numpy(np.ascontiguousarray(arr)).to(dev)
+        if x.dim() != 4:
+            raise TypeError('data: np.array([tilts,x,y,stack])')
+        if x.shape[0] != 5:
+            raise ValueError('data need 1 + 4 measurements')
+        if x.shape[-1] <= 3:
+            raise ValueError('data needs at least 3 equidistant rotations')
+        x = x.to(torch.float64).contiguous()
+        nx, ny = int(x.shape[1]), int(x.shape[2])
+        m = None
+        if mask is not None:
+            m = torch.as_tensor(np.asarray(mask.cpu() if _is_tensor(mask) else mask, dtype=bool)
+                                .astype(np.uint8), device=dev).contiguous()
+            if tuple(m.shape) != (nx, ny):
+                raise ValueError('mask shape does not match the images')
+        out = torch.empty((8, nx, ny), dtype=torch.float64, device=dev)
+        _lib.rofl(x, tilt_angle, 3.0, 0.0, m, out)
+        if itemsize <= 4:
+            out = out.to(torch.float32)
+        if not as_tensor:
+            out = out.cpu().numpy()
+        return out[0], out[1], out[2], (out[3], out[4], out[5], out[6], out[7])
+
     # ------------------------------------------------------------ helpers
     def crop_tilt_pixel(self):
         """number of boundary pixels affected by tilting (simpli.py:246-262)"""

@@ -149,6 +149,18 @@ int fpl_optic_resample(const double *in, int64_t nx, int64_t ny, int64_t n_rho, 
 int fpl_epa(const double *in, int64_t n_pix, int64_t n_rho, double *transmittance,
             double *direction, double *retardation, void *stream, char *msg, size_t msg_len);
 
+/* ROFL tilt analysis per pixel (src/fastpli/analysis/rofl.py:17-60 with the
+ * local fit of _ROFL_with_jacobi.py:403-522), as Simpli.apply_rofl calls it
+ * (src/fastpli/simulation/__simpli.py:623-706) for every pixel.
+ *   in   DEVICE float64 (5, n_pix, n_rho): the tilts (0, 0), (tau, 0),
+ *        (tau, pi/2), (tau, pi), (tau, 3 pi/2), n_rho equidistant in [0, pi)
+ *   mask DEVICE uint8 (n_pix) or NULL; masked-out pixels get NaN
+ *   out  DEVICE float64 (8, n_pix): direction, inclination, t_rel, their
+ *        deviations, chi^2, model evaluations */
+int fpl_rofl(const double *in, int64_t n_pix, int64_t n_rho, double tilt_angle, double gain,
+             double dir_offset, const uint8_t *mask, double *out, void *stream, char *msg,
+             size_t msg_len);
+
 /* Exact count of Mueller voxel-steps of one tilt: n_rho x rays x z-march
  * iterations (simulator.cpp:259-261).  Host only. */
 int64_t fpl_count_voxel_steps(const fpl_setup *setup, const int64_t dim[3],

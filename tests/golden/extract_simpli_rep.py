@@ -40,6 +40,8 @@ for t in range(5):
     for m in ('transmittance', 'direction', 'retardation'):
         DATASETS[f'/analysis/epa/{t}/{m}'] = (f'epa_{t}_{m}.npy', np.float64)
     DATASETS[f'/simulation/optic/{t}'] = (f'optic_{t}.npy', np.int64)
+for m in ('direction', 'inclination', 'trel'):
+    DATASETS[f'/analysis/rofl/{m}'] = (f'rofl_{m}.npy', np.float64)
 
 
 def shape_of(dset):

@@ -1,0 +1,67 @@
+"""Pins of the ROFL oracle: the reference's golden ROFL maps and its closed form (CPU)."""
+
+import os
+
+import numpy as np
+
+import oracle
+from oracle import rofl
+from tests.conftest import GOLDEN
+
+TILTS = np.deg2rad([(0, 0), (5.5, 0), (5.5, 90), (5.5, 180), (5.5, 270)])
+
+
+def golden_stack():
+    return np.array([np.load(os.path.join(GOLDEN, 'simpli_rep', f'optic_{t}.npy'))
+                     for t in range(5)])
+
+
+def test_rofl_reproduces_golden_maps():
+    # simpli_rep.py:95-104: apply_rofl on the noisy optic stack, all pixels;
+    # the reference's h5diff tolerance is 1e-5 relative
+    d, i, t, _ = rofl.rofl_map(golden_stack(), np.deg2rad(5.5))
+    for name, got in [('direction', np.rad2deg(d)), ('inclination', np.rad2deg(i)), ('trel', t)]:
+        gold = np.load(os.path.join(GOLDEN, 'simpli_rep', f'rofl_{name}.npy'))
+        np.testing.assert_allclose(got, gold, rtol=1e-6)
+
+
+def single_voxel_stack():
+    # rofl_test.py:15-61: one voxel, axis at 45 deg inclination, tilted sensor,
+    # step size 1 / cos(theta)
+    tissue = np.ones((1, 1, 1), np.int32)
+    axis = np.zeros((1, 1, 1, 3), np.float32)
+    axis[:] = [np.cos(np.deg2rad(45)), 0, np.sin(np.deg2rad(45))]
+    props = np.array([[0, 0], [-0.001, 0]])
+    rho = np.deg2rad([0, 30, 60, 90, 120, 150])
+    stack = []
+    for theta, phi in TILTS:
+        img, _ = oracle.simulate(tissue, axis, props, theta, phi, step_size=1 / np.cos(theta),
+                                 voxel_size=60, wavelength=525, light_intensity=26000,
+                                 untilt_sensor_view=False, interpolate='Slerp',
+                                 filter_rotations=rho)
+        stack.append(img)
+    return np.array(stack)
+
+
+def test_rofl_closed_form_single_voxel():
+    stack = single_voxel_stack()
+    d, i, t, *_ = rofl.fit(stack[:, 0, 0, :], TILTS[1, 0])
+    t_rel = 4 * 60 * 0.001 / (525 / 1e3)
+    assert abs(d - np.pi) < 1e-10 or abs(d) < 1e-10
+    assert abs(abs(i) - np.deg2rad(45)) < 1e-9
+    assert abs(t - t_rel) < 1e-8
+
+
+def test_rofl_noise_free_model_recovers_parameters():
+    # intensities of the fit model itself: the fit returns the generating
+    # parameters (direction in [0, pi), inclination sign per the symmetrization)
+    rng = np.random.default_rng(4)
+    mats = rofl.tilt_matrices(4, TILTS[1, 0])
+    for _ in range(5):
+        phi, alpha, tr = rng.uniform(0.2, 2.9), rng.uniform(-1.2, 1.2), rng.uniform(0.1, 0.8)
+        d, a, tt = rofl.rotated_params(phi, alpha, tr, mats, TILTS[1, 0])
+        model = rofl.intensities(d, a, tt, 9, 0).reshape(5, 9)
+        stack = 1000.0 * (1.0 + model)
+        out = rofl.fit(stack, TILTS[1, 0])
+        assert abs(out[0] - phi) < 1e-6 and abs(abs(out[1]) - abs(alpha)) < 1e-6
+        assert abs(out[2] - tr) < 1e-6
