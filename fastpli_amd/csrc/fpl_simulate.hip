@@ -488,39 +488,47 @@ __global__ void __launch_bounds__(256)
     k_pack(const int32_t *__restrict__ label, const float *__restrict__ vec, int64_t nx,
            int64_t ny, int64_t nz, void *__restrict__ rec_a, uint32_t *__restrict__ rec_b,
            int32_t *mm) {
+   // one workgroup per (x, 32 rows of y), walking the whole z extent: the
+   // source rows (1200 B of labels, 3600 B of axes for nz = 300) are not
+   // 128-byte aligned, and a row split over workgroups would fetch its
+   // boundary lines twice
    __shared__ int4 tile[32][33];
    const int64_t nyb = (ny + 31) / 32;
    const int64_t x = blockIdx.x / nyb;
-   const int64_t y0 = (blockIdx.x % nyb) * 32, z0 = (int64_t)blockIdx.y * 32;
+   const int64_t y0 = (blockIdx.x % nyb) * 32;
    const int t = threadIdx.x;
    int32_t lo = 0x7fffffff, hi = (int32_t)0x80000000;
-   for (int r = 0; r < 4; r++) {
-      const int yy = (t >> 5) + 8 * r, zz = t & 31;
-      const int64_t y = y0 + yy, z = z0 + zz;
-      if (y < ny && z < nz) {
-         const int64_t i = (x * ny + y) * nz + z;
-         const int32_t l = label[i];
-         lo = l < lo ? l : lo;
-         hi = l > hi ? l : hi;
-         encode(ST, l, vec[3 * i], vec[3 * i + 1], vec[3 * i + 2], tile[zz][yy]);
-      }
-   }
-   __syncthreads();
-   for (int r = 0; r < 4; r++) {
-      const int yy = t & 31, zz = (t >> 5) + 8 * r;
-      const int64_t y = y0 + yy, z = z0 + zz;
-      if (y < ny && z < nz) {
-         const int64_t o = (x * nz + z) * ny + y;
-         const int4 e = tile[zz][yy];
-         if (ST == ST_F32) {
-            ((int4 *)rec_a)[o] = e;
-         } else if (ST == ST_F16) {
-            ((uint2 *)rec_a)[o] = make_uint2((uint32_t)e.x, (uint32_t)e.y);
-         } else {
-            ((uint16_t *)rec_a)[o] = (uint16_t)e.x;
-            rec_b[o] = (uint32_t)e.y;
+   for (int64_t z0 = 0; z0 < nz; z0 += 32) {
+      for (int r = 0; r < 4; r++) {
+         const int yy = (t >> 5) + 8 * r, zz = t & 31;
+         const int64_t y = y0 + yy, z = z0 + zz;
+         if (y < ny && z < nz) {
+            const int64_t i = (x * ny + y) * nz + z;
+            const int32_t l = label[i];
+            lo = l < lo ? l : lo;
+            hi = l > hi ? l : hi;
+            const float3 v = *(const float3 *)(vec + 3 * i);
+            encode(ST, l, v.x, v.y, v.z, tile[zz][yy]);
          }
       }
+      __syncthreads();
+      for (int r = 0; r < 4; r++) {
+         const int yy = t & 31, zz = (t >> 5) + 8 * r;
+         const int64_t y = y0 + yy, z = z0 + zz;
+         if (y < ny && z < nz) {
+            const int64_t o = (x * nz + z) * ny + y;
+            const int4 e = tile[zz][yy];
+            if (ST == ST_F32) {
+               ((int4 *)rec_a)[o] = e;
+            } else if (ST == ST_F16) {
+               ((uint2 *)rec_a)[o] = make_uint2((uint32_t)e.x, (uint32_t)e.y);
+            } else {
+               ((uint16_t *)rec_a)[o] = (uint16_t)e.x;
+               rec_b[o] = (uint32_t)e.y;
+            }
+         }
+      }
+      __syncthreads();
    }
    for (int o = 32; o > 0; o >>= 1) {
       const int32_t a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
@@ -737,7 +745,7 @@ void clear_flags(uint32_t *d_flags, int32_t *d_mm, hipStream_t st) {
 
 void pack(int storage, const int64_t dim[3], const int32_t *label, const float *vector,
           void *packed, uint32_t *packed_b, int32_t *d_mm, hipStream_t st) {
-   const dim3 grid((unsigned)(dim[0] * ((dim[1] + 31) / 32)), (unsigned)((dim[2] + 31) / 32));
+   const dim3 grid((unsigned)(dim[0] * ((dim[1] + 31) / 32)));
    if (storage == ST_F32)
       hipLaunchKernelGGL(k_pack<ST_F32>, grid, dim3(256), 0, st, label, vector, dim[0], dim[1],
                          dim[2], packed, packed_b, d_mm);
