@@ -50,7 +50,7 @@ struct TiltDev {
    double a, b, c, d, D;
    float R[9];           // Mat3RotZYZ(phi, theta, -phi) for the optical axis
    int32_t rotate;       // theta != 0
-   int32_t pad;
+   int32_t centers;      // every sample point is a voxel centre: no interpolation
 };
 
 struct Params {
@@ -253,9 +253,11 @@ template <typename Idx, int ST>
 __device__ float3 get_vec_interp(const Field<Idx, ST> &F, double px, double py, double pz,
                                  int32_t lab, const float3 &own, int slerp) {
    const double fx = px - 0.5, fy = py - 0.5, fz = pz - 0.5;
-   const int32_t x0 = clampi((int32_t)floor(fx), F.nx), x1 = clampi((int32_t)ceil(fx), F.nx);
-   const int32_t y0 = clampi((int32_t)floor(fy), F.ny), y1 = clampi((int32_t)ceil(fy), F.ny);
-   const int32_t z0 = clampi((int32_t)floor(fz), F.nz), z1 = clampi((int32_t)ceil(fz), F.nz);
+   const double flx = floor(fx), fly = floor(fy), flz = floor(fz);
+   // ceil(f) = floor(f) + (f != floor(f)); both clamped to the volume
+   const int32_t x0 = clampi((int32_t)flx, F.nx), x1 = clampi((int32_t)flx + (fx != flx), F.nx);
+   const int32_t y0 = clampi((int32_t)fly, F.ny), y1 = clampi((int32_t)fly + (fy != fly), F.ny);
+   const int32_t z0 = clampi((int32_t)flz, F.nz), z1 = clampi((int32_t)flz + (fz != flz), F.nz);
    if (x0 == x1 && y0 == y1 && z0 == z1) return own;
    const bool dx = x0 != x1, dy = y0 != y1, dz = z0 != z1;
    const float xd = dx ? (float)((px - (double)x0) - 0.5) : 0.0f;
@@ -325,8 +327,10 @@ __device__ void tilted_start(const TiltDev &T, int64_t i, int64_t j, double &tx,
 constexpr int RAYS_X = 4;  // ray tile of one workgroup: 4 (x) x 64 (y), one wave per x row
 constexpr int RAYS_Y = 64;
 
+// at most 64 VGPRs (8 waves per SIMD): the march is latency bound, and the
+// wider occupancy hides the record loads of the interpolation better
 template <typename Idx, int ST>
-__global__ void __launch_bounds__(WG)
+__global__ void __launch_bounds__(WG, 8)
     k_march(const Params *__restrict__ prm, const TiltDev *__restrict__ tilts,
             const PropDev *__restrict__ props, const void *__restrict__ rec_a,
             const uint32_t *__restrict__ rec_b, double *__restrict__ images, uint32_t *flags) {
@@ -391,8 +395,9 @@ __global__ void __launch_bounds__(WG)
       if (pd.bg) {
          if (pd.mu_t != 0.0f) two_sum(mh, ml, pd.mu_t);
       } else {
-         float3 v = (P.interp == 0) ? r.v
-                                    : get_vec_interp(F, px, py, pz, lab, r.v, P.interp == 2);
+         float3 v = (P.interp == 0 || T.centers)
+                        ? r.v
+                        : get_vec_interp(F, px, py, pz, lab, r.v, P.interp == 2);
          if (is_zero(v)) {
             warn |= FPL_WARN_ZERO_VEC; // skipped without attenuation (simulator.cpp:292-297)
          } else {
@@ -636,6 +641,9 @@ TiltDev make_tilt(const fpl_setup &S, const int64_t dim[3], double theta, double
    const M3 R = rot_zyz(phi, theta, -phi);
    for (int k = 0; k < 9; k++) T.R[k] = (float)R.a[k];
    T.rotate = theta != 0;
+   // theta = 0 and step 1 with the untilted start: points (i + 1/2, j + 1/2,
+   // k + 1/2), where all corner indices of GetVec coincide (simulator.cpp:516-521)
+   T.centers = (theta == 0 && S.step_size == 1.0 && S.untilt_sensor_view) ? 1 : 0;
    return T;
 }
 
