@@ -11,7 +11,9 @@ Mueller voxel-step = (tilt, ray, z-step, rotation); a step does
 9 * (360000 * 300 + 4 * 360000 * 301) = 4.873e9 of them.
 
 Prints one JSON line.  The CPU baseline is the FP64 OpenMP oracle (a port of
-the reference's C++ core) timed on a bounded y-slab of the same workload.
+the reference's C++ core) timed on the same workload (by default the whole
+volume, one step; --cpu-fraction bounds it to a y-slab).  The `serving`
+object reports p50 / p99 latency of a mixed-size request stream.
 """
 
 import argparse
@@ -184,7 +186,8 @@ def serving_bench(n_requests, load, seed=0):
 
 
 def cpu_baseline(fbs, fraction, threads):
-    """Oracle (FP64, OpenMP) on the y-slab [0, fraction * 120 um] of the workload."""
+    """Oracle (FP64, OpenMP) on the y-slab [0, fraction * 120 um] of the workload
+    (fraction 1: the whole volume)."""
     import oracle
     dim = (int(round(VOI[0] / VOXEL)), max(1, int(round(VOI[1] * fraction / VOXEL))),
            int(round(VOI[2] / VOXEL)))
@@ -202,7 +205,8 @@ def cpu_baseline(fbs, fraction, threads):
         'unit': 'Mueller voxel-steps/s',
         'cores': threads,
         'kind': 'port',
-        'sample': (f'y-slab {dim[0]}x{dim[1]}x{dim[2]} voxels of the same workload '
+        'sample': (('whole volume' if fraction >= 1 else 'y-slab') +
+                   f' {dim[0]}x{dim[1]}x{dim[2]} voxels of the same workload '
                    f'(generate + 5 tilts x 9 rotations, Slerp), {w:.3e} voxel-steps in '
                    f'{dt:.2f} s, FP64 OpenMP oracle'),
     }
