@@ -42,9 +42,11 @@ constexpr int TX = 8, TY = 8, TZ = 32; // voxel tile of one workgroup
 constexpr int WG = 256;                // 4 waves; wave w owns the 8x8x8 slab z in [8w, 8w + 8)
 constexpr int SZ = 8;                  // slab height
 constexpr int SLOTS = 64 * SZ;         // voxels of a slab: slot = (z - z_slab) * 64 + column
-constexpr int CHUNK = 64;              // segment records staged in LDS at a time
+constexpr int CHUNK = 40;              // segment records staged in LDS at a time (LDS per workgroup
+                                       // stays well below 32 KB: 5 workgroups per CU)
 constexpr int QN = 128;                // candidate ring of a wave (never holds 128 entries)
-constexpr int LSQ_LDS = 256;           // layer tables up to this many rows are read from LDS
+constexpr int LSQ_LDS = 64;            // layer tables up to this many rows are read from LDS
+constexpr uint32_t SHORT_LIST = 65535; // longest tile list of the 16-bit winner indices
 
 struct SegRec {        // 128 bytes, one per segment (indexed by its first point)
    double p[3];        // start point, voxel units
@@ -207,7 +209,10 @@ __global__ void k_bundle_boxes(Params pr, const BundleDev *__restrict__ bd,
                                uint32_t *flags) {
    __shared__ double red[6][4];
    const int b = blockIdx.x;
-   if (b == 0 && threadIdx.x == 0) flags[0] = 0;
+   if (b == 0 && threadIdx.x == 0) {
+      flags[0] = 0; // warnings
+      flags[1] = 0; // voxelizer: a tile list too long for 16-bit positions
+   }
    double lo[3], hi[3];
    box_empty(lo, hi);
    for (int64_t f = bd[b].fiber_begin + threadIdx.x; f < bd[b].fiber_end; f += blockDim.x)
@@ -455,9 +460,12 @@ __device__ __forceinline__ float uniform(float v) {
 // the reference's sequential rule "update if stored >= new" ends in.
 constexpr unsigned long long KEY_EMPTY = (0x7F800000ull << 32) | 0x80000000ull; // +inf, label 0
 
-struct WaveState {
+// SegT: uint16_t stores the winner's position in the tile list + 1 (0: none),
+// which keeps the LDS of a workgroup within 32 KB (5 workgroups per CU);
+// int32_t stores its global index (-1: none), for tiles with longer lists
+template <typename SegT> struct WaveState {
    unsigned long long key[SLOTS];
-   int32_t seg[SLOTS];  // global index of the stored segment, -1 none
+   SegT seg[SLOTS];
    uint32_t queue[QN];  // candidates (chunk index << 9 | slot), ring
    uint2 tab[64];       // per chunk record: candidate prefix | box << 16, division magics
 };
@@ -469,7 +477,8 @@ struct alignas(16) LdsSeg { // 144-byte stride spreads records over the LDS bank
 
 // Exact FP64 test of up to 64 queued candidates (one per lane) and their
 // ordered application to the slab state.
-__device__ __forceinline__ void run_batch(WaveState &W, const LdsSeg *chunk,
+template <typename SegT>
+__device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chunk,
                                           const uint32_t *chunk_idx, int n, int q_head,
                                           int lane, int xt, int yt, int zw, int max_layer,
                                           const double *lsq, uint32_t *flags, bool &ov) {
@@ -542,7 +551,7 @@ __device__ __forceinline__ void run_batch(WaveState &W, const LdsSeg *chunk,
       wave_sync();
       if (acc && W.key[slot] == k) {
          W.key[slot] = rest;
-         W.seg[slot] = (int32_t)chunk_idx[c];
+         W.seg[slot] = (SegT)chunk_idx[c];
       }
    } else {
       uint32_t overlap = 0;
@@ -553,7 +562,7 @@ __device__ __forceinline__ void run_batch(WaveState &W, const LdsSeg *chunk,
                const int ol = (int)(uint32_t)(cur & 0x7FFFFFFFull);
                if (ol != 0 && abs(ol - new_label) >= max_layer) overlap = 1;
                W.key[slot] = rest;
-               W.seg[slot] = (int32_t)chunk_idx[c];
+               W.seg[slot] = (SegT)chunk_idx[c];
             }
          }
          wave_sync();
@@ -576,13 +585,14 @@ __device__ __forceinline__ void run_batch(WaveState &W, const LdsSeg *chunk,
 // stored distance, with margins that keep every candidate the exact test
 // could accept; survivors enter the wave's queue in order.  Stage B
 // (run_batch) runs the exact FP64 test on 64 queued candidates at a time.
+template <typename SegT>
 __global__ void __launch_bounds__(WG)
     k_voxelize(Params pr, const SegRec *__restrict__ segs, const uint32_t *__restrict__ list,
                const uint32_t *__restrict__ range, const BundleDev *__restrict__ bd,
                const double *__restrict__ layer_sq, const uint8_t *__restrict__ layer_model,
                int only_label, int32_t *__restrict__ label, float *__restrict__ vector,
                uint32_t *flags) {
-   __shared__ WaveState wstate[WG / 64];
+   __shared__ WaveState<SegT> wstate[WG / 64];
    __shared__ LdsSeg chunk[CHUNK];
    __shared__ uint32_t chunk_idx[CHUNK];
 
@@ -607,17 +617,24 @@ __global__ void __launch_bounds__(WG)
       for (int i = threadIdx.x; i < pr.n_lsq; i += WG) lsq_lds[i] = layer_sq[i];
    const double *lsq = lsq_in_lds ? (const double *)lsq_lds : layer_sq;
 
-   WaveState &W = wstate[wave];
+   const uint32_t beg = range[2 * t], end = range[2 * t + 1];
+   constexpr bool short_seg = sizeof(SegT) == 2;
+   if (short_seg && end - beg > SHORT_LIST) {
+      // this tile's list does not fit 16-bit positions: the host reruns the
+      // volume with 32-bit winner indices (uniform over the workgroup)
+      if (threadIdx.x == 0) atomicOr(&flags[1], 1u);
+      return;
+   }
+   WaveState<SegT> &W = wstate[wave];
    for (int s = lane; s < SLOTS; s += 64) {
       W.key[s] = KEY_EMPTY;
-      W.seg[s] = -1;
+      W.seg[s] = short_seg ? (SegT)0 : (SegT)-1;
    }
    bool ov = (__atomic_load_n(&flags[0], __ATOMIC_RELAXED) & FPL_WARN_OVERLAP) != 0;
    ov = uniform((int)ov) != 0;
    int q_head = 0, q_tail = 0;
    wave_sync();
 
-   const uint32_t beg = range[2 * t], end = range[2 * t + 1];
    for (uint32_t base = beg; base < end; base += CHUNK) {
       const int n = (int)((end - base) < (uint32_t)CHUNK ? (end - base) : (uint32_t)CHUNK);
       __syncthreads(); // every wave is done with the previous chunk
@@ -625,7 +642,7 @@ __global__ void __launch_bounds__(WG)
          const int c = e >> 3, piece = e & 7;
          const uint32_t sidx = list[base + c];
          ((int4 *)&chunk[c].r)[piece] = ((const int4 *)&segs[sidx])[piece];
-         if (piece == 0) chunk_idx[c] = sidx;
+         if (piece == 0) chunk_idx[c] = short_seg ? base - beg + (uint32_t)c + 1u : sidx;
       }
       __syncthreads();
 
@@ -752,7 +769,8 @@ __global__ void __launch_bounds__(WG)
       if (vx >= pr.nx || vy >= pr.ny || vz >= pr.nz) continue;
       const int64_t ind = ((int64_t)vx * pr.ny + vy) * pr.nz + vz;
       const int32_t lab = (int32_t)(uint32_t)(W.key[slot] & 0x7FFFFFFFull);
-      const int32_t sg = W.seg[slot];
+      const int32_t sg = short_seg ? (W.seg[slot] ? (int32_t)list[beg + W.seg[slot] - 1u] : -1)
+                                   : (int32_t)W.seg[slot];
       label[ind] = lab;
       if (!only_label) {
          float vo[3] = {0.0f, 0.0f, 0.0f};
@@ -971,14 +989,27 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
                          (uint32_t *)rng);
    }
    KernelTimer kt("voxelize", st);
-   hipLaunchKernelGGL(k_voxelize, dim3((unsigned)n_tiles), dim3(WG), 0, st, pr,
+   hipLaunchKernelGGL(k_voxelize<uint16_t>, dim3((unsigned)n_tiles), dim3(WG), 0, st, pr,
                       (const SegRec *)segs, (const uint32_t *)vb, (const uint32_t *)rng, d_bd,
                       d_lsq, d_lm, only_label, label, vector, d_flags);
    kt.stop();
    FPL_HIP(hipGetLastError());
-   FPL_HIP(hipMemcpyAsync(stage, d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+   FPL_HIP(hipMemcpyAsync(stage, d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
    FPL_HIP(hipStreamSynchronize(st));
    kt.collect();
+   if (((uint32_t *)stage)[1] & 1u) {
+      // a tile list longer than 65535: the whole volume again with 32-bit
+      // winner indices (every voxel is rewritten; warnings are an OR)
+      KernelTimer kt2("voxelize", st);
+      hipLaunchKernelGGL(k_voxelize<int32_t>, dim3((unsigned)n_tiles), dim3(WG), 0, st, pr,
+                         (const SegRec *)segs, (const uint32_t *)vb, (const uint32_t *)rng, d_bd,
+                         d_lsq, d_lm, only_label, label, vector, d_flags);
+      kt2.stop();
+      FPL_HIP(hipGetLastError());
+      FPL_HIP(hipMemcpyAsync(stage, d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      FPL_HIP(hipStreamSynchronize(st));
+      kt2.collect();
+   }
    if (warn) *warn = *(uint32_t *)stage;
    return Status();
 }

@@ -111,6 +111,22 @@ def test_generate_adjacent_bundles_without_overlap():
     assert not ow & oracle.WARN_OVERLAP
 
 
+def test_generate_long_tile_list():
+    # > 65535 segments in one 8 x 8 x 32 voxel tile: the 16-bit winner
+    # positions of the fast kernel do not fit, and the volume is redone with
+    # 32-bit indices
+    _say('generate: one tile with 79200 segments')
+    rng = np.random.default_rng(9)
+    fibers = []
+    for _ in range(800):
+        start = rng.uniform(2.5, 5.5, size=3)
+        steps = rng.normal(scale=0.05, size=(99, 3))
+        pts = np.clip(np.vstack([start, start + np.cumsum(steps, axis=0)]), 1.0, 7.0)
+        fibers.append(np.hstack([pts, np.full((100, 1), 0.5)]))
+    fbs = objects.FiberBundles([fibers], [[(0.6, 0, 1, 'b'), (1.0, -0.004, 1, 'p')]])
+    _compare(fbs, (8, 8, 8), [0.0] * 3, 1.0)
+
+
 def test_generate_overlap_warning():
     _say('generate: overlap warning')
     fbs = objects.FiberBundles([[[[0, 3, 3, 2], [8, 3, 3, 2]]], [[[3, 0, 3, 2], [3, 8, 3, 2]]]],
