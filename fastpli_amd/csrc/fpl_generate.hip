@@ -459,6 +459,19 @@ __device__ __forceinline__ float uniform(float v) {
 // the smallest distance with ties going to the latest candidate -- the state
 // the reference's sequential rule "update if stored >= new" ends in.
 constexpr unsigned long long KEY_EMPTY = (0x7F800000ull << 32) | 0x80000000ull; // +inf, label 0
+constexpr int32_t LABEL_LATER = 0x7FFFFFFF; // label left to the final pass
+
+// std::lower_bound over the bundle's double scale^2 list with the float ratio
+// (generator.cpp:345-349) -> label ly + 1 + bundle * max_layer (:351); the
+// list ends with a +inf sentinel, so the search stops at n_layers at the latest
+__device__ __forceinline__ int32_t layer_label(float dist_sq, float r_sq, const double *lsq,
+                                               int32_t layer_off, int32_t bundle, int max_layer) {
+   const double ratio = (double)(dist_sq / r_sq);
+   const double *L = lsq + layer_off;
+   int ly = 0;
+   while (L[ly] < ratio) ly++;
+   return ly + 1 + bundle * max_layer;
+}
 
 // SegT: uint16_t stores the winner's position in the tile list + 1 (0: none),
 // which keeps the LDS of a workgroup within 32 KB (5 workgroups per CU);
@@ -509,25 +522,19 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
       const double rr = rp * (1 - cp.t) + rq * cp.t;
       const float r_sq = (float)(rr * rr);
       acc = !(dist_sq > f_last * r_sq);
-      if (acc) {
-         // std::lower_bound over the bundle's double scale^2 list with the
-         // float ratio (generator.cpp:345-349); the list ends with a +inf
-         // sentinel, so the search stops at n_layers at the latest
-         const double ratio = (double)(dist_sq / r_sq);
-         const double *L = lsq + R.layer_off;
-         int ly = 0;
-         while (L[ly] < ratio) ly++;
-         new_label = ly + 1 + R.bundle * max_layer;
-      }
+      // the label is needed here only while the overlap warning is undecided;
+      // otherwise the final pass computes it for the winning segment alone
+      new_label = LABEL_LATER;
+      if (acc && !ov) new_label = layer_label(dist_sq, r_sq, lsq, R.layer_off, R.bundle, max_layer);
    }
-   unsigned long long old = KEY_EMPTY;
-   if (acc) old = W.key[slot];
    // The overlap warning (generator.cpp:353-359) depends on the order of the
    // updates.  Unless it is already raised, a batch whose labels (new ones and
    // the ones they may replace) all lie within max_layer of each other cannot
    // raise it; any other batch is applied one candidate at a time.
    bool serial = false;
    if (!ov) {
+      unsigned long long old = KEY_EMPTY;
+      if (acc) old = W.key[slot];
       int mn = INT_MAX, mx = INT_MIN;
       if (acc) {
          mn = mx = new_label;
@@ -768,30 +775,46 @@ __global__ void __launch_bounds__(WG)
       const int vx = xt + (col >> 3), vy = yt + (col & 7), vz = zw + zz;
       if (vx >= pr.nx || vy >= pr.ny || vz >= pr.nz) continue;
       const int64_t ind = ((int64_t)vx * pr.ny + vy) * pr.nz + vz;
-      const int32_t lab = (int32_t)(uint32_t)(W.key[slot] & 0x7FFFFFFFull);
+      int32_t lab = (int32_t)(uint32_t)(W.key[slot] & 0x7FFFFFFFull);
       const int32_t sg = short_seg ? (W.seg[slot] ? (int32_t)list[beg + W.seg[slot] - 1u] : -1)
                                    : (int32_t)W.seg[slot];
+      const double pt[3] = {vx + 0.5, vy + 0.5, vz + 0.5};
+      SegExact E;
+      Closest cp;
+      int32_t layer_off = 0, bundle = 0;
+      if (sg >= 0) {
+         const SegRec &R = segs[sg];
+         const double p[3] = {R.p[0], R.p[1], R.p[2]};
+         const double q[3] = {R.q[0], R.q[1], R.q[2]};
+         E = seg_exact(p, q);
+         layer_off = R.layer_off;
+         bundle = R.bundle;
+         if (lab == LABEL_LATER) {
+            // the winner's label from the same arithmetic as its exact test
+            cp = closest(pt, E);
+            const double d0 = cp.c[0] - pt[0], d1 = cp.c[1] - pt[1], d2v = cp.c[2] - pt[2];
+            double dd = 0.0;
+            dd = dd + d0 * d0;
+            dd = dd + d1 * d1;
+            dd = dd + d2v * d2v;
+            const double rr = R.rp * (1 - cp.t) + R.rq * cp.t;
+            lab = layer_label((float)dd, (float)(rr * rr), lsq, layer_off, bundle, max_layer);
+         }
+      }
       label[ind] = lab;
       if (!only_label) {
          float vo[3] = {0.0f, 0.0f, 0.0f};
          if (sg >= 0) {
-            const SegRec &R = segs[sg];
             // a float ratio just above the last scale^2 gives the past-the-end
             // layer index; the padded model table repeats the outermost
             // layer's orientation there
-            const int ly = lab - (1 + R.bundle * max_layer);
-            const uint8_t m = layer_model[R.layer_off + ly];
-            if (m != 'b') {
-               const double p[3] = {R.p[0], R.p[1], R.p[2]};
-               const double q[3] = {R.q[0], R.q[1], R.q[2]};
-               const SegExact E = seg_exact(p, q);
-               if (m == 'p') {
-                  unit_to_float(E.v[0], E.v[1], E.v[2], vo);
-               } else {
-                  const double pt[3] = {vx + 0.5, vy + 0.5, vz + 0.5};
-                  const Closest cp = closest(pt, E);
-                  unit_to_float(pt[0] - cp.c[0], pt[1] - cp.c[1], pt[2] - cp.c[2], vo);
-               }
+            const int ly = lab - (1 + bundle * max_layer);
+            const uint8_t m = layer_model[layer_off + ly];
+            if (m == 'p') {
+               unit_to_float(E.v[0], E.v[1], E.v[2], vo);
+            } else if (m == 'r') {
+               cp = closest(pt, E);
+               unit_to_float(pt[0] - cp.c[0], pt[1] - cp.c[1], pt[2] - cp.c[2], vo);
             }
          }
          vector[3 * ind + 0] = vo[0];
@@ -869,6 +892,8 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
    }
    // GetPropertyList (generator.cpp:496-526)
    const int64_t n_props = nb * max_layer + 1;
+   if (n_props >= 0x7FFFFFFF) // labels are int32; 0x7FFFFFFF is an internal marker
+      return make_status(FPL_E_INVALID, "too many labels: %lld", (long long)n_props);
    if (props) {
       for (int64_t i = 0; i < 2 * n_props; i++) props[i] = 0.0;
       for (int64_t b = 0; b < nb; b++)
