@@ -19,7 +19,8 @@
 //   k_emit        (tile, segment) pairs in segment order
 //   radix sort    stable by tile -> per tile list in global segment order
 //   k_voxelize    one workgroup per tile, the tile's segment records staged in
-//                 LDS; each wave owns an 8x8x8 slab whose state lives in LDS.
+//                 LDS; each wave owns every fourth z layer of the tile (8x8x8
+//                 voxels), whose state lives in LDS.
 //                 A float lower-bound test (conservative by construction)
 //                 queues candidates; the exact FP64 test runs on full batches of
 //                 64 queued candidates; per voxel the reference's update rule is
@@ -39,9 +40,9 @@ namespace fpl {
 namespace gen {
 
 constexpr int TX = 8, TY = 8, TZ = 32; // voxel tile of one workgroup
-constexpr int WG = 256;                // 4 waves; wave w owns the 8x8x8 slab z in [8w, 8w + 8)
-constexpr int SZ = 8;                  // slab height
-constexpr int SLOTS = 64 * SZ;         // voxels of a slab: slot = (z - z_slab) * 64 + column
+constexpr int WG = 256;                // 4 waves; wave w owns the layers z = w + 4 l, l < 8
+constexpr int SZ = 8;                  // layers of a wave
+constexpr int SLOTS = 64 * SZ;         // voxels of a wave: slot = l * 64 + column
 constexpr int CHUNK = 40;              // segment records staged in LDS at a time (LDS per workgroup
                                        // stays well below 32 KB: 5 workgroups per CU)
 constexpr int QN = 128;                // candidate ring of a wave (never holds 128 entries)
@@ -489,7 +490,7 @@ struct alignas(16) LdsSeg { // 144-byte stride spreads records over the LDS bank
 };
 
 // Exact FP64 test of up to 64 queued candidates (one per lane) and their
-// ordered application to the slab state.
+// ordered application to the wave's voxel state.
 template <typename SegT>
 __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chunk,
                                           const uint32_t *chunk_idx, int n, int q_head,
@@ -510,7 +511,7 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
       const SegExact E = seg_exact(p, q);
       const int col = slot & 63;
       const double pt[3] = {(double)(xt + (col >> 3)) + 0.5, (double)(yt + (col & 7)) + 0.5,
-                            (double)(zw + (slot >> 6)) + 0.5};
+                            (double)(zw + 4 * (slot >> 6)) + 0.5};
       // FillVoxelsAroundFiberSegment inner loop (generator.cpp:322-349)
       const Closest cp = closest(pt, E);
       const double d0 = cp.c[0] - pt[0], d1 = cp.c[1] - pt[1], d2v = cp.c[2] - pt[2];
@@ -582,10 +583,11 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
    wave_sync();
 }
 
-// One workgroup per 8x8x32 tile, wave w owning the slab z in [8w, 8w + 8)
-// (slot = z * 64 + column, column = x * 8 + y within the tile).  The tile's
-// segments (global order) are staged in LDS in chunks of 64.  Stage A: one
-// lane per record computes the record's box within the slab; the candidate
+// One workgroup per 8x8x32 tile, wave w owning the layers z = 4 l + w, l < 8
+// (slot = l * 64 + column, column = x * 8 + y within the tile): a segment's
+// candidates spread evenly over the waves, which meet at every chunk.  The tile's
+// segments (global order) are staged in LDS in chunks of CHUNK.  Stage A: one
+// lane per record computes the record's box within the wave's layers; the candidate
 // voxels of all records are then enumerated 64 at a time in record order, and
 // each is tested with a float lower bound of its squared distance (distance
 // to the segment's box) against the outer-layer threshold and the voxel's
@@ -612,7 +614,7 @@ __global__ void __launch_bounds__(WG)
    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
    const int lane = threadIdx.x & 63;
    const int xt = tx * TX, yt = ty * TY;
-   const int zw = tz * TZ + wave * SZ;
+   const int zt = tz * TZ, zw = zt + wave; // the wave's layers: z = zw + 4 * (0..7)
    const int max_layer = pr.max_layer;
    // float margins of the stage A test: relative 2^-12, absolute 2^-6
    const float rel = 1.0f + 0x1p-12f, absm = 0x1p-6f;
@@ -654,8 +656,8 @@ __global__ void __launch_bounds__(WG)
       __syncthreads();
 
       // Stage A set-up, one lane per record of the chunk: the record's integer
-      // box within the slab and its z range narrowed by the nearest columns of
-      // the tile.  The in-box columns nearest to the segment's box bound every
+      // box within the tile and its z range narrowed by the nearest columns of
+      // the tile, then restricted to the wave's layers.  The in-box columns nearest to the segment's box bound every
       // voxel's float ex and ey from below (float rounding is monotone), so a
       // voxel that passes the float radius test RN(ez^2 + exy) <= thr_m has
       // ez^2 <= rem + ulp(thr_m), rem = thr_m - exy_min >= 0; h covers that
@@ -665,7 +667,7 @@ __global__ void __launch_bounds__(WG)
       uint32_t geo = 0, magic = 0;
       if (lane < n) {
          const SegRec &R = chunk[lane].r;
-         const int z0 = max(R.lo[2], zw), z1 = min(R.hi[2], zw + SZ);
+         const int z0 = max(R.lo[2], zt), z1 = min(R.hi[2], zt + TZ);
          const int cx0 = max(R.lo[0], xt), cx1 = min(R.hi[0], xt + TX);
          const int cy0 = max(R.lo[1], yt), cy1 = min(R.hi[1], yt + TY);
          if (z0 < z1 && cx0 < cx1 && cy0 < cy1) {
@@ -680,11 +682,13 @@ __global__ void __launch_bounds__(WG)
                                     1.0f + 0x1p-10f, 0x1p-10f);
                const int za = max(z0, (int)ceil((double)R.bl[2] - (double)h));
                const int zb = min(z1, (int)floor((double)R.bh[2] + (double)h) + 1);
-               if (za < zb) {
-                  const int ncy = cy1 - cy0, nzz = zb - za;
+               // the wave's layers in [za, zb): z = zw + 4 * l, l in [la, lb)
+               const int la = (za - zw + 3) >> 2, lb = (zb - zw + 3) >> 2;
+               if (la < lb) {
+                  const int ncy = cy1 - cy0, nzz = lb - la;
                   cnt = (cx1 - cx0) * ncy * nzz;
                   geo = (uint32_t)(cx0 - xt) | (uint32_t)(cy0 - yt) << 3 |
-                        (uint32_t)(za - zw) << 6 | (uint32_t)(ncy - 1) << 9 |
+                        (uint32_t)la << 6 | (uint32_t)(ncy - 1) << 9 |
                         (uint32_t)(nzz - 1) << 12;
                   // floor(i / n) = (i * ceil(2^15 / n)) >> 15 for i < 512, n <= 8
                   magic = (uint32_t)(32767 / nzz + 1) | (uint32_t)(32767 / ncy + 1) << 16;
@@ -730,7 +734,7 @@ __global__ void __launch_bounds__(WG)
             const int lz = (int)((g >> 6) & 7) + iz;
             const int slot = lz * 64 + (lx << 3 | ly);
             const SegRec &R = chunk[c].r;
-            const float xf = (float)(xt + lx), yf = (float)(yt + ly), zf = (float)(zw + lz);
+            const float xf = (float)(xt + lx), yf = (float)(yt + ly), zf = (float)(zw + 4 * lz);
             const float ex = fmaxf(fmaxf(R.bl[0] - xf, xf - R.bh[0]), 0.0f);
             const float ey = fmaxf(fmaxf(R.bl[1] - yf, yf - R.bh[1]), 0.0f);
             const float ez = fmaxf(fmaxf(R.bl[2] - zf, zf - R.bh[2]), 0.0f);
@@ -772,7 +776,7 @@ __global__ void __launch_bounds__(WG)
    for (int j = 0; j < 8; j++) {
       const int col = j * 8 + (lane >> 3), zz = lane & 7;
       const int slot = zz * 64 + col;
-      const int vx = xt + (col >> 3), vy = yt + (col & 7), vz = zw + zz;
+      const int vx = xt + (col >> 3), vy = yt + (col & 7), vz = zw + 4 * zz;
       if (vx >= pr.nx || vy >= pr.ny || vz >= pr.nz) continue;
       const int64_t ind = ((int64_t)vx * pr.ny + vy) * pr.nz + vz;
       int32_t lab = (int32_t)(uint32_t)(W.key[slot] & 0x7FFFFFFFull);
