@@ -57,7 +57,8 @@ struct SegRec {        // 128 bytes, one per segment (indexed by its first point
    int32_t hi[3];
    float bl[3];        // min(p, q) - 0.5 rounded down and max(p, q) - 0.5 rounded up: the
    float bh[3];        // segment's box in the frame where voxel i has its centre at i
-   float thr;          // f_last * float(max(rp, rq)^2): outer-layer test at the larger radius
+   float thr;          // f_last * float(max(rp, rq)^2), the outer-layer test at the larger
+                       // radius, widened by the stage A margins
    float f_last;       // float(last scale^2) of the bundle
    int32_t layer_off;  // the bundle's rows in the (sentinel-padded) layer tables
    int32_t bundle;
@@ -264,10 +265,27 @@ __global__ void k_fibers(Params pr, const int64_t *__restrict__ fiber_ptr,
    fiber_ok[f] = ok;
 }
 
+// vm::unit (vemath.hpp:390-402) rounded to float
+__device__ __forceinline__ void unit_to_float(double x, double y, double z, float *out) {
+   double n2 = 0.0;
+   n2 = n2 + x * x;
+   n2 = n2 + y * y;
+   n2 = n2 + z * z;
+   if (n2 > 0) {
+      const double inv = 1.0 / sqrt(n2);
+      x = x * inv;
+      y = y * inv;
+      z = z * inv;
+   }
+   out[0] = (float)x;
+   out[1] = (float)y;
+   out[2] = (float)z;
+}
+
 __global__ void k_segments(Params pr, const double *__restrict__ P,
                            const int64_t *__restrict__ fiber_ptr,
                            const BundleDev *__restrict__ bd, const uint8_t *__restrict__ fiber_ok,
-                           SegRec *segs, int64_t *seg_count) {
+                           SegRec *segs, float4 *seg_dir, int64_t *seg_count) {
    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
    if (i >= pr.n_points) return;
    const int64_t f = find_fiber(fiber_ptr, pr.n_fibers, i);
@@ -307,10 +325,16 @@ __global__ void k_segments(Params pr, const double *__restrict__ P,
             R.hi[k] = (int32_t)ihi[k];
          }
          R.f_last = bd[b].f_last_sq;
-         R.thr = R.f_last * (float)(rmax * rmax);
+         // outer-layer threshold with the stage A margins (relative 2^-12, absolute 2^-6)
+         R.thr = fmaf(R.f_last * (float)(rmax * rmax), 1.0f + 0x1p-12f, 0x1p-6f);
          R.layer_off = bd[b].layer_off;
          R.bundle = b;
          segs[i] = R;
+         // the 'p' orientation of the segment, unit(q - p) as the voxel update
+         // computes it (generator.cpp:365-378)
+         float d[3];
+         unit_to_float(R.q[0] - R.p[0], R.q[1] - R.p[1], R.q[2] - R.p[2], d);
+         seg_dir[i] = make_float4(d[0], d[1], d[2], 0.0f);
          const int64_t tx0 = ilo[0] / TX, tx1 = (ihi[0] - 1) / TX;
          const int64_t ty0 = ilo[1] / TY, ty1 = (ihi[1] - 1) / TY;
          const int64_t tz0 = ilo[2] / TZ, tz1 = (ihi[2] - 1) / TZ;
@@ -351,23 +375,6 @@ __global__ void k_ranges(const uint32_t *__restrict__ keys, int64_t n, uint32_t 
    const uint32_t t = keys[p];
    if (p == 0 || keys[p - 1] != t) range[2 * (int64_t)t] = (uint32_t)p;
    if (p == n - 1 || keys[p + 1] != t) range[2 * (int64_t)t + 1] = (uint32_t)(p + 1);
-}
-
-// vm::unit (vemath.hpp:390-402) rounded to float
-__device__ __forceinline__ void unit_to_float(double x, double y, double z, float *out) {
-   double n2 = 0.0;
-   n2 = n2 + x * x;
-   n2 = n2 + y * y;
-   n2 = n2 + z * z;
-   if (n2 > 0) {
-      const double inv = 1.0 / sqrt(n2);
-      x = x * inv;
-      y = y * inv;
-      z = z * inv;
-   }
-   out[0] = (float)x;
-   out[1] = (float)y;
-   out[2] = (float)z;
 }
 
 struct SegExact { // what the FP64 distance test reads of a segment
@@ -421,6 +428,43 @@ __device__ __forceinline__ Closest closest(const double pt[3], const SegExact &R
       r.t = b;
    }
    return r;
+}
+
+// the parameter t of ShortestPointToLineSegmentVecCalculation
+// (generator.cpp:475-494), with the operations of seg_exact and closest
+__device__ __forceinline__ double closest_t(const double pt[3], const double p[3],
+                                           const double q[3]) {
+   const SegExact E = seg_exact(p, q);
+   const double w0 = pt[0] - p[0], w1 = pt[1] - p[1], w2 = pt[2] - p[2];
+   double c1 = 0.0;
+   c1 = c1 + w0 * E.v[0];
+   c1 = c1 + w1 * E.v[1];
+   c1 = c1 + w2 * E.v[2];
+   if (c1 < 0) return 0.0;
+   if (E.c2 <= c1) return 1.0;
+   return c1 / E.c2;
+}
+
+struct WinRec { // what the final pass reads of a winning segment
+   double p[3], q[3], rp, rq;
+   int32_t layer_off, bundle;
+   float dir[3]; // unit(q - p) as float, the 'p' orientation
+};
+
+__device__ __forceinline__ WinRec load_winner(const SegRec &R, const float4 &d) {
+   WinRec o;
+   for (int k = 0; k < 3; k++) {
+      o.p[k] = R.p[k];
+      o.q[k] = R.q[k];
+   }
+   o.rp = R.rp;
+   o.rq = R.rq;
+   o.layer_off = R.layer_off;
+   o.bundle = R.bundle;
+   o.dir[0] = d.x;
+   o.dir[1] = d.y;
+   o.dir[2] = d.z;
+   return o;
 }
 
 __device__ __forceinline__ int tile_of_block(int bid, int n) {
@@ -596,7 +640,8 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
 // (run_batch) runs the exact FP64 test on 64 queued candidates at a time.
 template <typename SegT>
 __global__ void __launch_bounds__(WG)
-    k_voxelize(Params pr, const SegRec *__restrict__ segs, const uint32_t *__restrict__ list,
+    k_voxelize(Params pr, const SegRec *__restrict__ segs, const float4 *__restrict__ seg_dir,
+               const uint32_t *__restrict__ list,
                const uint32_t *__restrict__ range, const BundleDev *__restrict__ bd,
                const double *__restrict__ layer_sq, const uint8_t *__restrict__ layer_model,
                int only_label, int32_t *__restrict__ label, float *__restrict__ vector,
@@ -621,10 +666,15 @@ __global__ void __launch_bounds__(WG)
 
    // the layer tables in LDS when small (read through a generic pointer)
    __shared__ double lsq_lds[LSQ_LDS];
+   __shared__ uint8_t lmodel_lds[LSQ_LDS];
    const bool lsq_in_lds = pr.n_lsq <= LSQ_LDS;
    if (lsq_in_lds)
-      for (int i = threadIdx.x; i < pr.n_lsq; i += WG) lsq_lds[i] = layer_sq[i];
+      for (int i = threadIdx.x; i < pr.n_lsq; i += WG) {
+         lsq_lds[i] = layer_sq[i];
+         lmodel_lds[i] = layer_model[i];
+      }
    const double *lsq = lsq_in_lds ? (const double *)lsq_lds : layer_sq;
+   const uint8_t *lmodel = lsq_in_lds ? (const uint8_t *)lmodel_lds : layer_model;
 
    const uint32_t beg = range[2 * t], end = range[2 * t + 1];
    constexpr bool short_seg = sizeof(SegT) == 2;
@@ -657,106 +707,171 @@ __global__ void __launch_bounds__(WG)
 
       // Stage A set-up, one lane per record of the chunk: the record's integer
       // box within the tile and its z range narrowed by the nearest columns of
-      // the tile, then restricted to the wave's layers.  The in-box columns nearest to the segment's box bound every
-      // voxel's float ex and ey from below (float rounding is monotone), so a
-      // voxel that passes the float radius test RN(ez^2 + exy) <= thr_m has
-      // ez^2 <= rem + ulp(thr_m), rem = thr_m - exy_min >= 0; h covers that
-      // with margin (the hardware square root is within 1 ulp), and the range
-      // ends are computed in double, exact for any float box and voxel index.
-      int cnt = 0;
-      uint32_t geo = 0, magic = 0;
+      // the tile, then restricted to the wave's layers.  The in-box columns
+      // nearest to the segment's box bound every voxel's float ex and ey from
+      // below (float rounding is monotone), so a voxel that passes the float
+      // radius test RN(ez^2 + exy) <= thr has ez^2 <= rem + ulp(thr),
+      // rem = thr - exy_min >= 0; h covers that with margin (the hardware
+      // square root is within 1 ulp), and the range ends are computed in
+      // double, exact for any float box and voxel index.
+      int nl = 0, la = 0;
       if (lane < n) {
          const SegRec &R = chunk[lane].r;
          const int z0 = max(R.lo[2], zt), z1 = min(R.hi[2], zt + TZ);
          const int cx0 = max(R.lo[0], xt), cx1 = min(R.hi[0], xt + TX);
          const int cy0 = max(R.lo[1], yt), cy1 = min(R.hi[1], yt + TY);
          if (z0 < z1 && cx0 < cx1 && cy0 < cy1) {
-            const float thr_m = fmaf(R.thr, rel, absm);
+            const float thr = R.thr;
             const float exm =
                 fmaxf(fmaxf(R.bl[0] - (float)(cx1 - 1), (float)cx0 - R.bh[0]), 0.0f);
             const float eym =
                 fmaxf(fmaxf(R.bl[1] - (float)(cy1 - 1), (float)cy0 - R.bh[1]), 0.0f);
-            const float rem = thr_m - fmaf(eym, eym, exm * exm);
+            const float rem = thr - fmaf(eym, eym, exm * exm);
             if (rem >= 0.0f) {
-               const float h = fmaf(__builtin_amdgcn_sqrtf(fmaf(thr_m, 0x1p-20f, rem)),
+               const float h = fmaf(__builtin_amdgcn_sqrtf(fmaf(thr, 0x1p-20f, rem)),
                                     1.0f + 0x1p-10f, 0x1p-10f);
                const int za = max(z0, (int)ceil((double)R.bl[2] - (double)h));
                const int zb = min(z1, (int)floor((double)R.bh[2] + (double)h) + 1);
                // the wave's layers in [za, zb): z = zw + 4 * l, l in [la, lb)
-               const int la = (za - zw + 3) >> 2, lb = (zb - zw + 3) >> 2;
-               if (la < lb) {
-                  const int ncy = cy1 - cy0, nzz = lb - la;
-                  cnt = (cx1 - cx0) * ncy * nzz;
-                  geo = (uint32_t)(cx0 - xt) | (uint32_t)(cy0 - yt) << 3 |
-                        (uint32_t)la << 6 | (uint32_t)(ncy - 1) << 9 |
-                        (uint32_t)(nzz - 1) << 12;
-                  // floor(i / n) = (i * ceil(2^15 / n)) >> 15 for i < 512, n <= 8
-                  magic = (uint32_t)(32767 / nzz + 1) | (uint32_t)(32767 / ncy + 1) << 16;
-               }
+               const int lb = (zb - zw + 3) >> 2;
+               la = (za - zw + 3) >> 2;
+               nl = max(lb - la, 0);
             }
          }
       }
-      // candidates in record order: exclusive prefix of the counts
-      int incl = cnt;
+      // (record, layer) pairs in record order: exclusive prefix of the counts
+      int incl = nl;
       for (int off = 1; off < 64; off <<= 1) {
          const int v = __shfl_up(incl, off);
          if (lane >= off) incl += v;
       }
-      const int P = incl - cnt;
-      const int total = uniform(__shfl(incl, 63));
-      W.tab[lane] = make_uint2((uint32_t)P | geo << 16, magic);
-      wave_sync();
-      int cur = 0; // record holding the entry before the window
-      for (int wb = 0; wb < total; wb += 64) {
-         const int e = wb + lane;
-         // records whose candidates start inside the window, in order
-         unsigned long long st = __ballot((cnt > 0) & (P >= wb) & (P < wb + 64));
+      const int PL = incl - nl;
+      const int n_pairs = uniform(__shfl(incl, 63));
+      const int rec_info = PL << 3 | la;
+      int rcur = 0; // record holding the pair before the group
+
+      for (int gb = 0; gb < n_pairs; gb += 64) {
+         // Pair set-up, one lane per (record, layer) pair: the columns of the
+         // layer that can pass, a rectangle.  A voxel that passes the float
+         // test (lb = RN(ez^2 + RN(ey^2 + RN(ex^2))) <= thr, all terms >= 0)
+         // has ex^2 + ey^2 + ez^2 <= thr (1 + 2^-22), so ex^2 <= Rx =
+         // thr (1 + 2^-20) - eym^2 - ez^2 (computed in double: the squares of
+         // floats are exact, the rest is covered by the margin), with eym the
+         // least ey of the tile's in-box rows; likewise for ey.  hx covers
+         // sqrt(Rx) with a relative margin that also covers the float rounding
+         // of ex = RN(bl - x) or RN(x - bh).
+         const int p = gb + lane;
+         unsigned long long st = __ballot((nl > 0) & (PL >= gb) & (PL < gb + 64));
          st = ((unsigned long long)(uint32_t)uniform((int)(uint32_t)(st >> 32)) << 32) |
               (uint32_t)uniform((int)(uint32_t)st);
-         int c = cur;
+         int c = rcur;
          while (st) {
             const int s0 = __builtin_ctzll(st);
             st &= st - 1;
-            if (e >= __builtin_amdgcn_readlane(P, s0)) c = s0;
-            cur = s0;
+            if (p >= __builtin_amdgcn_readlane(PL, s0)) c = s0;
+            rcur = s0;
          }
-         bool pass = false;
-         uint32_t entry = 0;
-         if (e < total) {
-            const uint2 tb = W.tab[c];
-            const uint32_t g = tb.x >> 16;
-            const int li = e - (int)(tb.x & 0xffffu);
-            const int q1 = (int)(((uint32_t)li * (tb.y & 0xffffu)) >> 15);
-            const int iz = li - q1 * (int)(((g >> 12) & 7) + 1);
-            const int ix = (int)(((uint32_t)q1 * (tb.y >> 16)) >> 15);
-            const int iy = q1 - ix * (int)(((g >> 9) & 7) + 1);
-            const int lx = (int)(g & 7) + ix, ly = (int)((g >> 3) & 7) + iy;
-            const int lz = (int)((g >> 6) & 7) + iz;
-            const int slot = lz * 64 + (lx << 3 | ly);
+         const int ri = __shfl(rec_info, c);
+         int cnt = 0;
+         uint32_t geo = 0, magic = 0;
+         if (p < n_pairs) {
             const SegRec &R = chunk[c].r;
-            const float xf = (float)(xt + lx), yf = (float)(yt + ly), zf = (float)(zw + 4 * lz);
-            const float ex = fmaxf(fmaxf(R.bl[0] - xf, xf - R.bh[0]), 0.0f);
-            const float ey = fmaxf(fmaxf(R.bl[1] - yf, yf - R.bh[1]), 0.0f);
+            const int l = (ri & 7) + (p - (ri >> 3));
+            const int cx0 = max(R.lo[0], xt), cx1 = min(R.hi[0], xt + TX);
+            const int cy0 = max(R.lo[1], yt), cy1 = min(R.hi[1], yt + TY);
+            const float exm =
+                fmaxf(fmaxf(R.bl[0] - (float)(cx1 - 1), (float)cx0 - R.bh[0]), 0.0f);
+            const float eym =
+                fmaxf(fmaxf(R.bl[1] - (float)(cy1 - 1), (float)cy0 - R.bh[1]), 0.0f);
+            const float zf = (float)(zw + 4 * l);
             const float ez = fmaxf(fmaxf(R.bl[2] - zf, zf - R.bh[2]), 0.0f);
-            const float lb = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-            const float stored = __uint_as_float((uint32_t)(W.key[slot] >> 32));
-            pass = (lb <= fmaf(R.thr, rel, absm)) & (lb <= fmaf(stored, rel, absm));
-            entry = ((uint32_t)c << 9) | (uint32_t)slot;
-         }
-         const unsigned long long m = __ballot(pass);
-         if (m) {
-            const int pos = q_tail + (int)__builtin_amdgcn_mbcnt_hi(
-                                         (uint32_t)(m >> 32),
-                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            if (pass) W.queue[pos & (QN - 1)] = entry;
-            q_tail += __popcll(m);
-            if (q_tail - q_head >= 64) {
-               wave_sync();
-               run_batch(W, chunk, chunk_idx, 64, q_head, lane, xt, yt, zw, max_layer, lsq,
-                         flags, ov);
-               q_head += 64;
+            const double tb = (double)R.thr * (1.0 + 0x1p-20) - (double)ez * (double)ez;
+            const double rx = tb - (double)eym * (double)eym;
+            const double ry = tb - (double)exm * (double)exm;
+            if (rx >= 0.0 && ry >= 0.0) {
+               const float hx =
+                   fmaf(__builtin_amdgcn_sqrtf((float)rx), 1.0f + 0x1p-10f, 0x1p-10f);
+               const float hy =
+                   fmaf(__builtin_amdgcn_sqrtf((float)ry), 1.0f + 0x1p-10f, 0x1p-10f);
+               const int xa = max(cx0, (int)ceil((double)R.bl[0] - (double)hx));
+               const int xb = min(cx1, (int)floor((double)R.bh[0] + (double)hx) + 1);
+               const int ya = max(cy0, (int)ceil((double)R.bl[1] - (double)hy));
+               const int yb = min(cy1, (int)floor((double)R.bh[1] + (double)hy) + 1);
+               if (xa < xb && ya < yb) {
+                  const int ncy = yb - ya;
+                  cnt = (xb - xa) * ncy;
+                  geo = (uint32_t)(xa - xt) | (uint32_t)(ya - yt) << 3 | (uint32_t)l << 6 |
+                        (uint32_t)(ncy - 1) << 9;
+                  // floor(i / n) = (i * ceil(2^15 / n)) >> 15 for i < 512, n <= 8
+                  magic = (uint32_t)(32767 / ncy + 1) | (uint32_t)c << 16;
+               }
             }
          }
+         // candidates of the group in pair order
+         int ci = cnt;
+         for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(ci, off);
+            if (lane >= off) ci += v;
+         }
+         const int P = ci - cnt;
+         const int total = uniform(__shfl(ci, 63));
+         W.tab[lane] = make_uint2((uint32_t)P | geo << 13, magic);
+         wave_sync();
+
+         // Stage A test, 64 candidates per window
+         int cur = 0; // pair holding the entry before the window
+         for (int wb = 0; wb < total; wb += 64) {
+            const int e = wb + lane;
+            // pairs whose candidates start inside the window, in order
+            unsigned long long sp = __ballot((cnt > 0) & (P >= wb) & (P < wb + 64));
+            sp = ((unsigned long long)(uint32_t)uniform((int)(uint32_t)(sp >> 32)) << 32) |
+                 (uint32_t)uniform((int)(uint32_t)sp);
+            int k = cur;
+            while (sp) {
+               const int s0 = __builtin_ctzll(sp);
+               sp &= sp - 1;
+               if (e >= __builtin_amdgcn_readlane(P, s0)) k = s0;
+               cur = s0;
+            }
+            bool pass = false;
+            uint32_t entry = 0;
+            if (e < total) {
+               const uint2 tbk = W.tab[k];
+               const uint32_t g = tbk.x >> 13;
+               const uint32_t li = (uint32_t)e - (tbk.x & 0x1fffu);
+               const uint32_t ix = __umul24(li, tbk.y & 0xffffu) >> 15;
+               const uint32_t iy = li - __umul24(ix, ((g >> 9) & 7) + 1);
+               const uint32_t lx = (g & 7) + ix, ly = ((g >> 3) & 7) + iy, lz = (g >> 6) & 7;
+               const uint32_t slot = lz << 6 | lx << 3 | ly;
+               const uint32_t rc = tbk.y >> 16;
+               const SegRec &R =
+                   *(const SegRec *)((const char *)chunk + __umul24(rc, (uint32_t)sizeof(LdsSeg)));
+               const float xf = (float)(xt + (int)lx), yf = (float)(yt + (int)ly);
+               const float zf = (float)(zw + 4 * (int)lz);
+               const float ex = fmaxf(fmaxf(R.bl[0] - xf, xf - R.bh[0]), 0.0f);
+               const float ey = fmaxf(fmaxf(R.bl[1] - yf, yf - R.bh[1]), 0.0f);
+               const float ez = fmaxf(fmaxf(R.bl[2] - zf, zf - R.bh[2]), 0.0f);
+               const float lb = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+               const float stored = __uint_as_float((uint32_t)(W.key[slot] >> 32));
+               pass = (lb <= R.thr) & (lb <= fmaf(stored, rel, absm));
+               entry = rc << 9 | slot;
+            }
+            const unsigned long long m = __ballot(pass);
+            if (m) {
+               const int pos = q_tail + (int)__builtin_amdgcn_mbcnt_hi(
+                                            (uint32_t)(m >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+               if (pass) W.queue[pos & (QN - 1)] = entry;
+               q_tail += __popcll(m);
+               if (q_tail - q_head >= 64) {
+                  wave_sync();
+                  run_batch(W, chunk, chunk_idx, 64, q_head, lane, xt, yt, zw, max_layer, lsq,
+                            flags, ov);
+                  q_head += 64;
+               }
+            }
+         }
+         wave_sync(); // the table is rewritten by the next group
       }
       // drain: the queue refers to this chunk's records
       wave_sync();
@@ -771,60 +886,63 @@ __global__ void __launch_bounds__(WG)
                             0)) != 0;
    }
 
-   // results: eight columns of eight voxels per pass, z fastest over the lanes
+   // Results, eight columns of eight voxels per pass.  The winners' records
+   // are fetched ahead: the global segment index (a load from the tile list
+   // for 16-bit positions) two passes ahead, the record one pass ahead.
    wave_sync();
+   auto winner = [&](int j) -> int32_t {
+      const uint32_t w = (uint32_t)W.seg[(lane & 7) * 64 + j * 8 + (lane >> 3)];
+      return short_seg ? (w ? (int32_t)list[beg + w - 1u] : -1) : (int32_t)w;
+   };
+   int32_t sg = winner(0), sg1 = winner(1);
+   WinRec cur, nxt;
+   if (sg >= 0) cur = load_winner(segs[sg], seg_dir[sg]);
    for (int j = 0; j < 8; j++) {
       const int col = j * 8 + (lane >> 3), zz = lane & 7;
       const int slot = zz * 64 + col;
+      const int32_t sg2 = j < 6 ? winner(j + 2) : -1;
+      if (j < 7 && sg1 >= 0) nxt = load_winner(segs[sg1], seg_dir[sg1]);
       const int vx = xt + (col >> 3), vy = yt + (col & 7), vz = zw + 4 * zz;
-      if (vx >= pr.nx || vy >= pr.ny || vz >= pr.nz) continue;
-      const int64_t ind = ((int64_t)vx * pr.ny + vy) * pr.nz + vz;
-      int32_t lab = (int32_t)(uint32_t)(W.key[slot] & 0x7FFFFFFFull);
-      const int32_t sg = short_seg ? (W.seg[slot] ? (int32_t)list[beg + W.seg[slot] - 1u] : -1)
-                                   : (int32_t)W.seg[slot];
-      const double pt[3] = {vx + 0.5, vy + 0.5, vz + 0.5};
-      SegExact E;
-      Closest cp;
-      int32_t layer_off = 0, bundle = 0;
-      if (sg >= 0) {
-         const SegRec &R = segs[sg];
-         const double p[3] = {R.p[0], R.p[1], R.p[2]};
-         const double q[3] = {R.q[0], R.q[1], R.q[2]};
-         E = seg_exact(p, q);
-         layer_off = R.layer_off;
-         bundle = R.bundle;
-         if (lab == LABEL_LATER) {
-            // the winner's label from the same arithmetic as its exact test
-            cp = closest(pt, E);
-            const double d0 = cp.c[0] - pt[0], d1 = cp.c[1] - pt[1], d2v = cp.c[2] - pt[2];
-            double dd = 0.0;
-            dd = dd + d0 * d0;
-            dd = dd + d1 * d1;
-            dd = dd + d2v * d2v;
-            const double rr = R.rp * (1 - cp.t) + R.rq * cp.t;
-            lab = layer_label((float)dd, (float)(rr * rr), lsq, layer_off, bundle, max_layer);
+      if (vx < pr.nx && vy < pr.ny && vz < pr.nz) {
+         const int64_t ind = ((int64_t)vx * pr.ny + vy) * pr.nz + vz;
+         const unsigned long long key = W.key[slot];
+         int32_t lab = (int32_t)(uint32_t)(key & 0x7FFFFFFFull);
+         const double pt[3] = {vx + 0.5, vy + 0.5, vz + 0.5};
+         if (sg >= 0 && lab == LABEL_LATER) {
+            // the winner's label from its exact test: the stored float distance
+            // and the closest point's parameter, computed as in that test
+            const double t = closest_t(pt, cur.p, cur.q);
+            const double rr = cur.rp * (1 - t) + cur.rq * t;
+            lab = layer_label(__uint_as_float((uint32_t)(key >> 32)), (float)(rr * rr), lsq,
+                              cur.layer_off, cur.bundle, max_layer);
          }
-      }
-      label[ind] = lab;
-      if (!only_label) {
-         float vo[3] = {0.0f, 0.0f, 0.0f};
-         if (sg >= 0) {
-            // a float ratio just above the last scale^2 gives the past-the-end
-            // layer index; the padded model table repeats the outermost
-            // layer's orientation there
-            const int ly = lab - (1 + bundle * max_layer);
-            const uint8_t m = layer_model[layer_off + ly];
-            if (m == 'p') {
-               unit_to_float(E.v[0], E.v[1], E.v[2], vo);
-            } else if (m == 'r') {
-               cp = closest(pt, E);
-               unit_to_float(pt[0] - cp.c[0], pt[1] - cp.c[1], pt[2] - cp.c[2], vo);
+         label[ind] = lab;
+         if (!only_label) {
+            float vo[3] = {0.0f, 0.0f, 0.0f};
+            if (sg >= 0) {
+               // a float ratio just above the last scale^2 gives the past-the-end
+               // layer index; the padded model table repeats the outermost
+               // layer's orientation there
+               const int ly = lab - (1 + cur.bundle * max_layer);
+               const uint8_t m = lmodel[cur.layer_off + ly];
+               if (m == 'p') {
+                  vo[0] = cur.dir[0];
+                  vo[1] = cur.dir[1];
+                  vo[2] = cur.dir[2];
+               } else if (m == 'r') {
+                  const SegExact E = seg_exact(cur.p, cur.q);
+                  const Closest cp = closest(pt, E);
+                  unit_to_float(pt[0] - cp.c[0], pt[1] - cp.c[1], pt[2] - cp.c[2], vo);
+               }
             }
+            vector[3 * ind + 0] = vo[0];
+            vector[3 * ind + 1] = vo[1];
+            vector[3 * ind + 2] = vo[2];
          }
-         vector[3 * ind + 0] = vo[0];
-         vector[3 * ind + 1] = vo[1];
-         vector[3 * ind + 2] = vo[2];
       }
+      cur = nxt;
+      sg = sg1;
+      sg1 = sg2;
    }
 }
 
@@ -951,7 +1069,8 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
    void *boxes, *fok, *segs, *cnt, *off, *flags;
    FPL_TRY(ws_get(st, WS_BOXES, sizeof(double) * 6 * (nf + nb + 1), &boxes));
    FPL_TRY(ws_get(st, WS_FIBER_OK, nf + 1, &fok));
-   FPL_TRY(ws_get(st, WS_SEGS, sizeof(SegRec) * (np + 1), &segs));
+   FPL_TRY(ws_get(st, WS_SEGS, (sizeof(SegRec) + sizeof(float4)) * (np + 1), &segs));
+   float4 *seg_dir = (float4 *)((SegRec *)segs + (np + 1));
    FPL_TRY(ws_get(st, WS_SEG_COUNT, sizeof(int64_t) * (np + 1), &cnt));
    FPL_TRY(ws_get(st, WS_SEG_OFF, sizeof(int64_t) * (np + 2), &off));
    FPL_TRY(ws_get(st, WS_FLAGS, 64, &flags));
@@ -973,7 +1092,7 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
    if (np > 0) {
       hipLaunchKernelGGL(k_segments, dim3(grid_for(np, B)), dim3(B), 0, st, pr, fbs->points,
                          fbs->fiber_ptr, d_bd, (const uint8_t *)fok, (SegRec *)segs,
-                         (int64_t *)cnt);
+                         seg_dir, (int64_t *)cnt);
       // exclusive scan over np + 1 entries (the last one is 0) gives the total
       FPL_HIP(hipMemsetAsync((int64_t *)cnt + np, 0, sizeof(int64_t), st));
       size_t tmp_bytes = 0;
@@ -1019,7 +1138,7 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
    }
    KernelTimer kt("voxelize", st);
    hipLaunchKernelGGL(k_voxelize<uint16_t>, dim3((unsigned)n_tiles), dim3(WG), 0, st, pr,
-                      (const SegRec *)segs, (const uint32_t *)vb, (const uint32_t *)rng, d_bd,
+                      (const SegRec *)segs, seg_dir, (const uint32_t *)vb, (const uint32_t *)rng, d_bd,
                       d_lsq, d_lm, only_label, label, vector, d_flags);
    kt.stop();
    FPL_HIP(hipGetLastError());
@@ -1031,7 +1150,7 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
       // winner indices (every voxel is rewritten; warnings are an OR)
       KernelTimer kt2("voxelize", st);
       hipLaunchKernelGGL(k_voxelize<int32_t>, dim3((unsigned)n_tiles), dim3(WG), 0, st, pr,
-                         (const SegRec *)segs, (const uint32_t *)vb, (const uint32_t *)rng, d_bd,
+                         (const SegRec *)segs, seg_dir, (const uint32_t *)vb, (const uint32_t *)rng, d_bd,
                          d_lsq, d_lm, only_label, label, vector, d_flags);
       kt2.stop();
       FPL_HIP(hipGetLastError());
