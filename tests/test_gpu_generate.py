@@ -127,6 +127,33 @@ def test_generate_long_tile_list():
     _compare(fbs, (8, 8, 8), [0.0] * 3, 1.0)
 
 
+def test_generate_thick_and_degenerate_segments():
+    # radii up to 12 voxels (a segment spans many tiles and all eight layers of
+    # every wave: several (segment, layer) groups per chunk), random directions,
+    # zero-length and very short segments, and axis-aligned fibers on voxel
+    # centres with integer radii (distances equal to the outer radius, equal
+    # distances of two fibers)
+    _say('generate: thick, degenerate and tied segments 64 x 48 x 80')
+    rng = np.random.default_rng(21)
+    thick = []
+    for _ in range(6):
+        start = rng.uniform([0, 0, 0], [64, 48, 80])
+        pts = np.vstack([start, start + np.cumsum(rng.normal(scale=4.0, size=(4, 3)), axis=0)])
+        pts[2] = pts[1]  # zero-length segment
+        pts[3] = pts[2] + 1e-3 * rng.normal(size=3)  # very short segment
+        r = rng.uniform(2.0, 12.0, size=(5, 1))
+        thick.append(np.hstack([pts, r]))
+    tied = []
+    for y, z in ((20.5, 30.5), (26.5, 30.5), (23.5, 40.5)):
+        tied.append(np.array([[-2.0, y, z, 3.0], [30.5, y, z, 3.0], [66.0, y, z, 3.0]]))
+    fbs = objects.FiberBundles([thick, tied],
+                               [[(0.3, 0, 1, 'b'), (0.7, 0.004, 2, 'r'), (1.0, -0.004, 1, 'p')],
+                                [(0.5, -0.004, 1, 'p'), (1.0, 0.0, 3, 'b')]])
+    with warnings.catch_warnings():
+        warnings.simplefilter('ignore')
+        _compare(fbs, (64, 48, 80), [0.0] * 3, 1.0)
+
+
 def test_generate_overlap_warning():
     _say('generate: overlap warning')
     fbs = objects.FiberBundles([[[[0, 3, 3, 2], [8, 3, 3, 2]]], [[[3, 0, 3, 2], [3, 8, 3, 2]]]],
