@@ -77,7 +77,7 @@ EXPORTS = ('fpl_num_properties', 'fpl_generate', 'fpl_simulate', 'fpl_optic_resa
            'fpl_count_voxel_steps', 'fpl_profile_enable', 'fpl_profile_read',
            'fpl_release_workspace', 'fpl_version', 'fpl_server_create', 'fpl_server_add_tissue',
            'fpl_server_remove_tissue', 'fpl_server_submit', 'fpl_server_wait',
-           'fpl_server_stats', 'fpl_server_destroy', 'fpl_rofl')
+           'fpl_server_stats', 'fpl_server_destroy', 'fpl_rofl', 'fpl_simulate_adjoint')
 
 
 def lib():
@@ -99,6 +99,11 @@ def lib():
         L.fpl_simulate.argtypes = [
             ctypes.POINTER(FplSetup), _c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_i64, _c_p,
             ctypes.POINTER(ctypes.c_uint32), _c_p, ctypes.c_char_p, ctypes.c_size_t
+        ]
+        L.fpl_simulate_adjoint.restype = ctypes.c_int
+        L.fpl_simulate_adjoint.argtypes = [
+            ctypes.POINTER(FplSetup), _c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_i64, _c_p, _c_p,
+            _c_p, _c_p, ctypes.c_char_p, ctypes.c_size_t
         ]
         L.fpl_optic_resample.restype = ctypes.c_int
         L.fpl_optic_resample.argtypes = [
@@ -230,6 +235,22 @@ def simulate(setup, dim, label, vector, props, tilts, images):
                             512)
     check(rc, msg)
     return warn.value
+
+
+def simulate_adjoint(setup, dim, label, vector, props, tilts, grad_images, grad_vector=None):
+    """fpl_simulate_adjoint on device tensors; returns dL/d props as (n_props, 2) float64."""
+    props = np.ascontiguousarray(props, dtype=np.float64)
+    tilts = np.ascontiguousarray(tilts, dtype=np.float64).reshape(-1, 2)
+    dim_arr = np.ascontiguousarray(dim, dtype=np.int64)
+    grad_props = np.zeros(props.shape, dtype=np.float64)
+    msg = ctypes.create_string_buffer(512)
+    rc = lib().fpl_simulate_adjoint(ctypes.byref(setup), dim_arr.ctypes.data, _ptr(label),
+                                    _ptr(vector), props.ctypes.data, props.shape[0],
+                                    tilts.ctypes.data, tilts.shape[0], _ptr(grad_images),
+                                    grad_props.ctypes.data, _ptr(grad_vector), _stream(), msg,
+                                    512)
+    check(rc, msg)
+    return grad_props
 
 
 def count_voxel_steps(setup, dim, theta, phi):

@@ -75,6 +75,7 @@ enum Slot : int {
    WS_PACKED,      // simulation: packed (label, axis) records in [x][z][y] order
    WS_PACKED_B,    // simulation: second array of the angle storage
    WS_ANALYSIS,    // optic: two image-sized temporaries
+   WS_ADJOINT,     // adjoint march: checkpoints + the gradient table
    WS_NUM_SLOTS
 };
 

@@ -19,7 +19,9 @@
 // (Rodrigues form, 1 - cos d without cancellation), sum mu t likewise; ray positions
 // are stepped in FP64 exactly as the reference so that the visited voxels are
 // identical.
+#include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include <hip/hip_fp16.h>
@@ -449,6 +451,317 @@ __global__ void __launch_bounds__(WG, 8)
    if (f) atomicOr(&flags[0], f);
 }
 
+// ------------------------------------------------------------------ adjoint
+// Reverse mode of the march: the gradient of L = sum(g * images) with respect
+// to the property table (dn, mu of every label) and, for nearest-neighbour
+// views, the per-voxel optical axis.  The reference has no derivative; the
+// forward it differentiates is the march above (the reference's per-rho
+// chain in the rho-invariant form).
+//
+// Per ray (one thread, tilt-interleaved workgroups as in k_march):
+//   forward   the march, keeping a checkpoint (FP64 position, Poincare state
+//             u) before every CK-th step;
+//   seeds     I_r = b (1 - c_r u1 + s_r u2), b = I0/4 exp(-M):
+//             dL/du = b (-sum g_r c_r, sum g_r s_r, 0), dL/dM = -sum g_r I_r;
+//   reverse   segment by segment from the last: replay the CK steps from the
+//             checkpoint (the states before each step go to LDS), then walk
+//             them backwards.  A retarder step is u' = (I - sin d K + (1 - cos d)
+//             K^2) u with K = [n]x, n = (C, -S, 0); its adjoint is
+//             ubar <- (I + sin d K + (1 - cos d) K^2) ubar, and
+//             dbar = ubar . du'/dd, Cbar = ubar . du'/dC, Sbar = ubar . du'/dS give
+//             dL/dK = dbar |v| cos^2(alpha) and, through C, S and d as functions of
+//             the axis, dL/dv.  Every step that attenuates adds dL/dM to mu.
+// Per-label sums run along the ray while the label stays the same, then go to
+// LDS (FP64 atomics) and once per workgroup to the device table.
+constexpr int CK = 8;              // steps per checkpoint segment
+constexpr int MAX_ADJ_PROPS = 256; // labels with LDS accumulators
+
+struct Ckpt { // state before the first step of a segment
+   double p[3];
+   float u[3];
+   float pad;
+};
+
+// What one z-step of the march does at a position (k_march's arithmetic).
+struct StepEval {
+   int kind;   // 0: zero axis, skipped; 1: attenuation only; 2: retarder
+   int32_t lab;
+   float K, sd, omc, C, S, g; // g = |v| cos^2(alpha), delta = K g
+   float3 v;                  // the axis after the tilt rotation
+   int32_t vx, vy, vz;        // the voxel read (flip_z applied to vz)
+};
+
+template <typename Idx>
+__device__ __forceinline__ StepEval eval_step(const Params &P, const TiltDev &T,
+                                              const PropDev *sp, const PropDev *props,
+                                              const Field<Idx, ST_F32> &F, double px, double py,
+                                              double pz) {
+   StepEval E;
+   const int32_t ix = ifloor(px), iy = ifloor(py), iz = ifloor(floor(pz));
+   Vox r{0, make_float3(0.0f, 0.0f, 0.0f), false};
+   if (F.inside(ix, iy, iz)) r = F.at(ix, iy, iz);
+   const int32_t lab = (r.lab < 0 || r.lab >= P.n_props) ? 0 : r.lab;
+   const PropDev pd = lab < MAX_ADJ_PROPS ? sp[lab] : props[lab];
+   E.lab = lab;
+   E.K = pd.K;
+   E.vx = ix;
+   E.vy = iy;
+   E.vz = F.flip_z ? F.nz - 1 - iz : iz;
+   if (pd.bg) {
+      E.kind = 1;
+      return E;
+   }
+   float3 v = (P.interp == 0 || T.centers)
+                  ? r.v
+                  : get_vec_interp(F, px, py, pz, lab, r.v, P.interp == 2);
+   if (is_zero(v)) {
+      E.kind = 0;
+      return E;
+   }
+   const float n2 = dot3(v, v);
+   const float nrm = sqrtf(n2);
+   if (T.rotate) {
+      const float3 w = v;
+      v.x = T.R[0] * w.x + T.R[1] * w.y + T.R[2] * w.z;
+      v.y = T.R[3] * w.x + T.R[4] * w.y + T.R[5] * w.z;
+      v.z = T.R[6] * w.x + T.R[7] * w.y + T.R[8] * w.z;
+   }
+   const float xx = v.x * v.x, yy = v.y * v.y;
+   const float rxy = xx + yy;
+   const float rr = rxy + v.z * v.z;
+   E.g = nrm * (rxy / rr);
+   const float delta = (pd.K * nrm) * (rxy / rr);
+   E.C = 1.0f;
+   E.S = 0.0f;
+   if (rxy > 0.0f) {
+      const float inv = 1.0f / rxy;
+      E.C = (xx - yy) * inv;
+      E.S = 2.0f * v.x * v.y * inv;
+   }
+   sin_omc(delta, E.sd, E.omc);
+   E.v = v;
+   E.kind = 2;
+   return E;
+}
+
+template <typename Idx>
+__global__ void __launch_bounds__(WG)
+    k_adjoint(const Params *__restrict__ prm, const TiltDev *__restrict__ tilts,
+              const PropDev *__restrict__ props, const void *__restrict__ rec_a,
+              const double *__restrict__ grad_images, Ckpt *__restrict__ ck, int64_t ck_stride,
+              double kfac, double mfac, double *__restrict__ g_props, float *__restrict__ g_vec) {
+   __shared__ PropDev sp[MAX_ADJ_PROPS];
+   __shared__ double acc[MAX_ADJ_PROPS][2];
+   __shared__ float ub[CK * 3][WG]; // states before the steps of the current segment
+   const Params &P = *prm;
+   const int np_lds = P.n_props < MAX_ADJ_PROPS ? (int)P.n_props : MAX_ADJ_PROPS;
+   for (int k = threadIdx.x; k < np_lds; k += WG) {
+      sp[k] = props[k];
+      acc[k][0] = 0.0;
+      acc[k][1] = 0.0;
+   }
+   __syncthreads();
+
+   const int tilt = (int)(blockIdx.x % (unsigned)P.n_tilts);
+   const int64_t tile = blockIdx.x / (unsigned)P.n_tilts;
+   const int64_t nty = (P.ny + RAYS_Y - 1) / RAYS_Y;
+   const int64_t i = (tile / nty) * RAYS_X + (threadIdx.x >> 6);
+   const int64_t j = (tile % nty) * RAYS_Y + (threadIdx.x & 63);
+   const int tid = threadIdx.x;
+   // no early return: the workgroup meets again to flush its label sums
+   if (i < P.nx && j < P.ny) {
+      const TiltDev &T = tilts[tilt];
+      const int64_t nxy = P.nx * P.ny;
+      const int64_t ray = i * P.ny + j;
+      const Field<Idx, ST_F32> F{rec_a, nullptr, (int32_t)P.nx, (int32_t)P.ny, (int32_t)P.nz,
+                                 P.flip_z};
+      Ckpt *my = ck + ((int64_t)tilt * nxy + ray) * ck_stride;
+
+      double px, py, pz = 0.0;
+      if (P.untilt) {
+         px = ((double)i + 0.5) - T.half_shift[0];
+         py = ((double)j + 0.5) - T.half_shift[1];
+      } else {
+         tilted_start(T, i, j, px, py);
+      }
+      px = px + T.half_step[0];
+      py = py + T.half_step[1];
+      pz = pz + T.half_step[2];
+      const double sx = T.step[0], sy = T.step[1], sz = T.step[2];
+      const double nzd = (double)P.nz;
+
+      // forward with checkpoints
+      float h1 = 0.0f, h2 = 0.0f, h3 = -1.0f, l1 = 0.0f, l2 = 0.0f, l3 = 0.0f;
+      float mh = 0.0f, ml = 0.0f;
+      int64_t n = 0;
+      for (;; n++) {
+         const double fz = floor(pz);
+         if (!(fz >= 0.0 && fz < nzd)) break;
+         if (n % CK == 0) {
+            Ckpt c;
+            c.p[0] = px;
+            c.p[1] = py;
+            c.p[2] = pz;
+            c.u[0] = h1 + l1;
+            c.u[1] = h2 + l2;
+            c.u[2] = h3 + l3;
+            c.pad = 0.0f;
+            my[n / CK] = c;
+         }
+         const StepEval E = eval_step(P, T, sp, props, F, px, py, pz);
+         if (E.kind == 1) {
+            const float mt = E.lab < MAX_ADJ_PROPS ? sp[E.lab].mu_t : props[E.lab].mu_t;
+            if (mt != 0.0f) two_sum(mh, ml, mt);
+         } else if (E.kind == 2) {
+            const float nv = E.C * h1 - E.S * h2;
+            const float d1 = E.sd * E.S * h3 + E.omc * (E.C * nv - h1);
+            const float d2 = E.sd * E.C * h3 + E.omc * (-E.S * nv - h2);
+            const float d3 = -E.sd * (E.C * h2 + E.S * h1) - E.omc * h3;
+            two_sum(h1, l1, d1);
+            two_sum(h2, l2, d2);
+            two_sum(h3, l3, d3);
+            const float mt = E.lab < MAX_ADJ_PROPS ? sp[E.lab].mu_t : props[E.lab].mu_t;
+            if (mt != 0.0f) two_sum(mh, ml, mt);
+         }
+         px = px + sx;
+         py = py + sy;
+         pz = pz + sz;
+      }
+
+      // seeds from the intensities
+      const double A = exp(-((double)mh + (double)ml));
+      const double u1 = (double)h1 + (double)l1, u2 = (double)h2 + (double)l2;
+      const double base = P.quarter_I0 * A;
+      const double *g = grad_images + ((int64_t)tilt * nxy + ray) * P.n_rho;
+      double gc = 0.0, gs = 0.0, mbar = 0.0;
+      for (int k = 0; k < P.n_rho; k++) {
+         const double I = base * (1.0 - (P.c2r[k] * u1 - P.s2r[k] * u2));
+         mbar -= g[k] * I;
+         gc += g[k] * P.c2r[k];
+         gs += g[k] * P.s2r[k];
+      }
+      float b1 = (float)(-base * gc), b2 = (float)(base * gs), b3 = 0.0f;
+
+      // the sums of the current label run
+      int32_t run_lab = -1;
+      double run_k = 0.0;
+      int64_t run_n = 0;
+      auto flush = [&]() {
+         if (run_lab < 0) return;
+         const double dk = run_k * kfac, dm = mbar * (double)run_n * mfac;
+         if (run_lab < MAX_ADJ_PROPS) {
+            atomicAdd(&acc[run_lab][0], dk);
+            atomicAdd(&acc[run_lab][1], dm);
+         } else {
+            atomicAdd(&g_props[2 * run_lab], dk);
+            atomicAdd(&g_props[2 * run_lab + 1], dm);
+         }
+      };
+
+      for (int64_t sgm = (n - 1) / CK; sgm >= 0 && n > 0; sgm--) {
+         const Ckpt c = my[sgm];
+         const int len = (int)min((int64_t)CK, n - sgm * CK);
+         // replay: the states before each step of the segment
+         double qx = c.p[0], qy = c.p[1], qz = c.p[2];
+         float a1 = c.u[0], a2 = c.u[1], a3 = c.u[2], e1 = 0.0f, e2 = 0.0f, e3 = 0.0f;
+         for (int t = 0; t < len; t++) {
+            ub[3 * t + 0][tid] = a1 + e1;
+            ub[3 * t + 1][tid] = a2 + e2;
+            ub[3 * t + 2][tid] = a3 + e3;
+            const StepEval E = eval_step(P, T, sp, props, F, qx, qy, qz);
+            if (E.kind == 2) {
+               const float nv = E.C * a1 - E.S * a2;
+               const float d1 = E.sd * E.S * a3 + E.omc * (E.C * nv - a1);
+               const float d2 = E.sd * E.C * a3 + E.omc * (-E.S * nv - a2);
+               const float d3 = -E.sd * (E.C * a2 + E.S * a1) - E.omc * a3;
+               two_sum(a1, e1, d1);
+               two_sum(a2, e2, d2);
+               two_sum(a3, e3, d3);
+            }
+            qx = qx + sx;
+            qy = qy + sy;
+            qz = qz + sz;
+         }
+         // backwards through the segment
+         for (int t = len - 1; t >= 0; t--) {
+            double rx = c.p[0], ry = c.p[1], rz = c.p[2];
+            for (int q = 0; q < t; q++) {
+               rx = rx + sx;
+               ry = ry + sy;
+               rz = rz + sz;
+            }
+            const StepEval E = eval_step(P, T, sp, props, F, rx, ry, rz);
+            if (E.kind == 0) continue;
+            if (E.lab != run_lab) {
+               flush();
+               run_lab = E.lab;
+               run_k = 0.0;
+               run_n = 0;
+            }
+            run_n++;
+            if (E.kind == 1) continue;
+            const float x1 = ub[3 * t + 0][tid], x2 = ub[3 * t + 1][tid], x3 = ub[3 * t + 2][tid];
+            const float sd = E.sd, omc = E.omc, co = 1.0f - omc, C = E.C, S = E.S;
+            const float nv = C * x1 - S * x2;
+            // derivatives of the state after the step
+            const float dd1 = co * S * x3 + sd * (C * nv - x1);
+            const float dd2 = co * C * x3 + sd * (-S * nv - x2);
+            const float dd3 = -co * (C * x2 + S * x1) - sd * x3;
+            const float dbar = b1 * dd1 + b2 * dd2 + b3 * dd3;
+            const float cbar = b1 * (omc * (nv + C * x1)) + b2 * (sd * x3 - omc * S * x1) +
+                               b3 * (-sd * x2);
+            const float sbar = b1 * (sd * x3 - omc * C * x2) + b2 * (omc * (S * x2 - nv)) +
+                               b3 * (-sd * x1);
+            run_k += (double)(dbar * E.g);
+            if (g_vec) {
+               const float vx = E.v.x, vy = E.v.y, vz = E.v.z;
+               const float rxy = vx * vx + vy * vy;
+               if (rxy > 0.0f) {
+                  const float n2 = rxy + vz * vz, nr = sqrtf(n2);
+                  const float in3 = 1.0f / (n2 * nr), ir2 = 1.0f / (rxy * rxy);
+                  // delta = K rxy / |v|
+                  const float ddx = E.K * (2.0f * vx / nr - rxy * vx * in3);
+                  const float ddy = E.K * (2.0f * vy / nr - rxy * vy * in3);
+                  const float ddz = -E.K * rxy * vz * in3;
+                  // C = (x^2 - y^2) / rxy, S = 2 x y / rxy
+                  const float dcx = 4.0f * vx * vy * vy * ir2, dcy = -4.0f * vy * vx * vx * ir2;
+                  const float dsx = 2.0f * vy * (vy * vy - vx * vx) * ir2;
+                  const float dsy = 2.0f * vx * (vx * vx - vy * vy) * ir2;
+                  float w0 = dbar * ddx + cbar * dcx + sbar * dsx;
+                  float w1 = dbar * ddy + cbar * dcy + sbar * dsy;
+                  float w2 = dbar * ddz;
+                  if (T.rotate) { // back through v' = R v
+                     const float a = T.R[0] * w0 + T.R[3] * w1 + T.R[6] * w2;
+                     const float b = T.R[1] * w0 + T.R[4] * w1 + T.R[7] * w2;
+                     const float cc = T.R[2] * w0 + T.R[5] * w1 + T.R[8] * w2;
+                     w0 = a;
+                     w1 = b;
+                     w2 = cc;
+                  }
+                  float *o = g_vec + 3 * (((int64_t)E.vx * P.ny + E.vy) * P.nz + E.vz);
+                  unsafeAtomicAdd(o + 0, w0);
+                  unsafeAtomicAdd(o + 1, w1);
+                  unsafeAtomicAdd(o + 2, w2);
+               }
+            }
+            // ubar <- (I + sin d K + (1 - cos d) K^2) ubar
+            const float nu = C * b1 - S * b2;
+            const float k1 = -S * b3, k2 = -C * b3, k3 = S * b1 + C * b2;
+            const float q1 = C * nu - b1, q2 = -S * nu - b2, q3 = -b3;
+            b1 = b1 + sd * k1 + omc * q1;
+            b2 = b2 + sd * k2 + omc * q2;
+            b3 = b3 + sd * k3 + omc * q3;
+         }
+      }
+      flush();
+   }
+   __syncthreads();
+   for (int k = threadIdx.x; k < np_lds; k += WG) {
+      if (acc[k][0] != 0.0) atomicAdd(&g_props[2 * k], acc[k][0]);
+      if (acc[k][1] != 0.0) atomicAdd(&g_props[2 * k + 1], acc[k][1]);
+   }
+}
+
 // Packs label[x][y][z] + vector[x][y][z][3] into the march's [x][z][y]
 // storage through a 32 x 32 (y, z) LDS tile, and reduces the label min / max
 // of CheckInput (simulator.cpp:149-167) on the way.
@@ -837,6 +1150,94 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
    return Status();
 }
 
+// z-march trip count of one tilt (simulator.cpp:259-261): the same for every
+// ray, which all start at z = 0 + step_z / 2
+int64_t tilt_steps(const TiltDev &T, int64_t nz) {
+   double z = 0.0;
+   z = z + T.half_step[2];
+   int64_t n = 0;
+   while (std::floor(z) >= 0 && std::floor(z) < (double)nz && n < ((int64_t)1 << 40)) {
+      n++;
+      z = z + T.step[2];
+   }
+   return n;
+}
+
+Status adjoint(const fpl_setup *setup, const int64_t dim[3], const int32_t *label,
+               const float *vector, const double *props, int64_t n_props, const double *tilts,
+               int64_t n_tilts, const double *grad_images, double *grad_props, float *grad_vector,
+               hipStream_t st) {
+   Block B;
+   FPL_TRY(prepare(setup, dim, props, n_props, tilts, n_tilts, B));
+   if (!label || !vector || !grad_images || !grad_props)
+      return make_status(FPL_E_INVALID, "null array");
+   if (B.storage != ST_F32)
+      return make_status(FPL_E_INVALID, "the adjoint reads the f32 field storage");
+   if (grad_vector && setup->interpolate != 0)
+      return make_status(FPL_E_INVALID,
+                         "optical_axis gradients need interpolate='NN' (the axis of a step is "
+                         "then the voxel's own)");
+   const TiltDev *tdev = (const TiltDev *)(B.bytes.data() + B.off_t);
+   int64_t max_steps = 0;
+   for (int64_t t = 0; t < n_tilts; t++) max_steps = std::max(max_steps, tilt_steps(tdev[t], dim[2]));
+   const int64_t ck_stride = std::max<int64_t>(1, (max_steps + CK - 1) / CK);
+   const double lambda = setup->wavelength * 1e-9;
+   const double thickness = setup->voxel_size * 1e-6 * setup->step_size;
+   const double kfac = M_PI_2 * (4.0 * thickness / lambda); // dK / d dn
+   const double mfac = 1e3 * thickness;                       // d(mu t) / d mu
+
+   void *small, *stage, *flags, *scratch;
+   FPL_TRY(ws_get(st, WS_SMALL, B.bytes.size(), &small));
+   FPL_TRY(ws_get(st, WS_FLAGS, 64, &flags));
+   const size_t gp_bytes = sizeof(double) * 2 * (size_t)n_props;
+   FPL_TRY(host_stage(st, std::max(B.bytes.size() + 64, gp_bytes), &stage));
+   const int64_t n_vox = dim[0] * dim[1] * dim[2];
+   const size_t ck_bytes = sizeof(Ckpt) * (size_t)n_tilts * (size_t)(dim[0] * dim[1]) *
+                           (size_t)ck_stride;
+   FPL_TRY(ws_get(st, WS_ADJOINT, ck_bytes + gp_bytes, &scratch));
+   Ckpt *d_ck = (Ckpt *)scratch;
+   double *d_gp = (double *)((char *)scratch + ck_bytes);
+   FPL_HIP(hipStreamSynchronize(st));
+   memcpy(stage, B.bytes.data(), B.bytes.size());
+   FPL_HIP(hipMemcpyAsync(small, stage, B.bytes.size(), hipMemcpyHostToDevice, st));
+   uint32_t *d_flags = (uint32_t *)flags;
+   int32_t *d_mm = (int32_t *)((char *)flags + 16);
+
+   void *packed;
+   FPL_TRY(ws_get(st, WS_PACKED, record_bytes(B.storage) * (size_t)n_vox, &packed));
+   clear_flags(d_flags, d_mm, st);
+   pack(B.storage, dim, label, vector, packed, nullptr, d_mm, st);
+   FPL_HIP(hipMemcpyAsync(stage, flags, 32, hipMemcpyDeviceToHost, st));
+   FPL_HIP(hipStreamSynchronize(st));
+   {
+      const int32_t lmin = ((int32_t *)((char *)stage + 16))[0];
+      const int32_t lmax = ((int32_t *)((char *)stage + 16))[1];
+      FPL_TRY(check_labels(lmin, lmax, n_props, B.storage));
+   }
+   FPL_HIP(hipMemsetAsync(d_gp, 0, gp_bytes, st));
+   if (grad_vector) FPL_HIP(hipMemsetAsync(grad_vector, 0, sizeof(float) * 3 * n_vox, st));
+
+   const int64_t n_tiles = ((dim[0] + RAYS_X - 1) / RAYS_X) * ((dim[1] + RAYS_Y - 1) / RAYS_Y);
+   const dim3 grid((unsigned)(n_tiles * n_tilts));
+   const Params *d_p = (const Params *)small;
+   const TiltDev *d_t = (const TiltDev *)((const char *)small + B.off_t);
+   const PropDev *d_pr = (const PropDev *)((const char *)small + B.off_pr);
+   KernelTimer kt("adjoint", st);
+   if (n_vox < ((int64_t)1 << 32))
+      hipLaunchKernelGGL(k_adjoint<uint32_t>, grid, dim3(WG), 0, st, d_p, d_t, d_pr, packed,
+                         grad_images, d_ck, ck_stride, kfac, mfac, d_gp, grad_vector);
+   else
+      hipLaunchKernelGGL(k_adjoint<int64_t>, grid, dim3(WG), 0, st, d_p, d_t, d_pr, packed,
+                         grad_images, d_ck, ck_stride, kfac, mfac, d_gp, grad_vector);
+   kt.stop();
+   FPL_HIP(hipGetLastError());
+   FPL_HIP(hipMemcpyAsync(stage, d_gp, gp_bytes, hipMemcpyDeviceToHost, st));
+   FPL_HIP(hipStreamSynchronize(st));
+   kt.collect();
+   memcpy(grad_props, stage, gp_bytes);
+   return Status();
+}
+
 } // namespace sim
 } // namespace fpl
 
@@ -847,5 +1248,16 @@ extern "C" int fpl_simulate(const fpl_setup *setup, const int64_t dim[3], const 
    if (warn) *warn = 0;
    fpl::Status s = fpl::sim::run(setup, dim, label, vector, props, n_props, tilts, n_tilts, images,
                                  warn, (hipStream_t)stream);
+   return fpl::finish(s, msg, msg_len);
+}
+
+extern "C" int fpl_simulate_adjoint(const fpl_setup *setup, const int64_t dim[3],
+                                    const int32_t *label, const float *vector,
+                                    const double *props, int64_t n_props, const double *tilts,
+                                    int64_t n_tilts, const double *grad_images,
+                                    double *grad_props, float *grad_vector, void *stream,
+                                    char *msg, size_t msg_len) {
+   fpl::Status s = fpl::sim::adjoint(setup, dim, label, vector, props, n_props, tilts, n_tilts,
+                                     grad_images, grad_props, grad_vector, (hipStream_t)stream);
    return fpl::finish(s, msg, msg_len);
 }

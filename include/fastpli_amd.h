@@ -131,6 +131,23 @@ int fpl_simulate(const fpl_setup *setup, const int64_t dim[3],
                  int64_t n_tilts, double *images, uint32_t *warn, void *stream,
                  char *msg, size_t msg_len);
 
+/* Reverse mode of fpl_simulate (no counterpart in the reference, which has no
+ * derivatives): for L = sum(grad_images * images), with images what
+ * fpl_simulate returns for the same arguments,
+ *   grad_images  DEVICE float64 (n_tilts, dim[0], dim[1], n_rho) = dL/dimages
+ *   grad_props   HOST float64 (n_props, 2) receives (dL/d dn, dL/d mu) per label;
+ *                labels on the reference's attenuation-only branch (dn == 0 or
+ *                label 0, simulator.cpp:280-288) do not depend on dn: 0 there
+ *   grad_vector  DEVICE float (dim..., 3) receives dL/d optical_axis, or NULL;
+ *                needs setup->interpolate == 0 (NN)
+ * The setup's field_storage must be 0 (f32).  The forward march is recomputed
+ * with a checkpoint every 8 steps of each ray; FP32 with FP64 sums per label. */
+int fpl_simulate_adjoint(const fpl_setup *setup, const int64_t dim[3],
+                         const int32_t *label, const float *vector,
+                         const double *props, int64_t n_props, const double *tilts,
+                         int64_t n_tilts, const double *grad_images, double *grad_props,
+                         float *grad_vector, void *stream, char *msg, size_t msg_len);
+
 /* Optical resampling of an image stack, the step after run_simulation in the
  * reference's pipeline: optic.filter_resample applied per rotation image
  * (src/fastpli/simulation/optic.py:169-187, called from
