@@ -81,6 +81,43 @@ def measured_traffic(kernel, storage):
 SERVE_SIZES = [(64, 64, 60), (128, 128, 120), (256, 256, 200), (512, 512, 300)]
 
 
+def adjoint_bench(reps=5):
+    """C4 (one vMF bundle, 300 x 300 x 150 @ 0.2 um), NN, 5 tilts x 9 rotations:
+    the forward march and the adjoint kernel (gradients w.r.t. the (dn, mu) table
+    and the optical-axis field of L = sum(g * images)), HIP-event times."""
+    import torch
+    from fastpli_amd import _lib, synthetic
+    from fastpli_amd.simulation import simulate_stack
+    dim = (300, 300, 150)
+    s = make_simpli(dim, synthetic.single_bundle(seed=0))
+    s.interpolate = 'NN'
+    dev = torch.device('cuda', 0)
+    with warnings.catch_warnings():
+        warnings.simplefilter('ignore')
+        label, vector, props = s.generate_tissue()
+    lab = torch.from_numpy(label).to(dev)
+    axis = torch.from_numpy(vector).to(dev).requires_grad_(True)
+    pr = torch.tensor(props, requires_grad=True)
+    g = None
+    for k in range(reps + 2):
+        if k == 2:
+            _lib.profile_enable(True)
+        img = simulate_stack(s, lab, axis, pr, TILTS)
+        if g is None:
+            g = torch.randn_like(img)
+        (img * g).sum().backward()
+        axis.grad = None
+        pr.grad = None
+    torch.cuda.synchronize()
+    fwd, nf = _lib.profile_read('march')
+    adj, na = _lib.profile_read('adjoint')
+    _lib.profile_enable(False)
+    return {'config': 'one vMF(kappa 10) bundle of 400 fibers, 300x300x150 voxels @0.2um, NN, '
+                      '5 tilts x 9 rotations',
+            'forward_march_ms': fwd / max(nf, 1), 'adjoint_ms': adj / max(na, 1),
+            'gradients': 'dn, mu per label; optical axis per voxel'}
+
+
 def serving_bench(n_requests, load, seed=0):
     """Mixed-size request stream through SimulationServer (p50 / p99 latency).
 
@@ -222,6 +259,8 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--field-storage', default='f32', choices=['f32', 'f16', 'angle16'],
                     help='storage of the fields inside the march (precision modes)')
+    ap.add_argument('--no-adjoint', action='store_true',
+                    help='skip the adjoint timing (C4 forward + reverse march)')
     ap.add_argument('--no-serving', action='store_true',
                     help='skip the mixed-size serving stream (p50 / p99 latency)')
     ap.add_argument('--serve-requests', type=int, default=1500)
@@ -319,6 +358,11 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
+    adjoint = None
+    if not args.no_adjoint:
+        log('adjoint: C4 forward + reverse march')
+        adjoint = adjoint_bench()
+
     serving = None
     if not args.no_serving:
         log(f'serving stream: {args.serve_requests} requests at load {args.serve_load}')
@@ -362,6 +406,7 @@ def main():
         'kernels': kernels,
         'cpu_baseline': cpu,
         'serving': serving,
+        'adjoint': adjoint,
     }
     if cpu:
         out['speedup_vs_cpu_baseline'] = out['value'] / cpu['value']

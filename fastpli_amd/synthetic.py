@@ -86,11 +86,13 @@ def crossing_bundles(seed=0, n_fibers=2000, size_xy=120.0, size_z=60.0, seg=0.5,
 
 
 def single_bundle(seed=0, n_fibers=400, size=(60.0, 60.0, 30.0), kappa=10.0, seg=0.5,
-                  radius_sigma=0.2, layers=MYELIN):
-    """One bundle of straight fibers with vMF-distributed directions."""
+                  radius_sigma=0.2, layers=MYELIN, axis=None):
+    """One bundle of straight fibers with vMF-distributed directions about a
+    seeded random axis (or the given unit `axis`; the random draws are the same)."""
     rng = np.random.default_rng(seed)
-    axis = rng.normal(size=3)
-    axis /= np.linalg.norm(axis)
+    drawn = rng.normal(size=3)
+    axis = drawn if axis is None else np.asarray(axis, dtype=float)
+    axis = axis / np.linalg.norm(axis)
     size = np.asarray(size, dtype=float)
     centre = size / 2.0
     fibers = []
