@@ -19,6 +19,10 @@ Parity pins (see tests/test_oracle_*.py):
   * Lerp, flip_z, step_size != 1, tilted sensor (untilt_sensor_view=False)
     and refraction are pinned only by invariants (parity partially unpinned:
     no reference vectors exist for them).
+
+``oracle.adjoint`` (torch float64) restates the NN march differentiably: the
+FP64 reference gradient of the adjoint kernel, its forward pinned to
+``simulate`` here and its gradient to central differences of itself.
 """
 
 import ctypes
