@@ -326,8 +326,12 @@ __device__ void tilted_start(const TiltDev &T, int64_t i, int64_t j, double &tx,
 }
 
 // ------------------------------------------------------------------ kernel
-constexpr int RAYS_X = 4;  // ray tile of one workgroup: 4 (x) x 64 (y), one wave per x row
-constexpr int RAYS_Y = 64;
+// A workgroup marches WG consecutive rays of the row-major (x, y) pixel grid:
+// the 64 rays of a wave are consecutive y (one contiguous record run per
+// z-step, two at a row end), and no lanes idle when ny is not a multiple of 64.
+__host__ __device__ inline int64_t ray_tiles(int64_t nx, int64_t ny) {
+   return (nx * ny + WG - 1) / WG;
+}
 
 // at most 64 VGPRs (8 waves per SIMD): the march is latency bound, and the
 // wider occupancy hides the record loads of the interpolation better
@@ -346,13 +350,11 @@ __global__ void __launch_bounds__(WG, 8)
    // overlapping columns at the same time, so re-reads hit L2 / MALL
    const int tilt = (int)(blockIdx.x % (unsigned)P.n_tilts);
    const int64_t tile = blockIdx.x / (unsigned)P.n_tilts;
-   const int64_t nty = (P.ny + RAYS_Y - 1) / RAYS_Y;
-   const int64_t i = (tile / nty) * RAYS_X + (threadIdx.x >> 6);
-   const int64_t j = (tile % nty) * RAYS_Y + (threadIdx.x & 63);
-   if (i >= P.nx || j >= P.ny) return;
+   const int64_t ray = tile * WG + threadIdx.x;
+   const int64_t i = ray / P.ny, j = ray - i * P.ny;
+   if (i >= P.nx) return;
    const TiltDev &T = tilts[tilt];
    const int64_t nxy = P.nx * P.ny;
-   const int64_t ray = i * P.ny + j;
    const Field<Idx, ST> F{rec_a, rec_b, (int32_t)P.nx, (int32_t)P.ny, (int32_t)P.nz, P.flip_z};
 
    double px, py, pz = 0.0;
@@ -564,15 +566,13 @@ __global__ void __launch_bounds__(WG)
 
    const int tilt = (int)(blockIdx.x % (unsigned)P.n_tilts);
    const int64_t tile = blockIdx.x / (unsigned)P.n_tilts;
-   const int64_t nty = (P.ny + RAYS_Y - 1) / RAYS_Y;
-   const int64_t i = (tile / nty) * RAYS_X + (threadIdx.x >> 6);
-   const int64_t j = (tile % nty) * RAYS_Y + (threadIdx.x & 63);
+   const int64_t ray = tile * WG + threadIdx.x;
+   const int64_t i = ray / P.ny, j = ray - i * P.ny;
    const int tid = threadIdx.x;
    // no early return: the workgroup meets again to flush its label sums
-   if (i < P.nx && j < P.ny) {
+   if (i < P.nx) {
       const TiltDev &T = tilts[tilt];
       const int64_t nxy = P.nx * P.ny;
-      const int64_t ray = i * P.ny + j;
       const Field<Idx, ST_F32> F{rec_a, nullptr, (int32_t)P.nx, (int32_t)P.ny, (int32_t)P.nz,
                                  P.flip_z};
       Ckpt *my = ck + ((int64_t)tilt * nxy + ray) * ck_stride;
@@ -1002,7 +1002,7 @@ Status prepare(const fpl_setup *setup, const int64_t dim[3], const double *props
       const int64_t nyb = (dim[1] + 31) / 32, nzb = (dim[2] + 31) / 32;
       if (dim[0] * nyb >= ((int64_t)1 << 31) || nzb > 65535)
          return make_status(FPL_E_INVALID, "volume too large");
-      const int64_t n_tiles = ((dim[0] + RAYS_X - 1) / RAYS_X) * ((dim[1] + RAYS_Y - 1) / RAYS_Y);
+      const int64_t n_tiles = ray_tiles(dim[0], dim[1]);
       if (n_tiles * n_tilts >= ((int64_t)1 << 31))
          return make_status(FPL_E_INVALID, "too many rays");
    }
@@ -1080,8 +1080,7 @@ void pack(int storage, const int64_t dim[3], const int32_t *label, const float *
 
 void march(const Block &B, const void *d_block, const void *packed, const uint32_t *packed_b,
            double *images, uint32_t *d_flags, hipStream_t st) {
-   const int64_t n_tiles =
-       ((B.dim[0] + RAYS_X - 1) / RAYS_X) * ((B.dim[1] + RAYS_Y - 1) / RAYS_Y);
+   const int64_t n_tiles = ray_tiles(B.dim[0], B.dim[1]);
    const Params *d_p = (const Params *)d_block;
    const TiltDev *d_t = (const TiltDev *)((const char *)d_block + B.off_t);
    const PropDev *d_pr = (const PropDev *)((const char *)d_block + B.off_pr);
@@ -1217,7 +1216,7 @@ Status adjoint(const fpl_setup *setup, const int64_t dim[3], const int32_t *labe
    FPL_HIP(hipMemsetAsync(d_gp, 0, gp_bytes, st));
    if (grad_vector) FPL_HIP(hipMemsetAsync(grad_vector, 0, sizeof(float) * 3 * n_vox, st));
 
-   const int64_t n_tiles = ((dim[0] + RAYS_X - 1) / RAYS_X) * ((dim[1] + RAYS_Y - 1) / RAYS_Y);
+   const int64_t n_tiles = ray_tiles(dim[0], dim[1]);
    const dim3 grid((unsigned)(n_tiles * n_tilts));
    const Params *d_p = (const Params *)small;
    const TiltDev *d_t = (const TiltDev *)((const char *)small + B.off_t);
