@@ -2,7 +2,8 @@
 //
 // Semantics: fastpli PliSimulator::RunSimulation (src/simulation/simulator.cpp:182-411).
 // One thread per ray (CCD pixel); all tilts of a call run in one launch, the
-// tilts of one ray tile in consecutive workgroups.
+// interpolating tilts of one ray tile in consecutive workgroups, the
+// nearest-neighbour tilts (short work) after them.
 //
 // Formulation (exact in exact arithmetic, derived from simulator.cpp:207-214,
 // 322-343, 355-360):
@@ -33,7 +34,7 @@ namespace fpl {
 namespace sim {
 
 constexpr int WG = 256;
-constexpr int MAX_LDS_PROPS = 1024;
+constexpr int MAX_LDS_PROPS = 128; // property rows held in LDS (more: read from HBM)
 constexpr int MAX_RHO = 64;
 
 struct PropDev {
@@ -53,12 +54,16 @@ struct TiltDev {
    float R[9];           // Mat3RotZYZ(phi, theta, -phi) for the optical axis
    int32_t rotate;       // theta != 0
    int32_t centers;      // every sample point is a voxel centre: no interpolation
+   int32_t out;          // index of the tilt in the caller's order (image slot)
+   int32_t pad_;
 };
 
 struct Params {
    int64_t nx, ny, nz;
    int64_t n_props;
    int32_t n_tilts, n_rho;
+   int32_t n_heavy;      // tilts [0, n_heavy) interpolate; they are launched first
+   int32_t pad_;
    int32_t interp;       // 0 NN, 1 Lerp, 2 Slerp
    int32_t untilt;       // untilt_sensor_view
    int32_t flip_z;
@@ -251,7 +256,9 @@ __device__ __forceinline__ int32_t clampi(int32_t v, int32_t n) {
 // (x - x0 - 0.5) / (x1 - x0) is (x - x0) - 0.5 exactly.  Levels whose two
 // corner indices coincide are skipped: with weight 0 and equal arguments both
 // interpolations return their first argument exactly.
-template <typename Idx, int ST>
+// With OWN false the record of the ray's voxel is not at hand: when all corners
+// coincide it is read like a corner (it carries the label `lab`).
+template <typename Idx, int ST, bool OWN = true>
 __device__ float3 get_vec_interp(const Field<Idx, ST> &F, double px, double py, double pz,
                                  int32_t lab, const float3 &own, int slerp) {
    const double fx = px - 0.5, fy = py - 0.5, fz = pz - 0.5;
@@ -260,7 +267,7 @@ __device__ float3 get_vec_interp(const Field<Idx, ST> &F, double px, double py, 
    const int32_t x0 = clampi((int32_t)flx, F.nx), x1 = clampi((int32_t)flx + (fx != flx), F.nx);
    const int32_t y0 = clampi((int32_t)fly, F.ny), y1 = clampi((int32_t)fly + (fy != fly), F.ny);
    const int32_t z0 = clampi((int32_t)flz, F.nz), z1 = clampi((int32_t)flz + (fz != flz), F.nz);
-   if (x0 == x1 && y0 == y1 && z0 == z1) return own;
+   if (OWN && x0 == x1 && y0 == y1 && z0 == z1) return own;
    const bool dx = x0 != x1, dy = y0 != y1, dz = z0 != z1;
    const float xd = dx ? (float)((px - (double)x0) - 0.5) : 0.0f;
    const float yd = dy ? (float)((py - (double)y0) - 0.5) : 0.0f;
@@ -333,35 +340,140 @@ __host__ __device__ inline int64_t ray_tiles(int64_t nx, int64_t ny) {
    return (nx * ny + WG - 1) / WG;
 }
 
-// at most 64 VGPRs (8 waves per SIMD): the march is latency bound, and the
-// wider occupancy hides the record loads of the interpolation better
+// Wave-cooperative march of interpolating views.  A quarter of the voxels of
+// a fiber tissue are birefringent, scattered over the 64 rays of a wave; one
+// thread per ray would run the costly part of a step (corner gather +
+// interpolation, axis rotation, retarder angles) for the whole wave whenever
+// one ray needs it.
+// Instead every ray walks its voxels one step per iteration and queues each
+// birefringent step in a per-wave LDS ring (ballot + mbcnt, no atomics);
+// attenuation-only steps are summed at once (they commute with the
+// retarders).  When 64 steps are queued (or no ray can go on) the wave
+// evaluates them, one per lane, and writes every step's retarder (cos 2phi,
+// sin 2phi, sin delta, 1 - cos delta, mu t) into its entry; each ray then
+// applies its own evaluated steps to its Poincare state in its own order.  A
+// ray holds at most QDEPTH unapplied steps: at the next birefringent voxel it
+// waits (attenuation-only voxels it still passes).
+constexpr int QCAP = 128;  // ring entries per wave: < 64 waiting + 64 new
+constexpr int QDEPTH = 4;  // queued steps per ray (8-bit sequence numbers in one word)
+
+__device__ __forceinline__ void wave_sync() {
+   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+   __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ double pack_double(uint32_t lo, uint32_t hi) {
+   return __hiloint2double((int)hi, (int)lo);
+}
+
+
+// The retarder of one birefringent step (simulator.cpp:292-320) in the
+// rho-invariant form: (cos 2phi, sin 2phi) of the tilted axis, sin delta and
+// 1 - cos delta.  False for a zero axis (skipped without attenuation,
+// simulator.cpp:292-297).  `nn`: the precision storages' "not normalized" bit.
+template <int ST>
+__device__ __forceinline__ bool retarder(const TiltDev &T, float K, float3 v, bool nn,
+                                         uint32_t &warn, float &C, float &S, float &sd,
+                                         float &omc) {
+   C = 1.0f;
+   S = 0.0f;
+   sd = 0.0f;
+   omc = 0.0f;
+   if (is_zero(v)) {
+      warn |= FPL_WARN_ZERO_VEC;
+      return false;
+   }
+   const float n2 = dot3(v, v);
+   if (ST == ST_F32 ? (n2 - 1.0f > 1e-5f) : nn) warn |= FPL_WARN_NOT_NORM;
+   const float dr = K * sqrtf(n2);
+   if (T.rotate) {
+      const float3 w = v;
+      v.x = T.R[0] * w.x + T.R[1] * w.y + T.R[2] * w.z;
+      v.y = T.R[3] * w.x + T.R[4] * w.y + T.R[5] * w.z;
+      v.z = T.R[6] * w.x + T.R[7] * w.y + T.R[8] * w.z;
+   }
+   const float xx = v.x * v.x, yy = v.y * v.y;
+   const float rxy = xx + yy;
+   const float rr = rxy + v.z * v.z;
+   const float delta = dr * (rxy / rr);
+   if (rxy > 0.0f) {
+      const float inv = 1.0f / rxy;
+      C = (xx - yy) * inv;
+      S = 2.0f * v.x * v.y * inv;
+   }
+   sin_omc(delta, sd, omc);
+   return true;
+}
+
+// u <- rotation of u by -delta about n = (cos 2phi, -sin 2phi, 0) (Rodrigues),
+// the increment added in float-float
+__device__ __forceinline__ void apply_retarder(float C, float S, float sd, float omc, float &h1,
+                                               float &h2, float &h3, float &l1, float &l2,
+                                               float &l3) {
+   const float nv = C * h1 - S * h2;
+   const float d1 = sd * S * h3 + omc * (C * nv - h1);
+   const float d2 = sd * C * h3 + omc * (-S * nv - h2);
+   const float d3 = -sd * (C * h2 + S * h1) - omc * h3;
+   two_sum(h1, l1, d1);
+   two_sum(h2, l2, d2);
+   two_sum(h3, l3, d3);
+}
+
+// at most 80 VGPRs (6 waves per SIMD): the queue evaluation's interpolation
+// needs them, and spilling costs more than the lost occupancy
 template <typename Idx, int ST>
-__global__ void __launch_bounds__(WG, 8)
+__global__ void __launch_bounds__(WG, 6)
     k_march(const Params *__restrict__ prm, const TiltDev *__restrict__ tilts,
             const PropDev *__restrict__ props, const void *__restrict__ rec_a,
             const uint32_t *__restrict__ rec_b, double *__restrict__ images, uint32_t *flags) {
    __shared__ PropDev sp[MAX_LDS_PROPS];
+   // queue entry words 0-3 / 4-7.  Queued: px, py, pz as doubles, label,
+   // "not normalized" flag of the ray's voxel.  Evaluated: cos 2phi, sin 2phi,
+   // sin delta, 1 - cos delta, mu t.
+   __shared__ uint4 qa[WG / 64][QCAP];
+   __shared__ uint4 qb[WG / 64][QCAP];
    const Params &P = *prm;
    const int np_lds = P.n_props < MAX_LDS_PROPS ? (int)P.n_props : MAX_LDS_PROPS;
    for (int k = threadIdx.x; k < np_lds; k += WG) sp[k] = props[k];
    __syncthreads();
 
-   // all tilts of one ray tile are consecutive workgroups: they march through
-   // overlapping columns at the same time, so re-reads hit L2 / MALL
-   const int tilt = (int)(blockIdx.x % (unsigned)P.n_tilts);
-   const int64_t tile = blockIdx.x / (unsigned)P.n_tilts;
+   // the interpolating tilts of one ray tile are consecutive workgroups: they
+   // march through overlapping columns at the same time, so re-reads hit L2 /
+   // MALL.  Tilts without interpolation (a third of the work per ray) come
+   // last and fill the end of the launch, when the long rays are finishing.
+   int tilt;
+   int64_t tile;
+   {
+      const unsigned nh = (unsigned)P.n_heavy, nt = (unsigned)P.n_tilts;
+      const unsigned split = nh * (unsigned)ray_tiles(P.nx, P.ny);
+      const unsigned b = blockIdx.x;
+      if (b < split) {
+         tilt = (int)(b % nh);
+         tile = b / nh;
+      } else {
+         const unsigned c = b - split;
+         tilt = (int)(nh + c % (nt - nh));
+         tile = c / (nt - nh);
+      }
+   }
+   const int lane = (int)(threadIdx.x & 63u);
+   uint4 *const QA = qa[threadIdx.x >> 6];
+   uint4 *const QB = qb[threadIdx.x >> 6];
    const int64_t ray = tile * WG + threadIdx.x;
    const int64_t i = ray / P.ny, j = ray - i * P.ny;
-   if (i >= P.nx) return;
+   // rays past the grid's end stay in the loop (their lanes evaluate queued steps)
+   const bool valid = i < P.nx;
    const TiltDev &T = tilts[tilt];
    const int64_t nxy = P.nx * P.ny;
    const Field<Idx, ST> F{rec_a, rec_b, (int32_t)P.nx, (int32_t)P.ny, (int32_t)P.nz, P.flip_z};
+   const bool interp = !(P.interp == 0 || T.centers);
+   const int slerp = P.interp == 2;
 
-   double px, py, pz = 0.0;
+   double px = 0.0, py = 0.0, pz = 0.0;
    if (P.untilt) {
       px = ((double)i + 0.5) - T.half_shift[0];
       py = ((double)j + 0.5) - T.half_shift[1];
-   } else {
+   } else if (valid) {
       tilted_start(T, i, j, px, py);
    }
    px = px + T.half_step[0];
@@ -374,81 +486,141 @@ __global__ void __launch_bounds__(WG, 8)
    float h1 = 0.0f, h2 = 0.0f, h3 = -1.0f, l1 = 0.0f, l2 = 0.0f, l3 = 0.0f;
    float mh = 0.0f, ml = 0.0f; // sum of mu * t
    uint32_t warn = 0, err = 0;
+   if (!interp) {
+      // voxel axes used as they are (nearest neighbour, or every sample point
+      // a voxel centre): a step is cheap, so every ray runs its own steps and
+      // the record of step k + 1 is requested before step k is processed
+      double fz = floor(pz);
+      bool live = valid && fz >= 0.0 && fz < nzd;
+      int32_t ix = ifloor(px), iy = ifloor(py), iz = ifloor(fz);
+      Vox r{0, make_float3(0.0f, 0.0f, 0.0f), false}; // outside: background (simulator.cpp:430-432)
+      if (live && F.inside(ix, iy, iz)) r = F.at(ix, iy, iz);
+      while (live) {
+         px = px + sx;
+         py = py + sy;
+         pz = pz + sz;
+         fz = floor(pz);
+         const bool qlive = fz >= 0.0 && fz < nzd;
+         const int32_t jx = ifloor(px), jy = ifloor(py), jz = ifloor(fz);
+         Vox rn{0, make_float3(0.0f, 0.0f, 0.0f), false};
+         if (qlive && F.inside(jx, jy, jz)) rn = F.at(jx, jy, jz);
 
-   // software pipelined: the record of step k+1 is requested before step k is
-   // processed; the loop condition is the reference's (simulator.cpp:259-261)
-   double fz = floor(pz);
-   bool live = fz >= 0.0 && fz < nzd;
-   int32_t ix = ifloor(px), iy = ifloor(py), iz = ifloor(fz);
-   Vox r{0, make_float3(0.0f, 0.0f, 0.0f), false}; // outside: background (simulator.cpp:430-432)
-   if (live && F.inside(ix, iy, iz)) r = F.at(ix, iy, iz);
-   while (live) {
-      const double qx = px + sx, qy = py + sy, qz = pz + sz;
-      const double qfz = floor(qz);
-      const bool qlive = qfz >= 0.0 && qfz < nzd;
-      const int32_t jx = ifloor(qx), jy = ifloor(qy), jz = ifloor(qfz);
-      Vox rn{0, make_float3(0.0f, 0.0f, 0.0f), false};
-      if (qlive && F.inside(jx, jy, jz)) rn = F.at(jx, jy, jz);
-
-      int32_t lab = r.lab;
-      if (lab < 0 || lab >= P.n_props) {
-         err = 1;
-         lab = 0;
-      }
-      const PropDev pd = lab < MAX_LDS_PROPS ? sp[lab] : props[lab];
-      if (pd.bg) {
-         if (pd.mu_t != 0.0f) two_sum(mh, ml, pd.mu_t);
-      } else {
-         float3 v = (P.interp == 0 || T.centers)
-                        ? r.v
-                        : get_vec_interp(F, px, py, pz, lab, r.v, P.interp == 2);
-         if (is_zero(v)) {
-            warn |= FPL_WARN_ZERO_VEC; // skipped without attenuation (simulator.cpp:292-297)
-         } else {
-            const float n2 = dot3(v, v);
-            if (ST == ST_F32 ? (n2 - 1.0f > 1e-5f) : r.nn) warn |= FPL_WARN_NOT_NORM;
-            const float dr = pd.K * sqrtf(n2);
-            if (T.rotate) {
-               const float3 w = v;
-               v.x = T.R[0] * w.x + T.R[1] * w.y + T.R[2] * w.z;
-               v.y = T.R[3] * w.x + T.R[4] * w.y + T.R[5] * w.z;
-               v.z = T.R[6] * w.x + T.R[7] * w.y + T.R[8] * w.z;
-            }
-            const float xx = v.x * v.x, yy = v.y * v.y;
-            const float rxy = xx + yy;
-            const float rr = rxy + v.z * v.z;
-            const float delta = dr * (rxy / rr);
-            float C = 1.0f, S = 0.0f;
-            if (rxy > 0.0f) {
-               const float inv = 1.0f / rxy;
-               C = (xx - yy) * inv;
-               S = 2.0f * v.x * v.y * inv;
-            }
-            float sd, omc; // sin(delta), 1 - cos(delta)
-            sin_omc(delta, sd, omc);
-            const float nv = C * h1 - S * h2;
-            const float d1 = sd * S * h3 + omc * (C * nv - h1);
-            const float d2 = sd * C * h3 + omc * (-S * nv - h2);
-            const float d3 = -sd * (C * h2 + S * h1) - omc * h3;
-            two_sum(h1, l1, d1);
-            two_sum(h2, l2, d2);
-            two_sum(h3, l3, d3);
+         int32_t lab = r.lab;
+         if (lab < 0 || lab >= P.n_props) {
+            err = 1;
+            lab = 0;
+         }
+         const PropDev pd = lab < MAX_LDS_PROPS ? sp[lab] : props[lab];
+         if (pd.bg) {
             if (pd.mu_t != 0.0f) two_sum(mh, ml, pd.mu_t);
+         } else {
+            float C, S, sd, omc;
+            if (retarder<ST>(T, pd.K, r.v, r.nn, warn, C, S, sd, omc)) {
+               apply_retarder(C, S, sd, omc, h1, h2, h3, l1, l2, l3);
+               if (pd.mu_t != 0.0f) two_sum(mh, ml, pd.mu_t);
+            }
+         }
+         live = qlive;
+         r = rn;
+      }
+   } else {
+      // interpolating views: the wave-cooperative queue described above
+      uint32_t pend = 0; // sequence numbers (mod 256) of the ray's queued steps, oldest lowest
+      int npend = 0;
+      uint32_t tail = 0, head = 0; // wave: steps queued / evaluated so far
+
+      // the record of the current step is loaded one iteration ahead; the loop
+      // condition is the reference's (simulator.cpp:259-261)
+      const double fz0 = floor(pz);
+      bool live = valid && fz0 >= 0.0 && fz0 < nzd;
+      Vox r{0, make_float3(0.0f, 0.0f, 0.0f), false}; // outside: background (simulator.cpp:430-432)
+      {
+         const int32_t ix = ifloor(px), iy = ifloor(py), iz = ifloor(fz0);
+         if (live && F.inside(ix, iy, iz)) r = F.at(ix, iy, iz);
+      }
+      for (;;) {
+         int32_t lab = r.lab;
+         if (lab < 0 || lab >= P.n_props) {
+            err = 1;
+            lab = 0;
+         }
+         const PropDev pd = lab < MAX_LDS_PROPS ? sp[lab] : props[lab];
+         const bool bire = live && !pd.bg;
+         const bool go = live && (!bire || npend < QDEPTH);
+         const uint64_t gomask = __ballot(go);
+         if (gomask == 0 && tail == head) break; // every ray ended, every step applied
+         if (gomask != 0) {
+            const uint64_t qmask = __ballot(go && bire);
+            if (go) {
+               if (bire) {
+                  const uint32_t seq =
+                      tail + __builtin_amdgcn_mbcnt_hi((uint32_t)(qmask >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)qmask, 0u));
+                  const uint32_t slot = seq & (QCAP - 1);
+                  QA[slot] = make_uint4(__double2loint(px), __double2hiint(px), __double2loint(py),
+                                        __double2hiint(py));
+                  QB[slot] = make_uint4(__double2loint(pz), __double2hiint(pz), (uint32_t)lab,
+                                        r.nn ? 1u : 0u);
+                  pend |= (seq & 255u) << (8 * npend);
+                  npend++;
+               } else if (pd.mu_t != 0.0f) {
+                  two_sum(mh, ml, pd.mu_t);
+               }
+               px = px + sx;
+               py = py + sy;
+               pz = pz + sz;
+               const double fz = floor(pz);
+               live = fz >= 0.0 && fz < nzd;
+               const int32_t jx = ifloor(px), jy = ifloor(py), jz = ifloor(fz);
+               Vox rn{0, make_float3(0.0f, 0.0f, 0.0f), false};
+               if (live && F.inside(jx, jy, jz)) rn = F.at(jx, jy, jz);
+               r = rn;
+            }
+            tail += (uint32_t)__popcll(qmask);
+         }
+         const uint32_t queued = tail - head;
+         if (queued >= 64 || gomask == 0) {
+            const uint32_t n = queued < 64 ? queued : 64;
+            wave_sync();
+            if ((uint32_t)lane < n) {
+               const uint32_t slot = (head + (uint32_t)lane) & (QCAP - 1);
+               const uint4 a = QA[slot], b = QB[slot];
+               const int32_t elab = (int32_t)b.z;
+               const PropDev epd = elab < MAX_LDS_PROPS ? sp[elab] : props[elab];
+               const float3 v = get_vec_interp<Idx, ST, false>(
+                   F, pack_double(a.x, a.y), pack_double(a.z, a.w), pack_double(b.x, b.y), elab,
+                   make_float3(0.0f, 0.0f, 0.0f), slerp);
+               float C, S, sd, omc, mu = 0.0f;
+               if (retarder<ST>(T, epd.K, v, b.w != 0, warn, C, S, sd, omc)) mu = epd.mu_t;
+               QA[slot] = make_uint4(__float_as_uint(C), __float_as_uint(S), __float_as_uint(sd),
+                                     __float_as_uint(omc));
+               QB[slot].x = __float_as_uint(mu);
+            }
+            wave_sync();
+            // every ray applies its steps of this batch, oldest first
+            while (npend > 0 && (((pend & 255u) - head) & 255u) < n) {
+               const uint32_t slot = pend & (QCAP - 1);
+               const uint4 e = QA[slot];
+               const float mu = __uint_as_float(QB[slot].x);
+               const float C = __uint_as_float(e.x), S = __uint_as_float(e.y);
+               const float sd = __uint_as_float(e.z), omc = __uint_as_float(e.w);
+               apply_retarder(C, S, sd, omc, h1, h2, h3, l1, l2, l3);
+               if (mu != 0.0f) two_sum(mh, ml, mu);
+               pend >>= 8;
+               npend--;
+            }
+            head += n;
          }
       }
-      px = qx;
-      py = qy;
-      pz = qz;
-      live = qlive;
-      r = rn;
    }
 
-   const double A = exp(-((double)mh + (double)ml));
-   const double u1 = (double)h1 + (double)l1, u2 = (double)h2 + (double)l2;
-   const double base = P.quarter_I0 * A;
-   double *out = images + ((int64_t)tilt * nxy + ray) * P.n_rho;
-   for (int k = 0; k < P.n_rho; k++) out[k] = base * (1.0 - (P.c2r[k] * u1 - P.s2r[k] * u2));
-
+   if (valid) {
+      const double A = exp(-((double)mh + (double)ml));
+      const double u1 = (double)h1 + (double)l1, u2 = (double)h2 + (double)l2;
+      const double base = P.quarter_I0 * A;
+      double *out = images + ((int64_t)T.out * nxy + ray) * P.n_rho;
+      for (int k = 0; k < P.n_rho; k++) out[k] = base * (1.0 - (P.c2r[k] * u1 - P.s2r[k] * u2));
+   }
    const uint32_t f = warn | (err ? 0x80000000u : 0u);
    if (f) atomicOr(&flags[0], f);
 }
@@ -460,7 +632,7 @@ __global__ void __launch_bounds__(WG, 8)
 // forward it differentiates is the march above (the reference's per-rho
 // chain in the rho-invariant form).
 //
-// Per ray (one thread, tilt-interleaved workgroups as in k_march):
+// Per ray (one thread, tilt-interleaved workgroups):
 //   forward   the march, keeping a checkpoint (FP64 position, Poincare state
 //             u) before every CK-th step;
 //   seeds     I_r = b (1 - c_r u1 + s_r u2), b = I0/4 exp(-M):
@@ -632,7 +804,7 @@ __global__ void __launch_bounds__(WG)
       const double A = exp(-((double)mh + (double)ml));
       const double u1 = (double)h1 + (double)l1, u2 = (double)h2 + (double)l2;
       const double base = P.quarter_I0 * A;
-      const double *g = grad_images + ((int64_t)tilt * nxy + ray) * P.n_rho;
+      const double *g = grad_images + ((int64_t)T.out * nxy + ray) * P.n_rho;
       double gc = 0.0, gs = 0.0, mbar = 0.0;
       for (int k = 0; k < P.n_rho; k++) {
          const double I = base * (1.0 - (P.c2r[k] * u1 - P.s2r[k] * u2));
@@ -1044,6 +1216,15 @@ Status prepare(const fpl_setup *setup, const int64_t dim[3], const double *props
          return make_status(FPL_E_INVALID, "tilt theta = %f gives no progress along z",
                             tilts[2 * t]);
    }
+   // tilts that interpolate (long rays of work) first, so that the short
+   // nearest-neighbour workgroups fill the end of the launch
+   for (int64_t t = 0; t < n_tilts; t++) tdev[t].out = (int32_t)t;
+   std::stable_partition(tdev.begin(), tdev.end(), [&](const TiltDev &T) {
+      return prm.interp != 0 && !T.centers;
+   });
+   prm.n_heavy = 0;
+   for (int64_t t = 0; t < n_tilts; t++) prm.n_heavy += (prm.interp != 0 && !tdev[t].centers);
+   if (prm.n_heavy == 0 || prm.n_heavy == n_tilts) prm.n_heavy = (int32_t)n_tilts;
    B.off_t = (sizeof(Params) + 255) & ~size_t(255);
    B.off_pr = (B.off_t + sizeof(TiltDev) * n_tilts + 255) & ~size_t(255);
    B.bytes.assign(B.off_pr + sizeof(PropDev) * n_props, 0);
