@@ -215,3 +215,33 @@ def test_simulate_device_tensors(golden):
     assert imgs.is_cuda and imgs.shape == (5, 24, 24, 6)
     for t in range(5):
         assert_intensity_close(imgs[t].cpu().numpy(), golden['data'][t], 26000)
+
+
+@pytest.mark.parametrize('interp', ['Slerp', 'Lerp'])
+def test_simulate_dense_birefringence_queue(interp):
+    """Interpolating views of a volume that is birefringent almost everywhere:
+    every ray reaches its limit of queued steps, so the wave evaluates partial
+    batches; rays of unequal length (tilted sensor) and a ray grid that is not a
+    multiple of the wave width (23 x 29 pixels) end at different times."""
+    _say(f'simulate: dense birefringent volume, {interp}, queued steps')
+    rng = np.random.default_rng(5)
+    dim = (23, 29, 40)
+    tissue = np.where(rng.random(dim) < 0.93, 1 + (rng.random(dim) < 0.3), 0).astype(np.int32)
+    # a smoothly varying field with noise, plus a few zero and non-unit axes
+    x, y, z = np.meshgrid(*[np.arange(n) for n in dim], indexing='ij')
+    a = 0.15 * x + 0.1 * z + 0.3 * rng.standard_normal(dim)
+    b = 0.4 + 0.02 * y + 0.2 * rng.standard_normal(dim)
+    axis = np.stack([np.cos(a) * np.sin(b), np.sin(a) * np.sin(b), np.cos(b)], -1)
+    axis = axis.astype(np.float32)
+    axis[rng.random(dim) < 0.01] = 0
+    axis[tissue == 0] = 0
+    props = np.array([[0.0, 0.5], [-0.004, 1.0], [0.006, 3.0]])
+    for opts in (dict(), dict(untilt_sensor_view=False, tissue_refrection=1.2), dict(flip_z_beam=True)):
+        s = _setup(dim, 0.5, interpolate=interp, rho=np.linspace(0, np.pi, 9, endpoint=False),
+                   **opts)
+        tilts = np.deg2rad([(0, 0), (5.5, 0), (8.0, 90), (12.0, 200), (30.0, 45)])
+        with pytest.warns(UserWarning, match='optical_axis is 0'):
+            imgs = s.run_simulation_tilts(tissue, axis, props, tilts)
+        for t, (theta, phi) in enumerate(tilts):
+            ref, _ = _oracle(s, tissue, axis, props, theta, phi)
+            assert_intensity_close(imgs[t], ref, 26000)
