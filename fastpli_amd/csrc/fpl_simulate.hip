@@ -34,7 +34,7 @@ namespace fpl {
 namespace sim {
 
 constexpr int WG = 256;
-constexpr int MAX_LDS_PROPS = 128; // property rows held in LDS (more: read from HBM)
+constexpr int MAX_LDS_PROPS = 128; // property tables up to this size are held in LDS
 constexpr int MAX_RHO = 64;
 
 struct PropDev {
@@ -371,8 +371,14 @@ __device__ __forceinline__ double pack_double(uint32_t lo, uint32_t hi) {
 // rho-invariant form: (cos 2phi, sin 2phi) of the tilted axis, sin delta and
 // 1 - cos delta.  False for a zero axis (skipped without attenuation,
 // simulator.cpp:292-297).  `nn`: the precision storages' "not normalized" bit.
+// The tilt's rotation of the optical axis, kept in registers
+struct AxisRot {
+   float R[9];
+   bool on; // theta != 0
+};
+
 template <int ST>
-__device__ __forceinline__ bool retarder(const TiltDev &T, float K, float3 v, bool nn,
+__device__ __forceinline__ bool retarder(const AxisRot &T, float K, float3 v, bool nn,
                                          uint32_t &warn, float &C, float &S, float &sd,
                                          float &omc) {
    C = 1.0f;
@@ -386,7 +392,7 @@ __device__ __forceinline__ bool retarder(const TiltDev &T, float K, float3 v, bo
    const float n2 = dot3(v, v);
    if (ST == ST_F32 ? (n2 - 1.0f > 1e-5f) : nn) warn |= FPL_WARN_NOT_NORM;
    const float dr = K * sqrtf(n2);
-   if (T.rotate) {
+   if (T.on) {
       const float3 w = v;
       v.x = T.R[0] * w.x + T.R[1] * w.y + T.R[2] * w.z;
       v.y = T.R[3] * w.x + T.R[4] * w.y + T.R[5] * w.z;
@@ -421,7 +427,10 @@ __device__ __forceinline__ void apply_retarder(float C, float S, float sd, float
 
 // at most 80 VGPRs (6 waves per SIMD): the queue evaluation's interpolation
 // needs them, and spilling costs more than the lost occupancy
-template <typename Idx, int ST>
+// PROPS_LDS: the whole property table fits the LDS copy (else it is read from
+// HBM through the caches); a compile-time choice keeps every lookup a plain
+// LDS read.
+template <typename Idx, int ST, bool PROPS_LDS>
 __global__ void __launch_bounds__(WG, 6)
     k_march(const Params *__restrict__ prm, const TiltDev *__restrict__ tilts,
             const PropDev *__restrict__ props, const void *__restrict__ rec_a,
@@ -433,9 +442,12 @@ __global__ void __launch_bounds__(WG, 6)
    __shared__ uint4 qa[WG / 64][QCAP];
    __shared__ uint4 qb[WG / 64][QCAP];
    const Params &P = *prm;
-   const int np_lds = P.n_props < MAX_LDS_PROPS ? (int)P.n_props : MAX_LDS_PROPS;
-   for (int k = threadIdx.x; k < np_lds; k += WG) sp[k] = props[k];
-   __syncthreads();
+   const int64_t n_props = P.n_props;
+   if (PROPS_LDS) {
+      for (int k = threadIdx.x; k < (int)n_props; k += WG) sp[k] = props[k];
+      __syncthreads();
+   }
+   auto prop = [&](int32_t l) -> PropDev { return PROPS_LDS ? sp[l] : props[l]; };
 
    // the interpolating tilts of one ray tile are consecutive workgroups: they
    // march through overlapping columns at the same time, so re-reads hit L2 /
@@ -468,6 +480,9 @@ __global__ void __launch_bounds__(WG, 6)
    const Field<Idx, ST> F{rec_a, rec_b, (int32_t)P.nx, (int32_t)P.ny, (int32_t)P.nz, P.flip_z};
    const bool interp = !(P.interp == 0 || T.centers);
    const int slerp = P.interp == 2;
+   AxisRot rot;
+   for (int k = 0; k < 9; k++) rot.R[k] = T.R[k];
+   rot.on = T.rotate != 0;
 
    double px = 0.0, py = 0.0, pz = 0.0;
    if (P.untilt) {
@@ -506,16 +521,16 @@ __global__ void __launch_bounds__(WG, 6)
          if (qlive && F.inside(jx, jy, jz)) rn = F.at(jx, jy, jz);
 
          int32_t lab = r.lab;
-         if (lab < 0 || lab >= P.n_props) {
+         if (lab < 0 || lab >= n_props) {
             err = 1;
             lab = 0;
          }
-         const PropDev pd = lab < MAX_LDS_PROPS ? sp[lab] : props[lab];
+         const PropDev pd = prop(lab);
          if (pd.bg) {
             if (pd.mu_t != 0.0f) two_sum(mh, ml, pd.mu_t);
          } else {
             float C, S, sd, omc;
-            if (retarder<ST>(T, pd.K, r.v, r.nn, warn, C, S, sd, omc)) {
+            if (retarder<ST>(rot, pd.K, r.v, r.nn, warn, C, S, sd, omc)) {
                apply_retarder(C, S, sd, omc, h1, h2, h3, l1, l2, l3);
                if (pd.mu_t != 0.0f) two_sum(mh, ml, pd.mu_t);
             }
@@ -540,11 +555,11 @@ __global__ void __launch_bounds__(WG, 6)
       }
       for (;;) {
          int32_t lab = r.lab;
-         if (lab < 0 || lab >= P.n_props) {
+         if (lab < 0 || lab >= n_props) {
             err = 1;
             lab = 0;
          }
-         const PropDev pd = lab < MAX_LDS_PROPS ? sp[lab] : props[lab];
+         const PropDev pd = prop(lab);
          const bool bire = live && !pd.bg;
          const bool go = live && (!bire || npend < QDEPTH);
          const uint64_t gomask = __ballot(go);
@@ -586,12 +601,12 @@ __global__ void __launch_bounds__(WG, 6)
                const uint32_t slot = (head + (uint32_t)lane) & (QCAP - 1);
                const uint4 a = QA[slot], b = QB[slot];
                const int32_t elab = (int32_t)b.z;
-               const PropDev epd = elab < MAX_LDS_PROPS ? sp[elab] : props[elab];
+               const PropDev epd = prop(elab);
                const float3 v = get_vec_interp<Idx, ST, false>(
                    F, pack_double(a.x, a.y), pack_double(a.z, a.w), pack_double(b.x, b.y), elab,
                    make_float3(0.0f, 0.0f, 0.0f), slerp);
                float C, S, sd, omc, mu = 0.0f;
-               if (retarder<ST>(T, epd.K, v, b.w != 0, warn, C, S, sd, omc)) mu = epd.mu_t;
+               if (retarder<ST>(rot, epd.K, v, b.w != 0, warn, C, S, sd, omc)) mu = epd.mu_t;
                QA[slot] = make_uint4(__float_as_uint(C), __float_as_uint(S), __float_as_uint(sd),
                                      __float_as_uint(omc));
                QB[slot].x = __float_as_uint(mu);
@@ -1267,9 +1282,16 @@ void march(const Block &B, const void *d_block, const void *packed, const uint32
    const PropDev *d_pr = (const PropDev *)((const char *)d_block + B.off_pr);
    const dim3 mgrid((unsigned)(n_tiles * B.n_tilts));
    const bool small_idx = B.dim[0] * B.dim[1] * B.dim[2] < ((int64_t)1 << 32);
+   const bool props_lds = B.n_props <= MAX_LDS_PROPS;
 #define FPL_MARCH(IDX, STV)                                                                       \
-   hipLaunchKernelGGL((k_march<IDX, STV>), mgrid, dim3(WG), 0, st, d_p, d_t, d_pr, packed,       \
-                      packed_b, images, d_flags)
+   do {                                                                                           \
+      if (props_lds)                                                                              \
+         hipLaunchKernelGGL((k_march<IDX, STV, true>), mgrid, dim3(WG), 0, st, d_p, d_t, d_pr,    \
+                            packed, packed_b, images, d_flags);                                   \
+      else                                                                                        \
+         hipLaunchKernelGGL((k_march<IDX, STV, false>), mgrid, dim3(WG), 0, st, d_p, d_t, d_pr,   \
+                            packed, packed_b, images, d_flags);                                   \
+   } while (0)
    if (B.storage == ST_F32) {
       if (small_idx) FPL_MARCH(uint32_t, ST_F32); else FPL_MARCH(int64_t, ST_F32);
    } else if (B.storage == ST_F16) {
