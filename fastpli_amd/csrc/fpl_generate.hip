@@ -638,7 +638,9 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
 // stored distance, with margins that keep every candidate the exact test
 // could accept; survivors enter the wave's queue in order.  Stage B
 // (run_batch) runs the exact FP64 test on 64 queued candidates at a time.
-template <typename SegT>
+// LSQ_IN_LDS: the layer tables fit the LDS copy; a compile-time choice keeps
+// the layer searches plain LDS reads
+template <typename SegT, bool LSQ_IN_LDS>
 __global__ void __launch_bounds__(WG)
     k_voxelize(Params pr, const SegRec *__restrict__ segs, const float4 *__restrict__ seg_dir,
                const uint32_t *__restrict__ list,
@@ -667,14 +669,13 @@ __global__ void __launch_bounds__(WG)
    // the layer tables in LDS when small (read through a generic pointer)
    __shared__ double lsq_lds[LSQ_LDS];
    __shared__ uint8_t lmodel_lds[LSQ_LDS];
-   const bool lsq_in_lds = pr.n_lsq <= LSQ_LDS;
-   if (lsq_in_lds)
+   if (LSQ_IN_LDS)
       for (int i = threadIdx.x; i < pr.n_lsq; i += WG) {
          lsq_lds[i] = layer_sq[i];
          lmodel_lds[i] = layer_model[i];
       }
-   const double *lsq = lsq_in_lds ? (const double *)lsq_lds : layer_sq;
-   const uint8_t *lmodel = lsq_in_lds ? (const uint8_t *)lmodel_lds : layer_model;
+   const double *lsq = LSQ_IN_LDS ? (const double *)lsq_lds : layer_sq;
+   const uint8_t *lmodel = LSQ_IN_LDS ? (const uint8_t *)lmodel_lds : layer_model;
 
    const uint32_t beg = range[2 * t], end = range[2 * t + 1];
    constexpr bool short_seg = sizeof(SegT) == 2;
@@ -1136,10 +1137,18 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
       hipLaunchKernelGGL(k_ranges, dim3(grid_for(total, B)), dim3(B), 0, st, (uint32_t *)kb, total,
                          (uint32_t *)rng);
    }
+   const bool lsq_lds = pr.n_lsq <= LSQ_LDS;
    KernelTimer kt("voxelize", st);
-   hipLaunchKernelGGL(k_voxelize<uint16_t>, dim3((unsigned)n_tiles), dim3(WG), 0, st, pr,
-                      (const SegRec *)segs, seg_dir, (const uint32_t *)vb, (const uint32_t *)rng, d_bd,
-                      d_lsq, d_lm, only_label, label, vector, d_flags);
+   if (lsq_lds)
+      hipLaunchKernelGGL((k_voxelize<uint16_t, true>), dim3((unsigned)n_tiles), dim3(WG), 0, st,
+                         pr, (const SegRec *)segs, seg_dir, (const uint32_t *)vb,
+                         (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
+                         d_flags);
+   else
+      hipLaunchKernelGGL((k_voxelize<uint16_t, false>), dim3((unsigned)n_tiles), dim3(WG), 0, st,
+                         pr, (const SegRec *)segs, seg_dir, (const uint32_t *)vb,
+                         (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
+                         d_flags);
    kt.stop();
    FPL_HIP(hipGetLastError());
    FPL_HIP(hipMemcpyAsync(stage, d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -1149,9 +1158,16 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
       // a tile list longer than 65535: the whole volume again with 32-bit
       // winner indices (every voxel is rewritten; warnings are an OR)
       KernelTimer kt2("voxelize", st);
-      hipLaunchKernelGGL(k_voxelize<int32_t>, dim3((unsigned)n_tiles), dim3(WG), 0, st, pr,
-                         (const SegRec *)segs, seg_dir, (const uint32_t *)vb, (const uint32_t *)rng, d_bd,
-                         d_lsq, d_lm, only_label, label, vector, d_flags);
+      if (lsq_lds)
+         hipLaunchKernelGGL((k_voxelize<int32_t, true>), dim3((unsigned)n_tiles), dim3(WG), 0, st,
+                            pr, (const SegRec *)segs, seg_dir, (const uint32_t *)vb,
+                            (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
+                            d_flags);
+      else
+         hipLaunchKernelGGL((k_voxelize<int32_t, false>), dim3((unsigned)n_tiles), dim3(WG), 0,
+                            st, pr, (const SegRec *)segs, seg_dir, (const uint32_t *)vb,
+                            (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
+                            d_flags);
       kt2.stop();
       FPL_HIP(hipGetLastError());
       FPL_HIP(hipMemcpyAsync(stage, d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));

@@ -680,7 +680,7 @@ struct StepEval {
    int32_t vx, vy, vz;        // the voxel read (flip_z applied to vz)
 };
 
-template <typename Idx>
+template <typename Idx, bool PROPS_LDS>
 __device__ __forceinline__ StepEval eval_step(const Params &P, const TiltDev &T,
                                               const PropDev *sp, const PropDev *props,
                                               const Field<Idx, ST_F32> &F, double px, double py,
@@ -690,7 +690,7 @@ __device__ __forceinline__ StepEval eval_step(const Params &P, const TiltDev &T,
    Vox r{0, make_float3(0.0f, 0.0f, 0.0f), false};
    if (F.inside(ix, iy, iz)) r = F.at(ix, iy, iz);
    const int32_t lab = (r.lab < 0 || r.lab >= P.n_props) ? 0 : r.lab;
-   const PropDev pd = lab < MAX_ADJ_PROPS ? sp[lab] : props[lab];
+   const PropDev pd = PROPS_LDS ? sp[lab] : props[lab];
    E.lab = lab;
    E.K = pd.K;
    E.vx = ix;
@@ -733,7 +733,8 @@ __device__ __forceinline__ StepEval eval_step(const Params &P, const TiltDev &T,
    return E;
 }
 
-template <typename Idx>
+// PROPS_LDS: the property table fits the LDS copy (every lookup a plain LDS read)
+template <typename Idx, bool PROPS_LDS>
 __global__ void __launch_bounds__(WG)
     k_adjoint(const Params *__restrict__ prm, const TiltDev *__restrict__ tilts,
               const PropDev *__restrict__ props, const void *__restrict__ rec_a,
@@ -795,9 +796,9 @@ __global__ void __launch_bounds__(WG)
             c.pad = 0.0f;
             my[n / CK] = c;
          }
-         const StepEval E = eval_step(P, T, sp, props, F, px, py, pz);
+         const StepEval E = eval_step<Idx, PROPS_LDS>(P, T, sp, props, F, px, py, pz);
          if (E.kind == 1) {
-            const float mt = E.lab < MAX_ADJ_PROPS ? sp[E.lab].mu_t : props[E.lab].mu_t;
+            const float mt = PROPS_LDS ? sp[E.lab].mu_t : props[E.lab].mu_t;
             if (mt != 0.0f) two_sum(mh, ml, mt);
          } else if (E.kind == 2) {
             const float nv = E.C * h1 - E.S * h2;
@@ -807,7 +808,7 @@ __global__ void __launch_bounds__(WG)
             two_sum(h1, l1, d1);
             two_sum(h2, l2, d2);
             two_sum(h3, l3, d3);
-            const float mt = E.lab < MAX_ADJ_PROPS ? sp[E.lab].mu_t : props[E.lab].mu_t;
+            const float mt = PROPS_LDS ? sp[E.lab].mu_t : props[E.lab].mu_t;
             if (mt != 0.0f) two_sum(mh, ml, mt);
          }
          px = px + sx;
@@ -855,7 +856,7 @@ __global__ void __launch_bounds__(WG)
             ub[3 * t + 0][tid] = a1 + e1;
             ub[3 * t + 1][tid] = a2 + e2;
             ub[3 * t + 2][tid] = a3 + e3;
-            const StepEval E = eval_step(P, T, sp, props, F, qx, qy, qz);
+            const StepEval E = eval_step<Idx, PROPS_LDS>(P, T, sp, props, F, qx, qy, qz);
             if (E.kind == 2) {
                const float nv = E.C * a1 - E.S * a2;
                const float d1 = E.sd * E.S * a3 + E.omc * (E.C * nv - a1);
@@ -877,7 +878,7 @@ __global__ void __launch_bounds__(WG)
                ry = ry + sy;
                rz = rz + sz;
             }
-            const StepEval E = eval_step(P, T, sp, props, F, rx, ry, rz);
+            const StepEval E = eval_step<Idx, PROPS_LDS>(P, T, sp, props, F, rx, ry, rz);
             if (E.kind == 0) continue;
             if (E.lab != run_lab) {
                flush();
@@ -1425,12 +1426,16 @@ Status adjoint(const fpl_setup *setup, const int64_t dim[3], const int32_t *labe
    const TiltDev *d_t = (const TiltDev *)((const char *)small + B.off_t);
    const PropDev *d_pr = (const PropDev *)((const char *)small + B.off_pr);
    KernelTimer kt("adjoint", st);
-   if (n_vox < ((int64_t)1 << 32))
-      hipLaunchKernelGGL(k_adjoint<uint32_t>, grid, dim3(WG), 0, st, d_p, d_t, d_pr, packed,
-                         grad_images, d_ck, ck_stride, kfac, mfac, d_gp, grad_vector);
-   else
-      hipLaunchKernelGGL(k_adjoint<int64_t>, grid, dim3(WG), 0, st, d_p, d_t, d_pr, packed,
-                         grad_images, d_ck, ck_stride, kfac, mfac, d_gp, grad_vector);
+#define FPL_ADJ(IDX, PL)                                                                          \
+   hipLaunchKernelGGL((k_adjoint<IDX, PL>), grid, dim3(WG), 0, st, d_p, d_t, d_pr, packed,          \
+                      grad_images, d_ck, ck_stride, kfac, mfac, d_gp, grad_vector)
+   const bool props_lds = n_props <= MAX_ADJ_PROPS;
+   if (n_vox < ((int64_t)1 << 32)) {
+      if (props_lds) FPL_ADJ(uint32_t, true); else FPL_ADJ(uint32_t, false);
+   } else {
+      if (props_lds) FPL_ADJ(int64_t, true); else FPL_ADJ(int64_t, false);
+   }
+#undef FPL_ADJ
    kt.stop();
    FPL_HIP(hipGetLastError());
    FPL_HIP(hipMemcpyAsync(stage, d_gp, gp_bytes, hipMemcpyDeviceToHost, st));
