@@ -23,6 +23,14 @@
 // Build with -ffp-contract=off so that no multiply-add is fused: every
 // floating point operation below rounds exactly once, in the order written.
 //
+// Pins (tests/test_oracle*.py): the reference's golden fixture (labels, axes,
+// 5 Slerp tilts), closed forms and brute force for generation and for NN /
+// Slerp / flip_z / step_size / background absorption.  Two branches have no
+// reference vector and only partial checks -- parity unpinned: the Lerp blend
+// of corners with differing axes (checked only on uniform fields, where it
+// reduces to NN) and the tilted-sensor start positions at theta != 0 (checked
+// only at normal incidence and through their step count).
+//
 // Parallelism mirrors the reference so that the timing is a fair CPU
 // baseline: generation lets every thread walk all segments and own the
 // x-slabs x % n_threads == tid (generator.cpp:202-205,315-317); simulation

@@ -245,3 +245,16 @@ def test_simulate_dense_birefringence_queue(interp):
         for t, (theta, phi) in enumerate(tilts):
             ref, _ = _oracle(s, tissue, axis, props, theta, phi)
             assert_intensity_close(imgs[t], ref, 26000)
+
+
+@pytest.mark.parametrize('interp', ['NN', 'Slerp'])
+def test_simulate_flip_z_is_an_index_mirror(golden, interp):
+    """flip_z mirrors every read in z: equal, bit for bit, to the z-mirrored
+    volume without it (simulator.cpp:427-428, 559-560)."""
+    _say(f'simulate: flip_z mirror, {interp}')
+    T, V, P = golden['tissue'], golden['optical_axis'], golden['tissue_properties']
+    Tm, Vm = np.ascontiguousarray(T[:, :, ::-1]), np.ascontiguousarray(V[:, :, ::-1])
+    a = _setup((24, 24, 24), 5.0, interpolate=interp, flip_z_beam=True).run_simulation_tilts(
+        T, V, P, GOLDEN_TILTS)
+    b = _setup((24, 24, 24), 5.0, interpolate=interp).run_simulation_tilts(Tm, Vm, P, GOLDEN_TILTS)
+    assert np.array_equal(a, b)

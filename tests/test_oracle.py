@@ -176,3 +176,94 @@ def test_oracle_errors():
     label[0, 0, 0] = 2
     with pytest.raises(ValueError):
         oracle.simulate(label, vec, props, 0, 0, filter_rotations=rho)
+
+
+# ------------------------------------------------- setup branches (simulator.cpp)
+@pytest.mark.parametrize('interp', ['NN', 'Lerp', 'Slerp'])
+def test_oracle_flip_z_is_an_index_mirror(golden, interp):
+    # GetLabel / GetVec map z -> nz - 1 - z before every read (simulator.cpp:
+    # 427-428, 559-560) and nothing else changes: flip_z on a volume equals the
+    # z-mirrored volume without it, bit for bit
+    T, V, P = golden['tissue'], golden['optical_axis'], golden['tissue_properties']
+    Tm, Vm = np.ascontiguousarray(T[:, :, ::-1]), np.ascontiguousarray(V[:, :, ::-1])
+    for theta, phi in GOLDEN_TILTS[:3]:
+        a, _ = oracle.simulate(T, V, P, theta, phi, voxel_size=5.0, interpolate=interp,
+                               flip_z=True, filter_rotations=GOLDEN_ROT, n_threads=4)
+        b, _ = oracle.simulate(Tm, Vm, P, theta, phi, voxel_size=5.0, interpolate=interp,
+                               flip_z=False, filter_rotations=GOLDEN_ROT, n_threads=4)
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize('step', [0.5, 1.7, 2.0])
+def test_oracle_step_size_uniform_column(step):
+    # uniform column, untilted: n steps of thickness step * voxel_size each
+    v = np.array([0.48, 0.6, 0.64], dtype=np.float32)
+    v = (v / np.linalg.norm(v)).astype(np.float32)
+    dim = (2, 3, 37)
+    dn, mu, vs, wl, I0 = -0.004, 2.0, 0.5, 525.0, 26000.0
+    label, vector, props = _uniform_volume(dim, v, dn, mu)
+    rho = np.linspace(0, np.pi, 9, endpoint=False)
+    img, _ = oracle.simulate(label, vector, props, 0.0, 0.0, step_size=step, voxel_size=vs,
+                             wavelength=wl, light_intensity=I0, filter_rotations=rho,
+                             interpolate='NN')
+    n = oracle.count_steps(dim, step, 0.0, 0.0) // (dim[0] * dim[1])
+    # positions step/2 + k step with floor inside [0, nz)
+    assert n == int(np.ceil((dim[2] - step / 2) / step))
+    t = vs * step * 1e-6
+    vd = v.astype(np.float64)
+    cos2a = 1.0 - (vd[2] / np.linalg.norm(vd))**2
+    delta = np.pi / 2 * dn * 4 * t / (wl * 1e-9) * np.linalg.norm(vd) * cos2a
+    phi = np.arctan2(vd[1], vd[0])
+    expect = I0 / 4 * np.exp(-mu * 1e3 * t * n) * (1 - np.sin(n * delta) * np.sin(2 * (rho - phi)))
+    assert rel_err(img, expect[None, None, :]).max() < 1e-10
+
+
+@pytest.mark.parametrize('opts', [dict(), dict(untilt_sensor_view=False, tissue_refrection=1.3),
+                                  dict(flip_z=True), dict(step_size=0.7)])
+def test_oracle_background_absorption_closed_form(opts):
+    # background only (label 0 inside and outside the volume): every step of
+    # every ray attenuates by exp(-mu_bg 1e3 t); the state stays (1, 0, 0, -1)
+    dim = (6, 5, 23)
+    label = np.zeros(dim, np.int32)
+    vector = np.zeros(dim + (3,), np.float32)
+    props = np.array([[0.0, 3.5], [-0.004, 1.0]])
+    rho = np.linspace(0, np.pi, 9, endpoint=False)
+    step = opts.get('step_size', 1.0)
+    for theta, phi in [(0.0, 0.0), (np.deg2rad(5.5), 0.3), (np.deg2rad(20.0), 4.0)]:
+        img, _ = oracle.simulate(label, vector, props, theta, phi, voxel_size=0.5,
+                                 filter_rotations=rho, interpolate='Slerp', **opts)
+        n = oracle.count_steps(dim, step, theta, phi) // (dim[0] * dim[1])
+        expect = 26000 / 4 * np.exp(-3.5e3 * 0.5e-6 * step * n)
+        assert rel_err(img, np.full_like(img, expect)).max() < 1e-12, (theta, phi, opts)
+
+
+@pytest.mark.parametrize('interp', ['Lerp', 'Slerp'])
+def test_oracle_interpolation_of_a_uniform_field(interp):
+    # all corners of every GetVec carry the same label and axis: Lerp and
+    # Slerp return that axis, so every tilted view equals its NN view
+    v = np.array([0.36, -0.48, 0.8], dtype=np.float32)
+    label, vector, props = _uniform_volume((9, 8, 30), v, -0.004, 1.5)
+    rho = np.linspace(0, np.pi, 9, endpoint=False)
+    for theta, phi in [(np.deg2rad(5.5), 0.0), (np.deg2rad(12.0), 2.2)]:
+        a, _ = oracle.simulate(label, vector, props, theta, phi, voxel_size=0.4,
+                               filter_rotations=rho, interpolate=interp)
+        b, _ = oracle.simulate(label, vector, props, theta, phi, voxel_size=0.4,
+                               filter_rotations=rho, interpolate='NN')
+        assert rel_err(a, b).max() < 1e-12
+
+
+def test_oracle_tilted_sensor_at_normal_incidence():
+    # theta = 0: the refracted angle is 0 and the tilted-sensor start positions
+    # (simulator.cpp:614-719) are the pixel centres of the untilted view
+    rng = np.random.default_rng(2)
+    dim = (7, 6, 9)
+    T = rng.integers(0, 2, dim).astype(np.int32)
+    V = rng.standard_normal(dim + (3,)).astype(np.float32)
+    V /= np.linalg.norm(V, axis=-1, keepdims=True)
+    P = np.array([[0.0, 0.5], [-0.004, 1.0]])
+    rho = np.linspace(0, np.pi, 9, endpoint=False)
+    for interp in ('NN', 'Slerp'):
+        a, _ = oracle.simulate(T, V, P, 0.0, 0.0, interpolate=interp, filter_rotations=rho,
+                               untilt_sensor_view=False, tissue_refrection=1.4)
+        b, _ = oracle.simulate(T, V, P, 0.0, 0.0, interpolate=interp, filter_rotations=rho)
+        assert rel_err(a, b).max() < 1e-12
