@@ -44,7 +44,7 @@ constexpr int WG = 256;                // 4 waves; wave w owns the layers z = w 
 constexpr int SZ = 8;                  // layers of a wave
 constexpr int SLOTS = 64 * SZ;         // voxels of a wave: slot = l * 64 + column
 constexpr int CHUNK = 40;              // segment records staged in LDS at a time (LDS per workgroup
-                                       // stays well below 32 KB: 5 workgroups per CU)
+                                       // 27 KB in the 16-bit build)
 constexpr int QN = 128;                // candidate ring of a wave (never holds 128 entries)
 constexpr int LSQ_LDS = 64;            // layer tables up to this many rows are read from LDS
 constexpr uint32_t SHORT_LIST = 65535; // longest tile list of the 16-bit winner indices
@@ -493,18 +493,43 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// ascending bitonic sort of one value per lane over the wave
+__device__ __forceinline__ uint32_t wave_sort(uint32_t v, int lane) {
+   for (int k = 2; k <= 64; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+         const uint32_t o = (uint32_t)__shfl_xor((int)v, j);
+         const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+         v = (lower == up) ? min(v, o) : max(v, o);
+      }
+   return v;
+}
 __device__ __forceinline__ float uniform(float v) {
    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
 // A voxel's state is one 64-bit key: the float bits of its stored squared
-// distance (high word; non-negative floats order as integers) and
-// 0x80000000 | label (low word).  A candidate of a batch enters with the low
-// word 63 - lane, so the minimum key over the stored state and the batch is
-// the smallest distance with ties going to the latest candidate -- the state
-// the reference's sequential rule "update if stored >= new" ends in.
+// distance (high word; non-negative floats order as integers) and a low word
+// that breaks ties.
+//
+// Ordered mode (the tile's candidates applied in global segment order, which
+// the "objects overlap" warning needs): at rest the low word is 0x80000000 |
+// label (32-bit winner indices, separate winner array) or 0x80000000 |
+// label15 << 16 | position + 1 (16-bit list positions, no winner array;
+// label15 0x7FFF = "label later").  A candidate of a batch enters with the
+// low word 63 - lane, so the minimum key over the stored state and the batch
+// is the smallest distance with ties going to the latest candidate -- the
+// state the reference's sequential rule "update if stored >= new" ends in.
+//
+// Free mode (16-bit positions; the warning already raised or the tile's
+// segments all from one bundle, so it cannot be raised here): the low word is
+// 0xFFFF - position.  The minimum key is then the smallest distance with ties
+// going to the latest segment whatever the order the candidates come in, so
+// a chunk's records are visited nearest-to-the-tile-centre first, which
+// finds each voxel's winner with fewer improving candidates.
 constexpr unsigned long long KEY_EMPTY = (0x7F800000ull << 32) | 0x80000000ull; // +inf, label 0
 constexpr int32_t LABEL_LATER = 0x7FFFFFFF; // label left to the final pass
+constexpr uint32_t LABEL15_LATER = 0x7FFF;  // the same in the packed low word
 
 // std::lower_bound over the bundle's double scale^2 list with the float ratio
 // (generator.cpp:345-349) -> label ly + 1 + bundle * max_layer (:351); the
@@ -518,14 +543,20 @@ __device__ __forceinline__ int32_t layer_label(float dist_sq, float r_sq, const 
    return ly + 1 + bundle * max_layer;
 }
 
-// SegT: uint16_t stores the winner's position in the tile list + 1 (0: none),
-// which keeps the LDS of a workgroup within 32 KB (5 workgroups per CU);
-// int32_t stores its global index (-1: none), for tiles with longer lists
+// SegT: uint16_t keeps the winner's position in the tile list inside the key
+// (LDS of a workgroup below 27 KB: 6 workgroups per CU); int32_t stores its
+// global index (-1: none) in a separate array, for tiles with longer lists or
+// labels that do not fit 15 bits
 template <typename SegT> struct WaveState {
    unsigned long long key[SLOTS];
    SegT seg[SLOTS];
    uint32_t queue[QN];  // candidates (chunk index << 9 | slot), ring
    uint2 tab[64];       // per chunk record: candidate prefix | box << 16, division magics
+};
+template <> struct WaveState<uint16_t> {
+   unsigned long long key[SLOTS];
+   uint32_t queue[QN];
+   uint2 tab[64];
 };
 
 struct alignas(16) LdsSeg { // 144-byte stride spreads records over the LDS banks
@@ -539,7 +570,9 @@ template <typename SegT>
 __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chunk,
                                           const uint32_t *chunk_idx, int n, int q_head,
                                           int lane, int xt, int yt, int zw, int max_layer,
-                                          const double *lsq, uint32_t *flags, bool &ov) {
+                                          const double *lsq, uint32_t *flags, bool &ov,
+                                          bool free_mode) {
+   constexpr bool packed = sizeof(SegT) == 2;
    const bool valid = lane < n;
    const uint32_t e = valid ? W.queue[(q_head + lane) & (QN - 1)] : 0u;
    const int c = (int)(e >> 9), slot = (int)(e & (SLOTS - 1));
@@ -570,8 +603,27 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
       // the label is needed here only while the overlap warning is undecided;
       // otherwise the final pass computes it for the winning segment alone
       new_label = LABEL_LATER;
-      if (acc && !ov) new_label = layer_label(dist_sq, r_sq, lsq, R.layer_off, R.bundle, max_layer);
+      if (acc && !ov && !free_mode)
+         new_label = layer_label(dist_sq, r_sq, lsq, R.layer_off, R.bundle, max_layer);
    }
+   if (packed && free_mode) {
+      // order-free minimum; chunk_idx holds the list position + 1
+      if (acc)
+         atomicMin(&W.key[slot], ((unsigned long long)__float_as_uint(dist_sq) << 32) |
+                                     (unsigned long long)(0x10000u - chunk_idx[c]));
+      wave_sync();
+      return;
+   }
+   // the low word a winner keeps at rest
+   uint32_t rest_lo = 0x80000000u | (uint32_t)new_label;
+   if (packed)
+      rest_lo = 0x80000000u |
+                (new_label == LABEL_LATER ? LABEL15_LATER : (uint32_t)new_label) << 16 |
+                (valid ? chunk_idx[c] : 0u);
+   auto label_of = [&](unsigned long long key) -> int {
+      if (packed) return (int)((uint32_t)(key >> 16) & 0x7FFFu);
+      return (int)(uint32_t)(key & 0x7FFFFFFFull);
+   };
    // The overlap warning (generator.cpp:353-359) depends on the order of the
    // updates.  Unless it is already raised, a batch whose labels (new ones and
    // the ones they may replace) all lie within max_layer of each other cannot
@@ -583,7 +635,7 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
       int mn = INT_MAX, mx = INT_MIN;
       if (acc) {
          mn = mx = new_label;
-         const int ol = (int)(uint32_t)(old & 0x7FFFFFFFull);
+         const int ol = label_of(old);
          if (ol != 0) {
             mn = min(mn, ol);
             mx = max(mx, ol);
@@ -594,8 +646,7 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
       serial = uniform((int)(mx != INT_MIN && (long long)mx - mn >= max_layer)) != 0;
    }
    wave_sync();
-   const unsigned long long rest =
-       ((unsigned long long)__float_as_uint(dist_sq) << 32) | 0x80000000ull | (uint32_t)new_label;
+   const unsigned long long rest = ((unsigned long long)__float_as_uint(dist_sq) << 32) | rest_lo;
    if (!serial) {
       const unsigned long long k =
           ((unsigned long long)__float_as_uint(dist_sq) << 32) | (unsigned long long)(63 - lane);
@@ -603,7 +654,7 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
       wave_sync();
       if (acc && W.key[slot] == k) {
          W.key[slot] = rest;
-         W.seg[slot] = (SegT)chunk_idx[c];
+         if constexpr (!packed) W.seg[slot] = (SegT)chunk_idx[c];
       }
    } else {
       uint32_t overlap = 0;
@@ -611,10 +662,10 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
          if (lane == i && acc) {
             const unsigned long long cur = W.key[slot];
             if (__uint_as_float((uint32_t)(cur >> 32)) >= dist_sq) {
-               const int ol = (int)(uint32_t)(cur & 0x7FFFFFFFull);
+               const int ol = label_of(cur);
                if (ol != 0 && abs(ol - new_label) >= max_layer) overlap = 1;
                W.key[slot] = rest;
-               W.seg[slot] = (SegT)chunk_idx[c];
+               if constexpr (!packed) W.seg[slot] = (SegT)chunk_idx[c];
             }
          }
          wave_sync();
@@ -688,10 +739,20 @@ __global__ void __launch_bounds__(WG)
    WaveState<SegT> &W = wstate[wave];
    for (int s = lane; s < SLOTS; s += 64) {
       W.key[s] = KEY_EMPTY;
-      W.seg[s] = short_seg ? (SegT)0 : (SegT)-1;
+      if constexpr (!short_seg) W.seg[s] = (SegT)-1;
    }
    bool ov = (__atomic_load_n(&flags[0], __ATOMIC_RELAXED) & FPL_WARN_OVERLAP) != 0;
    ov = uniform((int)ov) != 0;
+   // free mode (see the key layout above): the warning is raised already, or
+   // the list (global segment order, bundle-major) starts and ends in one
+   // bundle, so no update here can replace another bundle's label
+   bool free_mode = false;
+   if (short_seg) {
+      bool single = true;
+      if (end > beg) single = segs[list[beg]].bundle == segs[list[end - 1]].bundle;
+      free_mode = uniform((int)(ov || single)) != 0;
+   }
+   const float tcx = (float)(xt + TX / 2), tcy = (float)(yt + TY / 2), tcz = (float)(zt + TZ / 2);
    int q_head = 0, q_tail = 0;
    wave_sync();
 
@@ -715,9 +776,22 @@ __global__ void __launch_bounds__(WG)
       // rem = thr - exy_min >= 0; h covers that with margin (the hardware
       // square root is within 1 ulp), and the range ends are computed in
       // double, exact for any float box and voxel index.
+      // free mode: the lanes take the records nearest to the tile centre first
+      int myrec = lane;
+      if (free_mode) {
+         uint32_t k = 0xFFFFFFFFu;
+         if (lane < n) {
+            const SegRec &R = chunk[lane].r;
+            const float mx = 0.5f * (float)(R.p[0] + R.q[0]) - tcx;
+            const float my = 0.5f * (float)(R.p[1] + R.q[1]) - tcy;
+            const float mz = 0.5f * (float)(R.p[2] + R.q[2]) - tcz;
+            k = (__float_as_uint(mx * mx + my * my + mz * mz) & ~63u) | (uint32_t)lane;
+         }
+         myrec = (int)(wave_sort(k, lane) & 63u);
+      }
       int nl = 0, la = 0;
       if (lane < n) {
-         const SegRec &R = chunk[lane].r;
+         const SegRec &R = chunk[myrec].r;
          const int z0 = max(R.lo[2], zt), z1 = min(R.hi[2], zt + TZ);
          const int cx0 = max(R.lo[0], xt), cx1 = min(R.hi[0], xt + TX);
          const int cy0 = max(R.lo[1], yt), cy1 = min(R.hi[1], yt + TY);
@@ -773,10 +847,11 @@ __global__ void __launch_bounds__(WG)
             rcur = s0;
          }
          const int ri = __shfl(rec_info, c);
+         const int rc = __shfl(myrec, c); // the record in the chunk
          int cnt = 0;
          uint32_t geo = 0, magic = 0;
          if (p < n_pairs) {
-            const SegRec &R = chunk[c].r;
+            const SegRec &R = chunk[rc].r;
             const int l = (ri & 7) + (p - (ri >> 3));
             const int cx0 = max(R.lo[0], xt), cx1 = min(R.hi[0], xt + TX);
             const int cy0 = max(R.lo[1], yt), cy1 = min(R.hi[1], yt + TY);
@@ -804,7 +879,7 @@ __global__ void __launch_bounds__(WG)
                   geo = (uint32_t)(xa - xt) | (uint32_t)(ya - yt) << 3 | (uint32_t)l << 6 |
                         (uint32_t)(ncy - 1) << 9;
                   // floor(i / n) = (i * ceil(2^15 / n)) >> 15 for i < 512, n <= 8
-                  magic = (uint32_t)(32767 / ncy + 1) | (uint32_t)c << 16;
+                  magic = (uint32_t)(32767 / ncy + 1) | (uint32_t)rc << 16;
                }
             }
          }
@@ -867,7 +942,7 @@ __global__ void __launch_bounds__(WG)
                if (q_tail - q_head >= 64) {
                   wave_sync();
                   run_batch(W, chunk, chunk_idx, 64, q_head, lane, xt, yt, zw, max_layer, lsq,
-                            flags, ov);
+                            flags, ov, free_mode);
                   q_head += 64;
                }
             }
@@ -879,7 +954,7 @@ __global__ void __launch_bounds__(WG)
       while (q_tail > q_head) {
          const int nb = min(64, q_tail - q_head);
          run_batch(W, chunk, chunk_idx, nb, q_head, lane, xt, yt, zw, max_layer, lsq, flags,
-                   ov);
+                   ov, free_mode);
          q_head += nb;
       }
       if (!ov)
@@ -892,8 +967,15 @@ __global__ void __launch_bounds__(WG)
    // for 16-bit positions) two passes ahead, the record one pass ahead.
    wave_sync();
    auto winner = [&](int j) -> int32_t {
-      const uint32_t w = (uint32_t)W.seg[(lane & 7) * 64 + j * 8 + (lane >> 3)];
-      return short_seg ? (w ? (int32_t)list[beg + w - 1u] : -1) : (int32_t)w;
+      const int slot = (lane & 7) * 64 + j * 8 + (lane >> 3);
+      if constexpr (short_seg) {
+         const unsigned long long key = W.key[slot];
+         uint32_t w = (uint32_t)key & 0xFFFFu; // list position + 1 (0: none)
+         if (free_mode) w = (uint32_t)(key >> 32) == 0x7F800000u ? 0u : 0x10000u - w;
+         return w ? (int32_t)list[beg + w - 1u] : -1;
+      } else {
+         return (int32_t)W.seg[slot];
+      }
    };
    int32_t sg = winner(0), sg1 = winner(1);
    WinRec cur, nxt;
@@ -908,6 +990,11 @@ __global__ void __launch_bounds__(WG)
          const int64_t ind = ((int64_t)vx * pr.ny + vy) * pr.nz + vz;
          const unsigned long long key = W.key[slot];
          int32_t lab = (int32_t)(uint32_t)(key & 0x7FFFFFFFull);
+         if (short_seg) {
+            const uint32_t l15 = (uint32_t)(key >> 16) & 0x7FFFu;
+            lab = free_mode ? (sg >= 0 ? LABEL_LATER : 0)
+                            : (l15 == LABEL15_LATER ? LABEL_LATER : (int32_t)l15);
+         }
          const double pt[3] = {vx + 0.5, vy + 0.5, vz + 0.5};
          if (sg >= 0 && lab == LABEL_LATER) {
             // the winner's label from its exact test: the stored float distance
@@ -1138,25 +1225,32 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
                          (uint32_t *)rng);
    }
    const bool lsq_lds = pr.n_lsq <= LSQ_LDS;
-   KernelTimer kt("voxelize", st);
-   if (lsq_lds)
-      hipLaunchKernelGGL((k_voxelize<uint16_t, true>), dim3((unsigned)n_tiles), dim3(WG), 0, st,
-                         pr, (const SegRec *)segs, seg_dir, (const uint32_t *)vb,
-                         (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
-                         d_flags);
-   else
-      hipLaunchKernelGGL((k_voxelize<uint16_t, false>), dim3((unsigned)n_tiles), dim3(WG), 0, st,
-                         pr, (const SegRec *)segs, seg_dir, (const uint32_t *)vb,
-                         (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
-                         d_flags);
-   kt.stop();
-   FPL_HIP(hipGetLastError());
-   FPL_HIP(hipMemcpyAsync(stage, d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-   FPL_HIP(hipStreamSynchronize(st));
-   kt.collect();
-   if (((uint32_t *)stage)[1] & 1u) {
-      // a tile list longer than 65535: the whole volume again with 32-bit
-      // winner indices (every voxel is rewritten; warnings are an OR)
+   // the 16-bit build packs a winner's label into 15 bits of its key
+   const bool short_labels = n_props <= (int64_t)LABEL15_LATER;
+   bool rerun = !short_labels;
+   if (short_labels) {
+      KernelTimer kt("voxelize", st);
+      if (lsq_lds)
+         hipLaunchKernelGGL((k_voxelize<uint16_t, true>), dim3((unsigned)n_tiles), dim3(WG), 0,
+                            st, pr, (const SegRec *)segs, seg_dir, (const uint32_t *)vb,
+                            (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
+                            d_flags);
+      else
+         hipLaunchKernelGGL((k_voxelize<uint16_t, false>), dim3((unsigned)n_tiles), dim3(WG), 0,
+                            st, pr, (const SegRec *)segs, seg_dir, (const uint32_t *)vb,
+                            (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
+                            d_flags);
+      kt.stop();
+      FPL_HIP(hipGetLastError());
+      FPL_HIP(hipMemcpyAsync(stage, d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      FPL_HIP(hipStreamSynchronize(st));
+      kt.collect();
+      rerun = (((uint32_t *)stage)[1] & 1u) != 0;
+   }
+   if (rerun) {
+      // a tile list longer than 65535 (or labels beyond 15 bits): the whole
+      // volume with 32-bit winner indices (every voxel is rewritten; warnings
+      // are an OR)
       KernelTimer kt2("voxelize", st);
       if (lsq_lds)
          hipLaunchKernelGGL((k_voxelize<int32_t, true>), dim3((unsigned)n_tiles), dim3(WG), 0, st,
