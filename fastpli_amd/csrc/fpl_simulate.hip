@@ -112,6 +112,10 @@ template <typename Idx, int ST> struct Field {
    __device__ __forceinline__ bool inside(int32_t x, int32_t y, int32_t z) const {
       return x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
    }
+   // record of the voxel holding the sample point (px, py, pz) whose z floor
+   // `fz` is inside [0, nz) when `live`; outside x / y it is background
+   // (simulator.cpp:430-432).  One branch: all tests are evaluated.
+   __device__ __forceinline__ Vox fetch(double px, double py, double fz, bool live) const;
    // record of an in-range voxel; flip_z mirrors z as GetLabel / GetVec do
    // (simulator.cpp:427-428, 559-560)
    __device__ __forceinline__ Vox at(int32_t x, int32_t y, int32_t z) const {
@@ -157,6 +161,17 @@ template <typename Idx, int ST> struct Field {
 // (they only ever select "outside -> background")
 __device__ __forceinline__ int32_t ifloor(double v) {
    return (int32_t)fmin(fmax(floor(v), -1073741824.0), 1073741824.0);
+}
+
+template <typename Idx, int ST>
+__device__ __forceinline__ Vox Field<Idx, ST>::fetch(double px, double py, double fz,
+                                                     bool live) const {
+   const int32_t jx = ifloor(px), jy = ifloor(py);
+   const int32_t jz = live ? (int32_t)fz : 0;
+   const bool in = live & ((uint32_t)jx < (uint32_t)nx) & ((uint32_t)jy < (uint32_t)ny);
+   Vox r{0, make_float3(0.0f, 0.0f, 0.0f), false};
+   if (in) r = at(jx, jy, jz);
+   return r;
 }
 
 __device__ __forceinline__ bool is_zero(const float3 &v) {
@@ -506,19 +521,15 @@ __global__ void __launch_bounds__(WG, 6)
       // a voxel centre): a step is cheap, so every ray runs its own steps and
       // the record of step k + 1 is requested before step k is processed
       double fz = floor(pz);
-      bool live = valid && fz >= 0.0 && fz < nzd;
-      int32_t ix = ifloor(px), iy = ifloor(py), iz = ifloor(fz);
-      Vox r{0, make_float3(0.0f, 0.0f, 0.0f), false}; // outside: background (simulator.cpp:430-432)
-      if (live && F.inside(ix, iy, iz)) r = F.at(ix, iy, iz);
+      bool live = valid & (fz >= 0.0) & (fz < nzd);
+      Vox r = F.fetch(px, py, fz, live);
       while (live) {
          px = px + sx;
          py = py + sy;
          pz = pz + sz;
          fz = floor(pz);
-         const bool qlive = fz >= 0.0 && fz < nzd;
-         const int32_t jx = ifloor(px), jy = ifloor(py), jz = ifloor(fz);
-         Vox rn{0, make_float3(0.0f, 0.0f, 0.0f), false};
-         if (qlive && F.inside(jx, jy, jz)) rn = F.at(jx, jy, jz);
+         const bool qlive = (fz >= 0.0) & (fz < nzd);
+         const Vox rn = F.fetch(px, py, fz, qlive);
 
          int32_t lab = r.lab;
          if (lab < 0 || lab >= n_props) {
@@ -547,12 +558,8 @@ __global__ void __launch_bounds__(WG, 6)
       // the record of the current step is loaded one iteration ahead; the loop
       // condition is the reference's (simulator.cpp:259-261)
       const double fz0 = floor(pz);
-      bool live = valid && fz0 >= 0.0 && fz0 < nzd;
-      Vox r{0, make_float3(0.0f, 0.0f, 0.0f), false}; // outside: background (simulator.cpp:430-432)
-      {
-         const int32_t ix = ifloor(px), iy = ifloor(py), iz = ifloor(fz0);
-         if (live && F.inside(ix, iy, iz)) r = F.at(ix, iy, iz);
-      }
+      bool live = valid & (fz0 >= 0.0) & (fz0 < nzd);
+      Vox r = F.fetch(px, py, fz0, live);
       for (;;) {
          int32_t lab = r.lab;
          if (lab < 0 || lab >= n_props) {
@@ -585,11 +592,8 @@ __global__ void __launch_bounds__(WG, 6)
                py = py + sy;
                pz = pz + sz;
                const double fz = floor(pz);
-               live = fz >= 0.0 && fz < nzd;
-               const int32_t jx = ifloor(px), jy = ifloor(py), jz = ifloor(fz);
-               Vox rn{0, make_float3(0.0f, 0.0f, 0.0f), false};
-               if (live && F.inside(jx, jy, jz)) rn = F.at(jx, jy, jz);
-               r = rn;
+               live = (fz >= 0.0) & (fz < nzd);
+               r = F.fetch(px, py, fz, live);
             }
             tail += (uint32_t)__popcll(qmask);
          }
