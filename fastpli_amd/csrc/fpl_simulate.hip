@@ -193,20 +193,11 @@ __device__ __forceinline__ float3 lerp_orient(const float3 &v, const float3 &u, 
 
 // sin(a theta) / sin(theta) = a 2F1(1 - a, 1 + a; 3/2; z), z = sin^2(theta/2),
 // theta in [0, pi/2].  Term ratio: (k^2 - a^2) z / ((k + 1/2) k).  For
-// z <= 1e-3 three terms, for z <= 0.1 (theta <= 37 deg) eight terms in Horner
-// form leave < 1e-9; beyond
-// that the series is summed until a term drops below 1e-8 (at most 27 terms).
+// z <= 0.1 (theta <= 37 deg) eight terms in Horner form leave < 1e-9 (one tier
+// for all small angles: lanes of a wave then take the same path); beyond that
+// the series is summed until a term drops below 1e-8 (at most 27 terms).
 __device__ __forceinline__ float slerp_weight(float a, float z) {
    const float a2 = a * a;
-   if (z <= 1e-3f) { // truncation < 1e-12
-      float acc = 1.0f;
-#pragma unroll
-      for (int k = 3; k >= 1; k--) {
-         const float r = ((float)(k * k) - a2) * (1.0f / ((k + 0.5f) * (float)k));
-         acc = fmaf(z * r, acc, 1.0f);
-      }
-      return a * acc;
-   }
    if (z <= 0.1f) {
       float acc = 1.0f;
 #pragma unroll
