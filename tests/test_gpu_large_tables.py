@@ -82,3 +82,20 @@ def test_padded_property_table_adjoint_unchanged():
     assert np.max(np.abs(gp2[:n] - gp)) <= 1e-12 * np.max(np.abs(gp))
     assert not np.any(gp2[n:])
     assert np.max(np.abs(gv2 - gv)) <= 1e-5 * np.max(np.abs(gv))
+
+
+def test_labels_beyond_15_bits():
+    _say('large tables: 33000 layers (labels beyond 15 bits), generate vs oracle')
+    from fastpli_amd.objects import FiberBundles
+    rng = np.random.default_rng(9)
+    fibers = []
+    for _ in range(3):
+        start = rng.uniform(0, 12, 3)
+        pts = start + np.cumsum(rng.normal(scale=2.0, size=(4, 3)), axis=0)
+        fibers.append(np.hstack([pts, rng.uniform(1.5, 3.0, (4, 1))]))
+    scales = np.linspace(0.3, 1.0, 33000)
+    layers = [(float(sc), (-0.004 if k % 2 else 0.0), float(k % 7), 'p' if k % 2 else 'b')
+              for k, sc in enumerate(scales)]
+    fbs = FiberBundles([fibers], [layers])
+    label, vector, _ = _compare(fbs, (12, 12, 12), [0.0, 0.0, 0.0], 1.0)
+    assert label.max() > 0x7FFF
