@@ -350,16 +350,15 @@ __host__ __device__ inline int64_t ray_tiles(int64_t nx, int64_t ny) {
 // a fiber tissue are birefringent, scattered over the 64 rays of a wave; one
 // thread per ray would run the costly part of a step (corner gather +
 // interpolation, axis rotation, retarder angles) for the whole wave whenever
-// one ray needs it.
-// Instead every ray walks its voxels one step per iteration and queues each
-// birefringent step in a per-wave LDS ring (ballot + mbcnt, no atomics);
-// attenuation-only steps are summed at once (they commute with the
-// retarders).  When 64 steps are queued (or no ray can go on) the wave
-// evaluates them, one per lane, and writes every step's retarder (cos 2phi,
-// sin 2phi, sin delta, 1 - cos delta, mu t) into its entry; each ray then
-// applies its own evaluated steps to its Poincare state in its own order.  A
-// ray holds at most QDEPTH unapplied steps: at the next birefringent voxel it
-// waits (attenuation-only voxels it still passes).
+// one ray needs it.  Instead every ray walks its voxels one step per
+// iteration and queues each birefringent step in a per-wave LDS ring (ballot
+// + mbcnt, no atomics); attenuation-only steps are summed at once (they
+// commute with the retarders).  When 64 steps are queued (or no ray can go
+// on) the wave evaluates them, one per lane, and writes every step's retarder
+// (cos 2phi, sin 2phi, sin delta, 1 - cos delta, mu t) into its entry; each
+// ray then applies its own evaluated steps to its Poincare state in its own
+// order.  A ray holds at most QDEPTH unapplied steps: at the next
+// birefringent voxel it waits (attenuation-only voxels it still passes).
 constexpr int QCAP = 128;  // ring entries per wave: < 64 waiting + 64 new
 constexpr int QDEPTH = 4;  // queued steps per ray (8-bit sequence numbers in one word)
 
@@ -431,8 +430,8 @@ __device__ __forceinline__ void apply_retarder(float C, float S, float sd, float
    two_sum(h3, l3, d3);
 }
 
-// at most 80 VGPRs (6 waves per SIMD): the queue evaluation's interpolation
-// needs them, and spilling costs more than the lost occupancy
+// At most 80 VGPRs (6 waves per SIMD): the queue evaluation's interpolation
+// needs them, and spilling costs more than the lost occupancy.
 // PROPS_LDS: the whole property table fits the LDS copy (else it is read from
 // HBM through the caches); a compile-time choice keeps every lookup a plain
 // LDS read.
