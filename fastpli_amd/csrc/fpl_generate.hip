@@ -282,10 +282,63 @@ __device__ __forceinline__ void unit_to_float(double x, double y, double z, floa
    out[2] = (float)z;
 }
 
+// The slab of a segment (free mode only; see the key layout at k_voxelize).
+// With u the unit direction of segment k = [P_k, P_k+1], a(v) = (v - P_k).u
+// and R_out = sqrt(f_last) max(r_k, r_k+1) (no voxel farther from the segment
+// can pass its radius test), a voxel with a(v) > L_k + m + d_e is beyond the
+// segment's end, and by the choice of d_e its projection on segment k+1 is at
+// least m inside it: segment k+1 is at least as close (exactly: |v - P_k+1|^2
+// minus the squared projection) and, with r_k+2 >= r_k+1, passes its radius
+// test whenever k does, so the later segment k+1 (or one it loses to) wins
+// over k.  Likewise a voxel with a(v) < -(m + d_s) is at least m^2 closer to
+// segment k-1 (strictly, also after rounding to float: m >= R_out 2^-9),
+// which wins when r_k-1 >= r_k.  d = (m (1 - c) + s R_out) / c with c, s the
+// cosine and sine of the bend to the neighbour (no clip for bends beyond
+// 60 deg, short segments, missing neighbours or the radius conditions
+// failing).  m also covers the float rounding of a(v) in the clip test.
+// seg_clip[2 i] = (u, a_lo), seg_clip[2 i + 1] = (a_hi, P_k - 0.5 as float)
+__device__ void segment_clip(const Params &pr, const double *P, int64_t i, bool has_prev,
+                             bool has_next, const double p[3], const double q[3], double rp,
+                             double rq, double r_out, float4 *clip) {
+   const double inf = __builtin_huge_val();
+   double d[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
+   const double L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+   const double m = fmax(0x1p-5, r_out * 0x1p-9) +
+                    (double)(pr.nx + pr.ny + pr.nz) * 0x1p-19;
+   double lo = -inf, hi = inf;
+   double u[3] = {0.0, 0.0, 0.0};
+   if (L >= 4.0 * m) {
+      for (int k = 0; k < 3; k++) u[k] = d[k] / L;
+      auto delta = [&](const double e[3], double Le) {
+         double c = 0.0;
+         for (int k = 0; k < 3; k++) c += u[k] * e[k] / Le;
+         if (!(c > 0.5)) return inf;
+         const double sn = sqrt(fmax(0.0, 1.0 - c * c));
+         return (m * (1.0 - c) + sn * r_out) / c;
+      };
+      if (has_next) {
+         double pn[3], rn;
+         load_point(P, i + 2, pr, pn, rn);
+         const double e[3] = {pn[0] - q[0], pn[1] - q[1], pn[2] - q[2]};
+         const double Le = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+         if (Le >= 2.0 * m && rn >= rq) hi = L + m + delta(e, Le);
+      }
+      if (has_prev) {
+         double pp[3], rpp;
+         load_point(P, i - 1, pr, pp, rpp);
+         const double e[3] = {p[0] - pp[0], p[1] - pp[1], p[2] - pp[2]};
+         const double Le = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+         if (Le >= 2.0 * m && rpp >= rp) lo = -(m + delta(e, Le));
+      }
+   }
+   clip[0] = make_float4((float)u[0], (float)u[1], (float)u[2], (float)lo);
+   clip[1] = make_float4((float)hi, (float)(p[0] - 0.5), (float)(p[1] - 0.5), (float)(p[2] - 0.5));
+}
+
 __global__ void k_segments(Params pr, const double *__restrict__ P,
                            const int64_t *__restrict__ fiber_ptr,
                            const BundleDev *__restrict__ bd, const uint8_t *__restrict__ fiber_ok,
-                           SegRec *segs, float4 *seg_dir, int64_t *seg_count) {
+                           SegRec *segs, float4 *seg_dir, float4 *seg_clip, int64_t *seg_count) {
    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
    if (i >= pr.n_points) return;
    const int64_t f = find_fiber(fiber_ptr, pr.n_fibers, i);
@@ -335,6 +388,8 @@ __global__ void k_segments(Params pr, const double *__restrict__ P,
          float d[3];
          unit_to_float(R.q[0] - R.p[0], R.q[1] - R.p[1], R.q[2] - R.p[2], d);
          seg_dir[i] = make_float4(d[0], d[1], d[2], 0.0f);
+         segment_clip(pr, P, i, i > fiber_ptr[f], i + 2 < fiber_ptr[f + 1], R.p, R.q, rp, rq,
+                      max_r, seg_clip + 2 * i);
          const int64_t tx0 = ilo[0] / TX, tx1 = (ihi[0] - 1) / TX;
          const int64_t ty0 = ilo[1] / TY, ty1 = (ihi[1] - 1) / TY;
          const int64_t tz0 = ilo[2] / TZ, tz1 = (ihi[2] - 1) / TZ;
@@ -694,6 +749,7 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
 template <typename SegT, bool LSQ_IN_LDS>
 __global__ void __launch_bounds__(WG)
     k_voxelize(Params pr, const SegRec *__restrict__ segs, const float4 *__restrict__ seg_dir,
+               const float4 *__restrict__ seg_clip,
                const uint32_t *__restrict__ list,
                const uint32_t *__restrict__ range, const BundleDev *__restrict__ bd,
                const double *__restrict__ layer_sq, const uint8_t *__restrict__ layer_model,
@@ -702,6 +758,7 @@ __global__ void __launch_bounds__(WG)
    __shared__ WaveState<SegT> wstate[WG / 64];
    __shared__ LdsSeg chunk[CHUNK];
    __shared__ uint32_t chunk_idx[CHUNK];
+   __shared__ uint32_t chunk_gidx[CHUNK]; // global segment index of each record
 
    const int n_tiles = pr.ntx * pr.nty * pr.ntz;
    const int t = tile_of_block((int)blockIdx.x, n_tiles);
@@ -763,7 +820,10 @@ __global__ void __launch_bounds__(WG)
          const int c = e >> 3, piece = e & 7;
          const uint32_t sidx = list[base + c];
          ((int4 *)&chunk[c].r)[piece] = ((const int4 *)&segs[sidx])[piece];
-         if (piece == 0) chunk_idx[c] = short_seg ? base - beg + (uint32_t)c + 1u : sidx;
+         if (piece == 0) {
+            chunk_idx[c] = short_seg ? base - beg + (uint32_t)c + 1u : sidx;
+            chunk_gidx[c] = sidx;
+         }
       }
       __syncthreads();
 
@@ -869,10 +929,34 @@ __global__ void __launch_bounds__(WG)
                    fmaf(__builtin_amdgcn_sqrtf((float)rx), 1.0f + 0x1p-10f, 0x1p-10f);
                const float hy =
                    fmaf(__builtin_amdgcn_sqrtf((float)ry), 1.0f + 0x1p-10f, 0x1p-10f);
-               const int xa = max(cx0, (int)ceil((double)R.bl[0] - (double)hx));
-               const int xb = min(cx1, (int)floor((double)R.bh[0] + (double)hx) + 1);
-               const int ya = max(cy0, (int)ceil((double)R.bl[1] - (double)hy));
-               const int yb = min(cy1, (int)floor((double)R.bh[1] + (double)hy) + 1);
+               int xa = max(cx0, (int)ceil((double)R.bl[0] - (double)hx));
+               int xb = min(cx1, (int)floor((double)R.bh[0] + (double)hx) + 1);
+               int ya = max(cy0, (int)ceil((double)R.bl[1] - (double)hy));
+               int yb = min(cy1, (int)floor((double)R.bh[1] + (double)hy) + 1);
+               if (free_mode && xa < xb && ya < yb) {
+                  // the columns of the segment's slab a in [a_lo, a_hi] (above);
+                  // a is linear in x and y, so a column (row) keeps a voxel iff
+                  // the range of a over its end voxels meets the slab
+                  const uint32_t gi = chunk_gidx[rc];
+                  const float4 c0 = seg_clip[2 * gi], c1 = seg_clip[2 * gi + 1];
+                  const float alo = c0.w - 0x1p-8f, ahi = c1.x + 0x1p-8f;
+                  const float a0 = (zf - c1.w) * c0.z - c1.y * c0.x - c1.z * c0.y;
+                  auto meets = [&](float x0, float x1, float y0, float y1) {
+                     const float p00 = fmaf(c0.x, x0, fmaf(c0.y, y0, a0));
+                     const float p11 = fmaf(c0.x, x1, fmaf(c0.y, y1, a0));
+                     const float p01 = fmaf(c0.x, x0, fmaf(c0.y, y1, a0));
+                     const float p10 = fmaf(c0.x, x1, fmaf(c0.y, y0, a0));
+                     const float lo = fminf(fminf(p00, p11), fminf(p01, p10));
+                     const float hi = fmaxf(fmaxf(p00, p11), fmaxf(p01, p10));
+                     return hi >= alo && lo <= ahi;
+                  };
+                  const float fya = (float)ya, fyb = (float)(yb - 1);
+                  const float fxa = (float)xa, fxb = (float)(xb - 1);
+                  while (xa < xb && !meets((float)xa, (float)xa, fya, fyb)) xa++;
+                  while (xa < xb && !meets((float)(xb - 1), (float)(xb - 1), fya, fyb)) xb--;
+                  while (ya < yb && !meets(fxa, fxb, (float)ya, (float)ya)) ya++;
+                  while (ya < yb && !meets(fxa, fxb, (float)(yb - 1), (float)(yb - 1))) yb--;
+               }
                if (xa < xb && ya < yb) {
                   const int ncy = yb - ya;
                   cnt = (xb - xa) * ncy;
@@ -1157,8 +1241,9 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
    void *boxes, *fok, *segs, *cnt, *off, *flags;
    FPL_TRY(ws_get(st, WS_BOXES, sizeof(double) * 6 * (nf + nb + 1), &boxes));
    FPL_TRY(ws_get(st, WS_FIBER_OK, nf + 1, &fok));
-   FPL_TRY(ws_get(st, WS_SEGS, (sizeof(SegRec) + sizeof(float4)) * (np + 1), &segs));
+   FPL_TRY(ws_get(st, WS_SEGS, (sizeof(SegRec) + 3 * sizeof(float4)) * (np + 1), &segs));
    float4 *seg_dir = (float4 *)((SegRec *)segs + (np + 1));
+   float4 *seg_clip = seg_dir + (np + 1);
    FPL_TRY(ws_get(st, WS_SEG_COUNT, sizeof(int64_t) * (np + 1), &cnt));
    FPL_TRY(ws_get(st, WS_SEG_OFF, sizeof(int64_t) * (np + 2), &off));
    FPL_TRY(ws_get(st, WS_FLAGS, 64, &flags));
@@ -1180,7 +1265,7 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
    if (np > 0) {
       hipLaunchKernelGGL(k_segments, dim3(grid_for(np, B)), dim3(B), 0, st, pr, fbs->points,
                          fbs->fiber_ptr, d_bd, (const uint8_t *)fok, (SegRec *)segs,
-                         seg_dir, (int64_t *)cnt);
+                         seg_dir, seg_clip, (int64_t *)cnt);
       // exclusive scan over np + 1 entries (the last one is 0) gives the total
       FPL_HIP(hipMemsetAsync((int64_t *)cnt + np, 0, sizeof(int64_t), st));
       size_t tmp_bytes = 0;
@@ -1232,12 +1317,12 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
       KernelTimer kt("voxelize", st);
       if (lsq_lds)
          hipLaunchKernelGGL((k_voxelize<uint16_t, true>), dim3((unsigned)n_tiles), dim3(WG), 0,
-                            st, pr, (const SegRec *)segs, seg_dir, (const uint32_t *)vb,
+                            st, pr, (const SegRec *)segs, seg_dir, seg_clip, (const uint32_t *)vb,
                             (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
                             d_flags);
       else
          hipLaunchKernelGGL((k_voxelize<uint16_t, false>), dim3((unsigned)n_tiles), dim3(WG), 0,
-                            st, pr, (const SegRec *)segs, seg_dir, (const uint32_t *)vb,
+                            st, pr, (const SegRec *)segs, seg_dir, seg_clip, (const uint32_t *)vb,
                             (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
                             d_flags);
       kt.stop();
@@ -1254,12 +1339,12 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
       KernelTimer kt2("voxelize", st);
       if (lsq_lds)
          hipLaunchKernelGGL((k_voxelize<int32_t, true>), dim3((unsigned)n_tiles), dim3(WG), 0, st,
-                            pr, (const SegRec *)segs, seg_dir, (const uint32_t *)vb,
+                            pr, (const SegRec *)segs, seg_dir, seg_clip, (const uint32_t *)vb,
                             (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
                             d_flags);
       else
          hipLaunchKernelGGL((k_voxelize<int32_t, false>), dim3((unsigned)n_tiles), dim3(WG), 0,
-                            st, pr, (const SegRec *)segs, seg_dir, (const uint32_t *)vb,
+                            st, pr, (const SegRec *)segs, seg_dir, seg_clip, (const uint32_t *)vb,
                             (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
                             d_flags);
       kt2.stop();

@@ -214,3 +214,26 @@ def test_generate_device_tensors(cube_bundles):
     ol, ov, _, _ = oracle.generate(cube_bundles, (24, 24, 24), np.array([-60.0] * 3), 5.0)
     assert np.array_equal(label.cpu().numpy(), ol)
     assert torch.equal(vector.cpu(), torch.from_numpy(ov))
+
+
+@pytest.mark.parametrize('seed', range(6))
+def test_generate_bent_fibers_slab_clip(seed):
+    """One-bundle scenes (the voxelizer's free mode, where each segment's
+    candidates are clipped to its slab between its neighbours): fibers with
+    gentle to sharp bends, constant or varying radii, long and very short
+    segments; bit-exact vs the oracle."""
+    _say(f'generate: bent fibers, slab clip, seed {seed}')
+    rng = np.random.default_rng(100 + seed)
+    fibers = []
+    for _ in range(12):
+        n = 40
+        th = np.cumsum(rng.normal(scale=[0.1, 0.3, 0.8, 0.05, 0.5, 1.2][seed], size=n))
+        ph = np.cumsum(rng.normal(scale=0.2, size=n))
+        step = rng.uniform(0.05, 1.5, size=(n, 1)) if seed >= 3 else rng.uniform(0.3, 1.5)
+        d = np.stack([np.cos(th) * np.cos(ph), np.sin(th) * np.cos(ph), np.sin(ph)], 1) * step
+        pts = rng.uniform(0, 20, 3) + np.cumsum(d, 0)
+        r = rng.uniform(0.5, 2.0, (n, 1)) if seed % 2 else np.full((n, 1), rng.uniform(0.5, 2.0))
+        fibers.append(np.hstack([pts, r]))
+    fbs = objects.FiberBundles([fibers], [[(0.5, 0, 1, 'b'), (0.8, 0.004, 1, 'r'),
+                                           (1.0, -0.004, 1, 'p')]])
+    _compare(fbs, (30, 30, 30), [0.0, 0.0, 0.0], 1.0)
