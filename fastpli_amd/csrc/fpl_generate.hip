@@ -549,18 +549,31 @@ __device__ __forceinline__ int wave_max(int v) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// ascending bitonic sort of one value per lane over the wave
-__device__ __forceinline__ uint32_t wave_sort(uint32_t v, int lane) {
-   for (int k = 2; k <= 64; k <<= 1)
-      for (int j = k >> 1; j > 0; j >>= 1) {
-         const uint32_t o = (uint32_t)__shfl_xor((int)v, j);
-         const bool up = (lane & k) == 0, lower = (lane & j) == 0;
-         v = (lower == up) ? min(v, o) : max(v, o);
-      }
-   return v;
-}
-__device__ __forceinline__ float uniform(float v) {
-   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+// Shrinks [xa, xb) to the x for which some y in [ya, yb) has
+// a_lo <= ux x + uy y + a0 <= a_hi, conservatively: the bounds are widened by
+// 1/64 in a and in x, which covers the float rounding for coordinates below
+// 2^16 (the segment clip's margin covers the rest).
+__device__ __forceinline__ void clip_axis(float ux, float uy, float a0, float alo, float ahi,
+                                          int &xa, int &xb, int ya, int yb) {
+   const float y0 = uy * (float)ya, y1 = uy * (float)(yb - 1);
+   const float lo_a = alo - a0 - fmaxf(y0, y1) - 0x1p-6f; // ux x must reach this
+   const float hi_a = ahi - a0 - fminf(y0, y1) + 0x1p-6f; // and stay below this
+   if (fabsf(ux) < 0x1p-12f) {
+      // ux x barely varies over the columns: keep all or none
+      const float r0 = ux * (float)xa, r1 = ux * (float)(xb - 1);
+      if (!(fmaxf(r0, r1) >= lo_a && fminf(r0, r1) <= hi_a)) xb = xa;
+      return;
+   }
+   float lo = lo_a / ux, hi = hi_a / ux;
+   if (ux < 0.0f) {
+      const float t = lo;
+      lo = hi;
+      hi = t;
+   }
+   lo = fmaxf(lo - 0x1p-6f, -0x1p30f);
+   hi = fminf(hi + 0x1p-6f, 0x1p30f);
+   xa = max(xa, (int)ceilf(lo));
+   xb = min(xb, (int)floorf(hi) + 1);
 }
 
 // A voxel's state is one 64-bit key: the float bits of its stored squared
@@ -579,9 +592,9 @@ __device__ __forceinline__ float uniform(float v) {
 // Free mode (16-bit positions; the warning already raised or the tile's
 // segments all from one bundle, so it cannot be raised here): the low word is
 // 0xFFFF - position.  The minimum key is then the smallest distance with ties
-// going to the latest segment whatever the order the candidates come in, so
-// a chunk's records are visited nearest-to-the-tile-centre first, which
-// finds each voxel's winner with fewer improving candidates.
+// going to the latest segment whatever the order the candidates come in, and
+// candidates that can never be that minimum may be left out: each segment's
+// candidates are clipped to its slab (segment_clip).
 constexpr unsigned long long KEY_EMPTY = (0x7F800000ull << 32) | 0x80000000ull; // +inf, label 0
 constexpr int32_t LABEL_LATER = 0x7FFFFFFF; // label left to the final pass
 constexpr uint32_t LABEL15_LATER = 0x7FFF;  // the same in the packed low word
@@ -809,7 +822,6 @@ __global__ void __launch_bounds__(WG)
       if (end > beg) single = segs[list[beg]].bundle == segs[list[end - 1]].bundle;
       free_mode = uniform((int)(ov || single)) != 0;
    }
-   const float tcx = (float)(xt + TX / 2), tcy = (float)(yt + TY / 2), tcz = (float)(zt + TZ / 2);
    int q_head = 0, q_tail = 0;
    wave_sync();
 
@@ -836,22 +848,9 @@ __global__ void __launch_bounds__(WG)
       // rem = thr - exy_min >= 0; h covers that with margin (the hardware
       // square root is within 1 ulp), and the range ends are computed in
       // double, exact for any float box and voxel index.
-      // free mode: the lanes take the records nearest to the tile centre first
-      int myrec = lane;
-      if (free_mode) {
-         uint32_t k = 0xFFFFFFFFu;
-         if (lane < n) {
-            const SegRec &R = chunk[lane].r;
-            const float mx = 0.5f * (float)(R.p[0] + R.q[0]) - tcx;
-            const float my = 0.5f * (float)(R.p[1] + R.q[1]) - tcy;
-            const float mz = 0.5f * (float)(R.p[2] + R.q[2]) - tcz;
-            k = (__float_as_uint(mx * mx + my * my + mz * mz) & ~63u) | (uint32_t)lane;
-         }
-         myrec = (int)(wave_sort(k, lane) & 63u);
-      }
       int nl = 0, la = 0;
       if (lane < n) {
-         const SegRec &R = chunk[myrec].r;
+         const SegRec &R = chunk[lane].r;
          const int z0 = max(R.lo[2], zt), z1 = min(R.hi[2], zt + TZ);
          const int cx0 = max(R.lo[0], xt), cx1 = min(R.hi[0], xt + TX);
          const int cy0 = max(R.lo[1], yt), cy1 = min(R.hi[1], yt + TY);
@@ -907,7 +906,7 @@ __global__ void __launch_bounds__(WG)
             rcur = s0;
          }
          const int ri = __shfl(rec_info, c);
-         const int rc = __shfl(myrec, c); // the record in the chunk
+         const int rc = c; // the record in the chunk
          int cnt = 0;
          uint32_t geo = 0, magic = 0;
          if (p < n_pairs) {
@@ -934,28 +933,13 @@ __global__ void __launch_bounds__(WG)
                int ya = max(cy0, (int)ceil((double)R.bl[1] - (double)hy));
                int yb = min(cy1, (int)floor((double)R.bh[1] + (double)hy) + 1);
                if (free_mode && xa < xb && ya < yb) {
-                  // the columns of the segment's slab a in [a_lo, a_hi] (above);
-                  // a is linear in x and y, so a column (row) keeps a voxel iff
-                  // the range of a over its end voxels meets the slab
+                  // the columns and rows whose voxels can lie in the segment's slab
+                  // a in [a_lo, a_hi] (above); a = u.x x + u.y y + a0 is linear
                   const uint32_t gi = chunk_gidx[rc];
                   const float4 c0 = seg_clip[2 * gi], c1 = seg_clip[2 * gi + 1];
-                  const float alo = c0.w - 0x1p-8f, ahi = c1.x + 0x1p-8f;
                   const float a0 = (zf - c1.w) * c0.z - c1.y * c0.x - c1.z * c0.y;
-                  auto meets = [&](float x0, float x1, float y0, float y1) {
-                     const float p00 = fmaf(c0.x, x0, fmaf(c0.y, y0, a0));
-                     const float p11 = fmaf(c0.x, x1, fmaf(c0.y, y1, a0));
-                     const float p01 = fmaf(c0.x, x0, fmaf(c0.y, y1, a0));
-                     const float p10 = fmaf(c0.x, x1, fmaf(c0.y, y0, a0));
-                     const float lo = fminf(fminf(p00, p11), fminf(p01, p10));
-                     const float hi = fmaxf(fmaxf(p00, p11), fmaxf(p01, p10));
-                     return hi >= alo && lo <= ahi;
-                  };
-                  const float fya = (float)ya, fyb = (float)(yb - 1);
-                  const float fxa = (float)xa, fxb = (float)(xb - 1);
-                  while (xa < xb && !meets((float)xa, (float)xa, fya, fyb)) xa++;
-                  while (xa < xb && !meets((float)(xb - 1), (float)(xb - 1), fya, fyb)) xb--;
-                  while (ya < yb && !meets(fxa, fxb, (float)ya, (float)ya)) ya++;
-                  while (ya < yb && !meets(fxa, fxb, (float)(yb - 1), (float)(yb - 1))) yb--;
+                  clip_axis(c0.x, c0.y, a0, c0.w, c1.x, xa, xb, ya, yb);
+                  clip_axis(c0.y, c0.x, a0, c0.w, c1.x, ya, yb, xa, xb);
                }
                if (xa < xb && ya < yb) {
                   const int ncy = yb - ya;
