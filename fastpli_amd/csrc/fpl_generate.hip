@@ -638,9 +638,10 @@ template <typename SegT>
 __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chunk,
                                           const uint32_t *chunk_idx, int n, int q_head,
                                           int lane, int xt, int yt, int zw, int max_layer,
-                                          const double *lsq, uint32_t *flags, bool &ov,
-                                          bool free_mode) {
+                                          const double *lsq, const uint8_t *lmodel,
+                                          uint32_t *flags, bool &ov, bool free_mode) {
    constexpr bool packed = sizeof(SegT) == 2;
+   uint32_t free_lo = 0; // free mode: the key's low word of an accepted candidate
    const bool valid = lane < n;
    const uint32_t e = valid ? W.queue[(q_head + lane) & (QN - 1)] : 0u;
    const int c = (int)(e >> 9), slot = (int)(e & (SLOTS - 1));
@@ -668,17 +669,27 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
       const double rr = rp * (1 - cp.t) + rq * cp.t;
       const float r_sq = (float)(rr * rr);
       acc = !(dist_sq > f_last * r_sq);
-      // the label is needed here only while the overlap warning is undecided;
-      // otherwise the final pass computes it for the winning segment alone
-      new_label = LABEL_LATER;
-      if (acc && !ov && !free_mode)
-         new_label = layer_label(dist_sq, r_sq, lsq, R.layer_off, R.bundle, max_layer);
+      if (packed && free_mode) {
+         // free mode keeps the winner's label and layer model in its key, so
+         // the final pass needs no record of it: (0x10000 - position - 1) << 16
+         // | model << 14 | label (model 0 'b', 1 'p', 2 'r'; labels < 2^14)
+         if (acc) {
+            const int32_t lab = layer_label(dist_sq, r_sq, lsq, R.layer_off, R.bundle, max_layer);
+            const uint8_t m = lmodel[R.layer_off + (lab - 1 - R.bundle * max_layer)];
+            const uint32_t m2 = m == 'p' ? 1u : (m == 'r' ? 2u : 0u);
+            free_lo = (0x10000u - chunk_idx[c]) << 16 | m2 << 14 | (uint32_t)lab;
+         }
+      } else {
+         // the label is needed here only while the overlap warning is
+         // undecided; otherwise the final pass computes it for the winner alone
+         new_label = LABEL_LATER;
+         if (acc && !ov)
+            new_label = layer_label(dist_sq, r_sq, lsq, R.layer_off, R.bundle, max_layer);
+      }
    }
    if (packed && free_mode) {
-      // order-free minimum; chunk_idx holds the list position + 1
-      if (acc)
-         atomicMin(&W.key[slot], ((unsigned long long)__float_as_uint(dist_sq) << 32) |
-                                     (unsigned long long)(0x10000u - chunk_idx[c]));
+      // order-free minimum: smallest distance, then latest list position
+      if (acc) atomicMin(&W.key[slot], ((unsigned long long)__float_as_uint(dist_sq) << 32) | free_lo);
       wave_sync();
       return;
    }
@@ -1010,7 +1021,7 @@ __global__ void __launch_bounds__(WG)
                if (q_tail - q_head >= 64) {
                   wave_sync();
                   run_batch(W, chunk, chunk_idx, 64, q_head, lane, xt, yt, zw, max_layer, lsq,
-                            flags, ov, free_mode);
+                            lmodel, flags, ov, free_mode);
                   q_head += 64;
                }
             }
@@ -1021,8 +1032,8 @@ __global__ void __launch_bounds__(WG)
       wave_sync();
       while (q_tail > q_head) {
          const int nb = min(64, q_tail - q_head);
-         run_batch(W, chunk, chunk_idx, nb, q_head, lane, xt, yt, zw, max_layer, lsq, flags,
-                   ov, free_mode);
+         run_batch(W, chunk, chunk_idx, nb, q_head, lane, xt, yt, zw, max_layer, lsq, lmodel,
+                   flags, ov, free_mode);
          q_head += nb;
       }
       if (!ov)
@@ -1030,16 +1041,64 @@ __global__ void __launch_bounds__(WG)
                             0)) != 0;
    }
 
-   // Results, eight columns of eight voxels per pass.  The winners' records
-   // are fetched ahead: the global segment index (a load from the tile list
-   // for 16-bit positions) two passes ahead, the record one pass ahead.
    wave_sync();
+   if (short_seg && free_mode) {
+      // Results of the free mode, eight columns of eight voxels per pass: the
+      // label and layer model come from the key; a 'p' voxel reads its
+      // winner's direction (list entry, then table), an 'r' voxel its points
+      int32_t sgs[8];
+      uint32_t los[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+         const unsigned long long key = W.key[(lane & 7) * 64 + j * 8 + (lane >> 3)];
+         const bool hit = (uint32_t)(key >> 32) != 0x7F800000u;
+         los[j] = hit ? (uint32_t)key : 0u;
+         sgs[j] = -1;
+         if (!only_label && ((los[j] >> 14) & 3u) != 0u)
+            sgs[j] = (int32_t)list[beg + (0x10000u - (los[j] >> 16)) - 1u];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+         const int col = j * 8 + (lane >> 3), zz = lane & 7;
+         const int vx = xt + (col >> 3), vy = yt + (col & 7), vz = zw + 4 * zz;
+         if (vx < pr.nx && vy < pr.ny && vz < pr.nz) {
+            const int64_t ind = ((int64_t)vx * pr.ny + vy) * pr.nz + vz;
+            const uint32_t lo = los[j];
+            label[ind] = (int32_t)(lo & 0x3FFFu);
+            if (!only_label) {
+               float vo[3] = {0.0f, 0.0f, 0.0f};
+               const uint32_t m2 = (lo >> 14) & 3u;
+               if (m2 == 1u) {
+                  const float4 d = seg_dir[sgs[j]];
+                  vo[0] = d.x;
+                  vo[1] = d.y;
+                  vo[2] = d.z;
+               } else if (m2 == 2u) {
+                  const SegRec &R = segs[sgs[j]];
+                  const double pt[3] = {vx + 0.5, vy + 0.5, vz + 0.5};
+                  const double pp[3] = {R.p[0], R.p[1], R.p[2]};
+                  const double qq[3] = {R.q[0], R.q[1], R.q[2]};
+                  const SegExact E = seg_exact(pp, qq);
+                  const Closest cp = closest(pt, E);
+                  unit_to_float(pt[0] - cp.c[0], pt[1] - cp.c[1], pt[2] - cp.c[2], vo);
+               }
+               vector[3 * ind + 0] = vo[0];
+               vector[3 * ind + 1] = vo[1];
+               vector[3 * ind + 2] = vo[2];
+            }
+         }
+      }
+      return;
+   }
+   // Results of the ordered mode, eight columns of eight voxels per pass.  The
+   // winners' records are fetched ahead: the global segment index (a load from
+   // the tile list for 16-bit positions) two passes ahead, the record one pass
+   // ahead.
    auto winner = [&](int j) -> int32_t {
       const int slot = (lane & 7) * 64 + j * 8 + (lane >> 3);
       if constexpr (short_seg) {
          const unsigned long long key = W.key[slot];
-         uint32_t w = (uint32_t)key & 0xFFFFu; // list position + 1 (0: none)
-         if (free_mode) w = (uint32_t)(key >> 32) == 0x7F800000u ? 0u : 0x10000u - w;
+         const uint32_t w = (uint32_t)key & 0xFFFFu; // list position + 1 (0: none)
          return w ? (int32_t)list[beg + w - 1u] : -1;
       } else {
          return (int32_t)W.seg[slot];
@@ -1060,8 +1119,7 @@ __global__ void __launch_bounds__(WG)
          int32_t lab = (int32_t)(uint32_t)(key & 0x7FFFFFFFull);
          if (short_seg) {
             const uint32_t l15 = (uint32_t)(key >> 16) & 0x7FFFu;
-            lab = free_mode ? (sg >= 0 ? LABEL_LATER : 0)
-                            : (l15 == LABEL15_LATER ? LABEL_LATER : (int32_t)l15);
+            lab = l15 == LABEL15_LATER ? LABEL_LATER : (int32_t)l15;
          }
          const double pt[3] = {vx + 0.5, vy + 0.5, vz + 0.5};
          if (sg >= 0 && lab == LABEL_LATER) {
@@ -1294,8 +1352,9 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
                          (uint32_t *)rng);
    }
    const bool lsq_lds = pr.n_lsq <= LSQ_LDS;
-   // the 16-bit build packs a winner's label into 15 bits of its key
-   const bool short_labels = n_props <= (int64_t)LABEL15_LATER;
+   // the 16-bit build packs a winner's label into 14 bits of its key (labels
+   // reach n_props when a float ratio rounds past the last layer)
+   const bool short_labels = n_props <= 0x3FFF;
    bool rerun = !short_labels;
    if (short_labels) {
       KernelTimer kt("voxelize", st);
