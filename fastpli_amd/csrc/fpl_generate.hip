@@ -881,6 +881,23 @@ __global__ void __launch_bounds__(WG)
                const int lb = (zb - zw + 3) >> 2;
                la = (za - zw + 3) >> 2;
                nl = max(lb - la, 0);
+               if (free_mode && nl > 0) {
+                  // no voxel of the record's box in this tile lies in the
+                  // segment's slab (a is linear: its extremes are at corners)
+                  const uint32_t gi = chunk_gidx[lane];
+                  const float4 c0 = seg_clip[2 * gi], c1 = seg_clip[2 * gi + 1];
+                  float amin = __builtin_huge_valf(), amax = -__builtin_huge_valf();
+#pragma unroll
+                  for (int k = 0; k < 8; k++) {
+                     const float x = (float)((k & 1) ? cx1 - 1 : cx0) - c1.y;
+                     const float y = (float)((k & 2) ? cy1 - 1 : cy0) - c1.z;
+                     const float z = (float)((k & 4) ? zb - 1 : za) - c1.w;
+                     const float a = fmaf(x, c0.x, fmaf(y, c0.y, z * c0.z));
+                     amin = fminf(amin, a);
+                     amax = fmaxf(amax, a);
+                  }
+                  if (amax < c0.w - 0x1p-6f || amin > c1.x + 0x1p-6f) nl = 0;
+               }
             }
          }
       }
