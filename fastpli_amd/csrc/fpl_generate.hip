@@ -424,12 +424,22 @@ __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
    if (i < n) p[i] = v;
 }
 
-__global__ void k_ranges(const uint32_t *__restrict__ keys, int64_t n, uint32_t *range) {
+// Each tile's [begin, end) in the sorted list, and the bundles of its first
+// and last segment (the list is in global segment order, bundle-major)
+__global__ void k_ranges(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+                         int64_t n, const SegRec *__restrict__ segs, uint32_t *range,
+                         int2 *tile_bundle) {
    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
    if (p >= n) return;
    const uint32_t t = keys[p];
-   if (p == 0 || keys[p - 1] != t) range[2 * (int64_t)t] = (uint32_t)p;
-   if (p == n - 1 || keys[p + 1] != t) range[2 * (int64_t)t + 1] = (uint32_t)(p + 1);
+   if (p == 0 || keys[p - 1] != t) {
+      range[2 * (int64_t)t] = (uint32_t)p;
+      tile_bundle[t].x = segs[vals[p]].bundle;
+   }
+   if (p == n - 1 || keys[p + 1] != t) {
+      range[2 * (int64_t)t + 1] = (uint32_t)(p + 1);
+      tile_bundle[t].y = segs[vals[p]].bundle;
+   }
 }
 
 struct SegExact { // what the FP64 distance test reads of a segment
@@ -549,6 +559,51 @@ __device__ __forceinline__ int wave_max(int v) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Inclusive prefix sum over the wave's 64 lanes: DPP row shifts within each
+// row of 16 lanes, then the broadcasts of lane 15 to rows 1 and 3 and of lane
+// 31 to rows 2 and 3.  Lanes with no source lane add 0.
+__device__ __forceinline__ int wave_scan_add(int x) {
+   x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false); // row_shr:1
+   x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false); // row_shr:2
+   x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false); // row_shr:4
+   x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false); // row_shr:8
+   x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false); // row_bcast:15
+   x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false); // row_bcast:31
+   return x;
+}
+
+// Inclusive prefix maximum of non-negative values, as wave_scan_add.
+__device__ __forceinline__ int wave_scan_max(int x) {
+   x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));
+   x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));
+   x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));
+   x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));
+   x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));
+   x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));
+   return x;
+}
+
+// Load-balancing search over a 64-lane window [wb, wb + 64) of a run of
+// items (the pairs of a chunk's records, the candidates of a group's pairs):
+// returns the owner of the window's item at this lane, the last owner i (a
+// lane) with items whose first item start[i] (non-decreasing in i) lies at or
+// before that position.  Owners with items that start inside the window mark
+// their start with 1 + i (a byte), a prefix maximum carries the marks
+// forward, and `carry` (1 + the owner of the item before the window, 0 at the
+// first window) covers the positions before the first mark; `carry` is
+// updated for the next window.
+__device__ __forceinline__ int lane_owner(uint8_t *own, int lane, bool has_items, int start,
+                                          int wb, int &carry) {
+   own[lane] = 0;
+   wave_sync();
+   if (has_items && start >= wb && start < wb + 64) own[start - wb] = (uint8_t)(lane + 1);
+   wave_sync();
+   const int o = max(wave_scan_max((int)own[lane]), carry);
+   carry = __builtin_amdgcn_readlane(o, 63);
+   wave_sync(); // the marks are rewritten for the next window
+   return o - 1;
+}
+
 // Shrinks [xa, xb) to the x for which some y in [ya, yb) has
 // a_lo <= ux x + uy y + a0 <= a_hi, conservatively: the bounds are widened by
 // 1/64 in a and in x, which covers the float rounding for coordinates below
@@ -564,7 +619,10 @@ __device__ __forceinline__ void clip_axis(float ux, float uy, float a0, float al
       if (!(fmaxf(r0, r1) >= lo_a && fminf(r0, r1) <= hi_a)) xb = xa;
       return;
    }
-   float lo = lo_a / ux, hi = hi_a / ux;
+   // rcp is within 1 ulp of 1 / ux: the quotients are within 1.5 * 2^-23 of
+   // their exact values, below 2^-6.4 for |x| < 2^16
+   const float inv = __builtin_amdgcn_rcpf(ux);
+   float lo = lo_a * inv, hi = hi_a * inv;
    if (ux < 0.0f) {
       const float t = lo;
       lo = hi;
@@ -620,11 +678,13 @@ template <typename SegT> struct WaveState {
    SegT seg[SLOTS];
    uint32_t queue[QN];  // candidates (chunk index << 9 | slot), ring
    uint2 tab[64];       // per chunk record: candidate prefix | box << 16, division magics
+   uint8_t own[64];     // owner marks of a 64-lane window (lane_owner)
 };
 template <> struct WaveState<uint16_t> {
    unsigned long long key[SLOTS];
    uint32_t queue[QN];
    uint2 tab[64];
+   uint8_t own[64];
 };
 
 struct alignas(16) LdsSeg { // 144-byte stride spreads records over the LDS banks
@@ -636,8 +696,9 @@ struct alignas(16) LdsSeg { // 144-byte stride spreads records over the LDS bank
 // ordered application to the wave's voxel state.
 template <typename SegT>
 __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chunk,
-                                          const uint32_t *chunk_idx, int n, int q_head,
-                                          int lane, int xt, int yt, int zw, int max_layer,
+                                          const uint32_t *chunk_gidx, uint32_t pos1, int n,
+                                          int q_head, int lane, int xt, int yt, int zw,
+                                          int max_layer,
                                           const double *lsq, const uint8_t *lmodel,
                                           uint32_t *flags, bool &ov, bool free_mode) {
    constexpr bool packed = sizeof(SegT) == 2;
@@ -645,6 +706,9 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
    const bool valid = lane < n;
    const uint32_t e = valid ? W.queue[(q_head + lane) & (QN - 1)] : 0u;
    const int c = (int)(e >> 9), slot = (int)(e & (SLOTS - 1));
+   // the winner index a candidate leaves: its list position + 1 (16-bit
+   // build) or its global segment index
+   const uint32_t widx = packed ? pos1 + (uint32_t)c : (valid ? chunk_gidx[c] : 0u);
    bool acc = false;
    float dist_sq = 0.0f;
    int32_t new_label = 0;
@@ -677,7 +741,7 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
             const int32_t lab = layer_label(dist_sq, r_sq, lsq, R.layer_off, R.bundle, max_layer);
             const uint8_t m = lmodel[R.layer_off + (lab - 1 - R.bundle * max_layer)];
             const uint32_t m2 = m == 'p' ? 1u : (m == 'r' ? 2u : 0u);
-            free_lo = (0x10000u - chunk_idx[c]) << 16 | m2 << 14 | (uint32_t)lab;
+            free_lo = (0x10000u - widx) << 16 | m2 << 14 | (uint32_t)lab;
          }
       } else {
          // the label is needed here only while the overlap warning is
@@ -689,7 +753,8 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
    }
    if (packed && free_mode) {
       // order-free minimum: smallest distance, then latest list position
-      if (acc) atomicMin(&W.key[slot], ((unsigned long long)__float_as_uint(dist_sq) << 32) | free_lo);
+      if (acc)
+         atomicMin(&W.key[slot], ((unsigned long long)__float_as_uint(dist_sq) << 32) | free_lo);
       wave_sync();
       return;
    }
@@ -698,7 +763,7 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
    if (packed)
       rest_lo = 0x80000000u |
                 (new_label == LABEL_LATER ? LABEL15_LATER : (uint32_t)new_label) << 16 |
-                (valid ? chunk_idx[c] : 0u);
+                widx;
    auto label_of = [&](unsigned long long key) -> int {
       if (packed) return (int)((uint32_t)(key >> 16) & 0x7FFFu);
       return (int)(uint32_t)(key & 0x7FFFFFFFull);
@@ -733,7 +798,7 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
       wave_sync();
       if (acc && W.key[slot] == k) {
          W.key[slot] = rest;
-         if constexpr (!packed) W.seg[slot] = (SegT)chunk_idx[c];
+         if constexpr (!packed) W.seg[slot] = (SegT)widx;
       }
    } else {
       uint32_t overlap = 0;
@@ -744,7 +809,7 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
                const int ol = label_of(cur);
                if (ol != 0 && abs(ol - new_label) >= max_layer) overlap = 1;
                W.key[slot] = rest;
-               if constexpr (!packed) W.seg[slot] = (SegT)chunk_idx[c];
+               if constexpr (!packed) W.seg[slot] = (SegT)widx;
             }
          }
          wave_sync();
@@ -771,17 +836,17 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
 // LSQ_IN_LDS: the layer tables fit the LDS copy; a compile-time choice keeps
 // the layer searches plain LDS reads
 template <typename SegT, bool LSQ_IN_LDS>
-__global__ void __launch_bounds__(WG)
+__global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
     k_voxelize(Params pr, const SegRec *__restrict__ segs, const float4 *__restrict__ seg_dir,
                const float4 *__restrict__ seg_clip,
                const uint32_t *__restrict__ list,
-               const uint32_t *__restrict__ range, const BundleDev *__restrict__ bd,
+               const uint32_t *__restrict__ range, const int2 *__restrict__ tile_bundle,
+               const BundleDev *__restrict__ bd,
                const double *__restrict__ layer_sq, const uint8_t *__restrict__ layer_model,
                int only_label, int32_t *__restrict__ label, float *__restrict__ vector,
                uint32_t *flags) {
    __shared__ WaveState<SegT> wstate[WG / 64];
    __shared__ LdsSeg chunk[CHUNK];
-   __shared__ uint32_t chunk_idx[CHUNK];
    __shared__ uint32_t chunk_gidx[CHUNK]; // global segment index of each record
 
    const int n_tiles = pr.ntx * pr.nty * pr.ntz;
@@ -830,7 +895,10 @@ __global__ void __launch_bounds__(WG)
    bool free_mode = false;
    if (short_seg) {
       bool single = true;
-      if (end > beg) single = segs[list[beg]].bundle == segs[list[end - 1]].bundle;
+      if (end > beg) {
+         const int2 tb = tile_bundle[t];
+         single = tb.x == tb.y;
+      }
       free_mode = uniform((int)(ov || single)) != 0;
    }
    int q_head = 0, q_tail = 0;
@@ -838,15 +906,13 @@ __global__ void __launch_bounds__(WG)
 
    for (uint32_t base = beg; base < end; base += CHUNK) {
       const int n = (int)((end - base) < (uint32_t)CHUNK ? (end - base) : (uint32_t)CHUNK);
+      const uint32_t pos1 = base - beg + 1u; // list position + 1 of the chunk's first record
       __syncthreads(); // every wave is done with the previous chunk
       for (int e = threadIdx.x; e < n * 8; e += WG) {
          const int c = e >> 3, piece = e & 7;
          const uint32_t sidx = list[base + c];
          ((int4 *)&chunk[c].r)[piece] = ((const int4 *)&segs[sidx])[piece];
-         if (piece == 0) {
-            chunk_idx[c] = short_seg ? base - beg + (uint32_t)c + 1u : sidx;
-            chunk_gidx[c] = sidx;
-         }
+         if (piece == 0) chunk_gidx[c] = sidx;
       }
       __syncthreads();
 
@@ -858,7 +924,8 @@ __global__ void __launch_bounds__(WG)
       // radius test RN(ez^2 + exy) <= thr has ez^2 <= rem + ulp(thr),
       // rem = thr - exy_min >= 0; h covers that with margin (the hardware
       // square root is within 1 ulp), and the range ends are computed in
-      // double, exact for any float box and voxel index.
+      // double, exact for any float box and voxel index.  (Each wave repeats
+      // the set-up: one lane per record, off the chunk's critical path.)
       int nl = 0, la = 0;
       if (lane < n) {
          const SegRec &R = chunk[lane].r;
@@ -902,15 +969,11 @@ __global__ void __launch_bounds__(WG)
          }
       }
       // (record, layer) pairs in record order: exclusive prefix of the counts
-      int incl = nl;
-      for (int off = 1; off < 64; off <<= 1) {
-         const int v = __shfl_up(incl, off);
-         if (lane >= off) incl += v;
-      }
+      const int incl = wave_scan_add(nl);
       const int PL = incl - nl;
-      const int n_pairs = uniform(__shfl(incl, 63));
+      const int n_pairs = __builtin_amdgcn_readlane(incl, 63);
       const int rec_info = PL << 3 | la;
-      int rcur = 0; // record holding the pair before the group
+      int rcarry = 0; // 1 + the record holding the pair before the group
 
       for (int gb = 0; gb < n_pairs; gb += 64) {
          // Pair set-up, one lane per (record, layer) pair: the columns of the
@@ -923,16 +986,7 @@ __global__ void __launch_bounds__(WG)
          // sqrt(Rx) with a relative margin that also covers the float rounding
          // of ex = RN(bl - x) or RN(x - bh).
          const int p = gb + lane;
-         unsigned long long st = __ballot((nl > 0) & (PL >= gb) & (PL < gb + 64));
-         st = ((unsigned long long)(uint32_t)uniform((int)(uint32_t)(st >> 32)) << 32) |
-              (uint32_t)uniform((int)(uint32_t)st);
-         int c = rcur;
-         while (st) {
-            const int s0 = __builtin_ctzll(st);
-            st &= st - 1;
-            if (p >= __builtin_amdgcn_readlane(PL, s0)) c = s0;
-            rcur = s0;
-         }
+         const int c = lane_owner(W.own, lane, nl > 0, PL, gb, rcarry);
          const int ri = __shfl(rec_info, c);
          const int rc = c; // the record in the chunk
          int cnt = 0;
@@ -974,37 +1028,27 @@ __global__ void __launch_bounds__(WG)
                   cnt = (xb - xa) * ncy;
                   geo = (uint32_t)(xa - xt) | (uint32_t)(ya - yt) << 3 | (uint32_t)l << 6 |
                         (uint32_t)(ncy - 1) << 9;
-                  // floor(i / n) = (i * ceil(2^15 / n)) >> 15 for i < 512, n <= 8
-                  magic = (uint32_t)(32767 / ncy + 1) | (uint32_t)rc << 16;
+                  // floor(i / n) = (i * ceil(2^15 / n)) >> 15 for i < 512, n <= 8;
+                  // 2^15 rcp(n) is within 2^15 / n * 2^-23 < 0.002 of 2^15 / n, which
+                  // is an integer (n = 1, 2, 4, 8) or 0.14 or more from one
+                  const uint32_t cdiv = (uint32_t)ceilf(
+                      32768.0f * __builtin_amdgcn_rcpf((float)ncy));
+                  magic = cdiv | (uint32_t)rc << 16;
                }
             }
          }
          // candidates of the group in pair order
-         int ci = cnt;
-         for (int off = 1; off < 64; off <<= 1) {
-            const int v = __shfl_up(ci, off);
-            if (lane >= off) ci += v;
-         }
+         const int ci = wave_scan_add(cnt);
          const int P = ci - cnt;
-         const int total = uniform(__shfl(ci, 63));
+         const int total = __builtin_amdgcn_readlane(ci, 63);
          W.tab[lane] = make_uint2((uint32_t)P | geo << 13, magic);
          wave_sync();
 
          // Stage A test, 64 candidates per window
-         int cur = 0; // pair holding the entry before the window
+         int kcarry = 0; // 1 + the pair holding the entry before the window
          for (int wb = 0; wb < total; wb += 64) {
             const int e = wb + lane;
-            // pairs whose candidates start inside the window, in order
-            unsigned long long sp = __ballot((cnt > 0) & (P >= wb) & (P < wb + 64));
-            sp = ((unsigned long long)(uint32_t)uniform((int)(uint32_t)(sp >> 32)) << 32) |
-                 (uint32_t)uniform((int)(uint32_t)sp);
-            int k = cur;
-            while (sp) {
-               const int s0 = __builtin_ctzll(sp);
-               sp &= sp - 1;
-               if (e >= __builtin_amdgcn_readlane(P, s0)) k = s0;
-               cur = s0;
-            }
+            const int k = lane_owner(W.own, lane, cnt > 0, P, wb, kcarry);
             bool pass = false;
             uint32_t entry = 0;
             if (e < total) {
@@ -1037,8 +1081,8 @@ __global__ void __launch_bounds__(WG)
                q_tail += __popcll(m);
                if (q_tail - q_head >= 64) {
                   wave_sync();
-                  run_batch(W, chunk, chunk_idx, 64, q_head, lane, xt, yt, zw, max_layer, lsq,
-                            lmodel, flags, ov, free_mode);
+                  run_batch(W, chunk, chunk_gidx, pos1, 64, q_head, lane, xt, yt, zw,
+                            max_layer, lsq, lmodel, flags, ov, free_mode);
                   q_head += 64;
                }
             }
@@ -1049,7 +1093,7 @@ __global__ void __launch_bounds__(WG)
       wave_sync();
       while (q_tail > q_head) {
          const int nb = min(64, q_tail - q_head);
-         run_batch(W, chunk, chunk_idx, nb, q_head, lane, xt, yt, zw, max_layer, lsq, lmodel,
+         run_batch(W, chunk, chunk_gidx, pos1, nb, q_head, lane, xt, yt, zw, max_layer, lsq, lmodel,
                    flags, ov, free_mode);
          q_head += nb;
       }
@@ -1347,7 +1391,8 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
    FPL_TRY(ws_get(st, WS_VALS_A, sizeof(uint32_t) * (total + 1), &va));
    FPL_TRY(ws_get(st, WS_KEYS_B, sizeof(uint32_t) * (total + 1), &kb));
    FPL_TRY(ws_get(st, WS_VALS_B, sizeof(uint32_t) * (total + 1), &vb));
-   FPL_TRY(ws_get(st, WS_TILE_RANGE, sizeof(uint32_t) * 2 * n_tiles, &rng));
+   FPL_TRY(ws_get(st, WS_TILE_RANGE, sizeof(uint32_t) * 4 * n_tiles, &rng));
+   int2 *tile_bundle = (int2 *)((uint32_t *)rng + 2 * n_tiles); // set with the ranges
    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(2 * n_tiles, B)), dim3(B), 0, st,
                       (uint32_t *)rng, 2 * n_tiles, 0u);
    if (total > 0) {
@@ -1365,8 +1410,9 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
       FPL_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, (uint32_t *)ka, (uint32_t *)kb,
                                                  (uint32_t *)va, (uint32_t *)vb, (int)total, 0,
                                                  nbits, st));
-      hipLaunchKernelGGL(k_ranges, dim3(grid_for(total, B)), dim3(B), 0, st, (uint32_t *)kb, total,
-                         (uint32_t *)rng);
+      hipLaunchKernelGGL(k_ranges, dim3(grid_for(total, B)), dim3(B), 0, st, (uint32_t *)kb,
+                         (const uint32_t *)vb, total, (const SegRec *)segs, (uint32_t *)rng,
+                         tile_bundle);
    }
    const bool lsq_lds = pr.n_lsq <= LSQ_LDS;
    // the 16-bit build packs a winner's label into 14 bits of its key (labels
@@ -1378,13 +1424,13 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
       if (lsq_lds)
          hipLaunchKernelGGL((k_voxelize<uint16_t, true>), dim3((unsigned)n_tiles), dim3(WG), 0,
                             st, pr, (const SegRec *)segs, seg_dir, seg_clip, (const uint32_t *)vb,
-                            (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
-                            d_flags);
+                            (const uint32_t *)rng, tile_bundle, d_bd, d_lsq, d_lm, only_label,
+                            label, vector, d_flags);
       else
          hipLaunchKernelGGL((k_voxelize<uint16_t, false>), dim3((unsigned)n_tiles), dim3(WG), 0,
                             st, pr, (const SegRec *)segs, seg_dir, seg_clip, (const uint32_t *)vb,
-                            (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
-                            d_flags);
+                            (const uint32_t *)rng, tile_bundle, d_bd, d_lsq, d_lm, only_label,
+                            label, vector, d_flags);
       kt.stop();
       FPL_HIP(hipGetLastError());
       FPL_HIP(hipMemcpyAsync(stage, d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -1400,13 +1446,13 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
       if (lsq_lds)
          hipLaunchKernelGGL((k_voxelize<int32_t, true>), dim3((unsigned)n_tiles), dim3(WG), 0, st,
                             pr, (const SegRec *)segs, seg_dir, seg_clip, (const uint32_t *)vb,
-                            (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
-                            d_flags);
+                            (const uint32_t *)rng, tile_bundle, d_bd, d_lsq, d_lm, only_label,
+                            label, vector, d_flags);
       else
          hipLaunchKernelGGL((k_voxelize<int32_t, false>), dim3((unsigned)n_tiles), dim3(WG), 0,
                             st, pr, (const SegRec *)segs, seg_dir, seg_clip, (const uint32_t *)vb,
-                            (const uint32_t *)rng, d_bd, d_lsq, d_lm, only_label, label, vector,
-                            d_flags);
+                            (const uint32_t *)rng, tile_bundle, d_bd, d_lsq, d_lm, only_label,
+                            label, vector, d_flags);
       kt2.stop();
       FPL_HIP(hipGetLastError());
       FPL_HIP(hipMemcpyAsync(stage, d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
