@@ -279,28 +279,32 @@ __device__ float3 get_vec_interp(const Field<Idx, ST> &F, double px, double py, 
    const float yd = dy ? (float)((py - (double)y0) - 0.5) : 0.0f;
    const float zd = dz ? (float)((pz - (double)z0) - 0.5) : 0.0f;
    // corners with another label are zeroed (simulator.cpp:533-540)
-   auto corner = [&](int32_t x, int32_t y, int32_t z) {
-      const Vox r = F.at(x, y, z);
+   auto corner = [&](const Vox &r) {
       if (r.lab != lab) return make_float3(0.0f, 0.0f, 0.0f);
       return r.v;
    };
    auto ip = [&](const float3 &a, const float3 &b, float t) {
       return slerp ? slerp_orient(a, b, t) : lerp_orient(a, b, t);
    };
+   // every corner read at once (an axis without spread repeats its corners)
+   const Vox r000 = F.at(x0, y0, z0), r100 = F.at(x1, y0, z0);
+   const Vox r010 = F.at(x0, y1, z0), r110 = F.at(x1, y1, z0);
+   const Vox r001 = F.at(x0, y0, z1), r101 = F.at(x1, y0, z1);
+   const Vox r011 = F.at(x0, y1, z1), r111 = F.at(x1, y1, z1);
    // level x
-   float3 c00 = corner(x0, y0, z0);
-   if (dx) c00 = ip(c00, corner(x1, y0, z0), xd);
+   float3 c00 = corner(r000);
+   if (dx) c00 = ip(c00, corner(r100), xd);
    float3 c01 = c00, c10 = c00, c11 = c00;
    if (dz) {
-      c01 = corner(x0, y0, z1);
-      if (dx) c01 = ip(c01, corner(x1, y0, z1), xd);
+      c01 = corner(r001);
+      if (dx) c01 = ip(c01, corner(r101), xd);
    }
    if (dy) {
-      c10 = corner(x0, y1, z0);
-      if (dx) c10 = ip(c10, corner(x1, y1, z0), xd);
+      c10 = corner(r010);
+      if (dx) c10 = ip(c10, corner(r110), xd);
       if (dz) {
-         c11 = corner(x0, y1, z1);
-         if (dx) c11 = ip(c11, corner(x1, y1, z1), xd);
+         c11 = corner(r011);
+         if (dx) c11 = ip(c11, corner(r111), xd);
       }
    }
    // level y
