@@ -335,6 +335,45 @@ __device__ void segment_clip(const Params &pr, const double *P, int64_t i, bool 
    clip[1] = make_float4((float)hi, (float)(p[0] - 0.5), (float)(p[1] - 0.5), (float)(p[2] - 0.5));
 }
 
+// A record's candidate region in the voxel tile at (xt, yt, zt): its integer
+// box within the tile, and the z range [za, zb) narrowed by the tile's nearest
+// in-box columns (empty: za >= zb).  The in-box columns nearest to the
+// segment's box bound every voxel's float ex and ey from below (float
+// rounding is monotone), so a voxel that passes the float radius test
+// RN(ez^2 + exy) <= thr has ez^2 <= rem + ulp(thr), rem = thr - exy_min >= 0;
+// h covers that with margin (the hardware square root is within 1 ulp), and
+// the range ends are computed in double, exact for any float box and voxel
+// index.  A tile where the range is empty holds no candidate of the segment:
+// k_segments and k_emit leave it out of the tile lists.
+struct TileBox {
+   int cx0, cx1, cy0, cy1, za, zb;
+};
+__device__ __forceinline__ TileBox tile_box(const SegRec &R, int xt, int yt, int zt) {
+   TileBox b;
+   const int z0 = max(R.lo[2], zt), z1 = min(R.hi[2], zt + TZ);
+   b.cx0 = max(R.lo[0], xt);
+   b.cx1 = min(R.hi[0], xt + TX);
+   b.cy0 = max(R.lo[1], yt);
+   b.cy1 = min(R.hi[1], yt + TY);
+   b.za = 0;
+   b.zb = 0;
+   if (z0 < z1 && b.cx0 < b.cx1 && b.cy0 < b.cy1) {
+      const float thr = R.thr;
+      const float exm =
+          fmaxf(fmaxf(R.bl[0] - (float)(b.cx1 - 1), (float)b.cx0 - R.bh[0]), 0.0f);
+      const float eym =
+          fmaxf(fmaxf(R.bl[1] - (float)(b.cy1 - 1), (float)b.cy0 - R.bh[1]), 0.0f);
+      const float rem = thr - fmaf(eym, eym, exm * exm);
+      if (rem >= 0.0f) {
+         const float h = fmaf(__builtin_amdgcn_sqrtf(fmaf(thr, 0x1p-20f, rem)),
+                              1.0f + 0x1p-10f, 0x1p-10f);
+         b.za = max(z0, (int)ceil((double)R.bl[2] - (double)h));
+         b.zb = min(z1, (int)floor((double)R.bh[2] + (double)h) + 1);
+      }
+   }
+   return b;
+}
+
 __global__ void k_segments(Params pr, const double *__restrict__ P,
                            const int64_t *__restrict__ fiber_ptr,
                            const BundleDev *__restrict__ bd, const uint8_t *__restrict__ fiber_ok,
@@ -390,16 +429,33 @@ __global__ void k_segments(Params pr, const double *__restrict__ P,
          seg_dir[i] = make_float4(d[0], d[1], d[2], 0.0f);
          segment_clip(pr, P, i, i > fiber_ptr[f], i + 2 < fiber_ptr[f + 1], R.p, R.q, rp, rq,
                       max_r, seg_clip + 2 * i);
-         const int64_t tx0 = ilo[0] / TX, tx1 = (ihi[0] - 1) / TX;
-         const int64_t ty0 = ilo[1] / TY, ty1 = (ihi[1] - 1) / TY;
-         const int64_t tz0 = ilo[2] / TZ, tz1 = (ihi[2] - 1) / TZ;
-         count = (tx1 - tx0 + 1) * (ty1 - ty0 + 1) * (tz1 - tz0 + 1);
+         // the tiles of the box that hold candidates of the segment; for boxes
+         // of up to 31 tiles their mask (bit k: the k-th tile in k_emit's loop
+         // order) goes into the direction entry's w for k_emit, else bit 31
+         // asks k_emit to test the tiles again
+         const int tx0 = R.lo[0] / TX, tx1 = (R.hi[0] - 1) / TX;
+         const int ty0 = R.lo[1] / TY, ty1 = (R.hi[1] - 1) / TY;
+         const int tz0 = R.lo[2] / TZ, tz1 = (R.hi[2] - 1) / TZ;
+         uint32_t mask = 0;
+         int k = 0;
+         for (int tx = tx0; tx <= tx1; tx++)
+            for (int ty = ty0; ty <= ty1; ty++)
+               for (int tz = tz0; tz <= tz1; tz++, k++) {
+                  const TileBox b = tile_box(R, tx * TX, ty * TY, tz * TZ);
+                  if (b.za < b.zb) {
+                     count++;
+                     if (k < 31) mask |= 1u << k;
+                  }
+               }
+         if (k > 31) mask = 0x80000000u;
+         seg_dir[i].w = __uint_as_float(mask);
       }
    }
    seg_count[i] = count;
 }
 
 __global__ void k_emit(Params pr, const SegRec *__restrict__ segs,
+                       const float4 *__restrict__ seg_dir,
                        const int64_t *__restrict__ seg_count, const int64_t *__restrict__ seg_off,
                        uint32_t *keys, uint32_t *vals) {
    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -410,12 +466,21 @@ __global__ void k_emit(Params pr, const SegRec *__restrict__ segs,
    const int tx0 = R.lo[0] / TX, tx1 = (R.hi[0] - 1) / TX;
    const int ty0 = R.lo[1] / TY, ty1 = (R.hi[1] - 1) / TY;
    const int tz0 = R.lo[2] / TZ, tz1 = (R.hi[2] - 1) / TZ;
+   const uint32_t mask = __float_as_uint(seg_dir[i].w); // k_segments' tile mask
+   int k = 0;
    for (int tx = tx0; tx <= tx1; tx++)
       for (int ty = ty0; ty <= ty1; ty++)
-         for (int tz = tz0; tz <= tz1; tz++) {
-            keys[o] = (uint32_t)(((int64_t)tx * pr.nty + ty) * pr.ntz + tz);
-            vals[o] = (uint32_t)i;
-            o++;
+         for (int tz = tz0; tz <= tz1; tz++, k++) {
+            bool keep = (mask >> k) & 1u;
+            if (mask & 0x80000000u) { // as k_segments counts
+               const TileBox b = tile_box(R, tx * TX, ty * TY, tz * TZ);
+               keep = b.za < b.zb;
+            }
+            if (keep) {
+               keys[o] = (uint32_t)(((int64_t)tx * pr.nty + ty) * pr.ntz + tz);
+               vals[o] = (uint32_t)i;
+               o++;
+            }
          }
 }
 
@@ -916,56 +981,32 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
       }
       __syncthreads();
 
-      // Stage A set-up, one lane per record of the chunk: the record's integer
-      // box within the tile and its z range narrowed by the nearest columns of
-      // the tile, then restricted to the wave's layers.  The in-box columns
-      // nearest to the segment's box bound every voxel's float ex and ey from
-      // below (float rounding is monotone), so a voxel that passes the float
-      // radius test RN(ez^2 + exy) <= thr has ez^2 <= rem + ulp(thr),
-      // rem = thr - exy_min >= 0; h covers that with margin (the hardware
-      // square root is within 1 ulp), and the range ends are computed in
-      // double, exact for any float box and voxel index.  (Each wave repeats
-      // the set-up: one lane per record, off the chunk's critical path.)
+      // Stage A set-up, one lane per record of the chunk: the record's candidate
+      // region in the tile (tile_box) restricted to the wave's layers.  (Each
+      // wave repeats the set-up: one lane per record, off the chunk's critical
+      // path.)
       int nl = 0, la = 0;
       if (lane < n) {
-         const SegRec &R = chunk[lane].r;
-         const int z0 = max(R.lo[2], zt), z1 = min(R.hi[2], zt + TZ);
-         const int cx0 = max(R.lo[0], xt), cx1 = min(R.hi[0], xt + TX);
-         const int cy0 = max(R.lo[1], yt), cy1 = min(R.hi[1], yt + TY);
-         if (z0 < z1 && cx0 < cx1 && cy0 < cy1) {
-            const float thr = R.thr;
-            const float exm =
-                fmaxf(fmaxf(R.bl[0] - (float)(cx1 - 1), (float)cx0 - R.bh[0]), 0.0f);
-            const float eym =
-                fmaxf(fmaxf(R.bl[1] - (float)(cy1 - 1), (float)cy0 - R.bh[1]), 0.0f);
-            const float rem = thr - fmaf(eym, eym, exm * exm);
-            if (rem >= 0.0f) {
-               const float h = fmaf(__builtin_amdgcn_sqrtf(fmaf(thr, 0x1p-20f, rem)),
-                                    1.0f + 0x1p-10f, 0x1p-10f);
-               const int za = max(z0, (int)ceil((double)R.bl[2] - (double)h));
-               const int zb = min(z1, (int)floor((double)R.bh[2] + (double)h) + 1);
-               // the wave's layers in [za, zb): z = zw + 4 * l, l in [la, lb)
-               const int lb = (zb - zw + 3) >> 2;
-               la = (za - zw + 3) >> 2;
-               nl = max(lb - la, 0);
-               if (free_mode && nl > 0) {
-                  // no voxel of the record's box in this tile lies in the
-                  // segment's slab (a is linear: its extremes are at corners)
-                  const uint32_t gi = chunk_gidx[lane];
-                  const float4 c0 = seg_clip[2 * gi], c1 = seg_clip[2 * gi + 1];
-                  float amin = __builtin_huge_valf(), amax = -__builtin_huge_valf();
+         const TileBox tb = tile_box(chunk[lane].r, xt, yt, zt);
+         // the wave's layers in [za, zb): z = zw + 4 * l, l in [la, la + nl)
+         la = (tb.za - zw + 3) >> 2;
+         nl = max(((tb.zb - zw + 3) >> 2) - la, 0);
+         if (free_mode && nl > 0) {
+            // no voxel of the record's box in this tile lies in the segment's
+            // slab (a is linear: its extremes are at corners)
+            const uint32_t gi = chunk_gidx[lane];
+            const float4 c0 = seg_clip[2 * gi], c1 = seg_clip[2 * gi + 1];
+            float amin = __builtin_huge_valf(), amax = -__builtin_huge_valf();
 #pragma unroll
-                  for (int k = 0; k < 8; k++) {
-                     const float x = (float)((k & 1) ? cx1 - 1 : cx0) - c1.y;
-                     const float y = (float)((k & 2) ? cy1 - 1 : cy0) - c1.z;
-                     const float z = (float)((k & 4) ? zb - 1 : za) - c1.w;
-                     const float a = fmaf(x, c0.x, fmaf(y, c0.y, z * c0.z));
-                     amin = fminf(amin, a);
-                     amax = fmaxf(amax, a);
-                  }
-                  if (amax < c0.w - 0x1p-6f || amin > c1.x + 0x1p-6f) nl = 0;
-               }
+            for (int k = 0; k < 8; k++) {
+               const float x = (float)((k & 1) ? tb.cx1 - 1 : tb.cx0) - c1.y;
+               const float y = (float)((k & 2) ? tb.cy1 - 1 : tb.cy0) - c1.z;
+               const float z = (float)((k & 4) ? tb.zb - 1 : tb.za) - c1.w;
+               const float a = fmaf(x, c0.x, fmaf(y, c0.y, z * c0.z));
+               amin = fminf(amin, a);
+               amax = fmaxf(amax, a);
             }
+            if (amax < c0.w - 0x1p-6f || amin > c1.x + 0x1p-6f) nl = 0;
          }
       }
       // (record, layer) pairs in record order: exclusive prefix of the counts
@@ -1397,8 +1438,8 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
                       (uint32_t *)rng, 2 * n_tiles, 0u);
    if (total > 0) {
       hipLaunchKernelGGL(k_emit, dim3(grid_for(np, B)), dim3(B), 0, st, pr, (const SegRec *)segs,
-                         (const int64_t *)cnt, (const int64_t *)off, (uint32_t *)ka,
-                         (uint32_t *)va);
+                         (const float4 *)seg_dir, (const int64_t *)cnt, (const int64_t *)off,
+                         (uint32_t *)ka, (uint32_t *)va);
       int nbits = 1;
       while (((int64_t)1 << nbits) < n_tiles) nbits++;
       size_t tmp_bytes = 0;
