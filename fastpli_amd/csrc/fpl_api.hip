@@ -16,6 +16,7 @@ struct Workspace {
    size_t cap[WS_NUM_SLOTS] = {};
    void *host = nullptr;
    size_t host_cap = 0;
+   int64_t pair_hint = 0; // generation: the last call's (tile, segment) pair count
 };
 
 std::mutex g_mutex;
@@ -29,6 +30,10 @@ Workspace &ws_for(void *stream) {
 }
 
 } // namespace
+
+int64_t pair_hint(void *stream) { return ws_for(stream).pair_hint; }
+
+void set_pair_hint(void *stream, int64_t n) { ws_for(stream).pair_hint = n; }
 
 Status ws_get(void *stream, int slot, size_t bytes, void **ptr) {
    Workspace &w = ws_for(stream);
@@ -78,6 +83,28 @@ namespace {
 std::mutex g_prof_mutex;
 bool g_prof_on = false;
 std::map<std::string, std::pair<double, int64_t>> g_prof;
+// timing events are reused (creating and destroying two events per launch
+// held up the launches that follow); only timers whose second event has
+// completed return theirs (collect()), others destroy them
+std::vector<hipEvent_t> g_event_pool;
+
+bool take_event(hipEvent_t *e) {
+   {
+      std::lock_guard<std::mutex> lock(g_prof_mutex);
+      if (!g_event_pool.empty()) {
+         *e = g_event_pool.back();
+         g_event_pool.pop_back();
+         return true;
+      }
+   }
+   return hipEventCreate(e) == hipSuccess;
+}
+
+void give_events(hipEvent_t a, hipEvent_t b) {
+   std::lock_guard<std::mutex> lock(g_prof_mutex);
+   g_event_pool.push_back(a);
+   g_event_pool.push_back(b);
+}
 } // namespace
 
 KernelTimer::KernelTimer(const char *n, hipStream_t s) : name(n), stream(s) {
@@ -86,7 +113,7 @@ KernelTimer::KernelTimer(const char *n, hipStream_t s) : name(n), stream(s) {
       on = g_prof_on;
    }
    if (!on) return;
-   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+   if (!take_event(&a) || !take_event(&b)) {
       on = false;
       return;
    }
@@ -100,11 +127,15 @@ void KernelTimer::stop() {
 void KernelTimer::collect() {
    if (!on) return;
    float ms = 0.0f;
-   if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess) {
-      std::lock_guard<std::mutex> lock(g_prof_mutex);
-      auto &e = g_prof[name];
-      e.first += ms;
-      e.second += 1;
+   if (hipEventSynchronize(b) == hipSuccess) {
+      if (hipEventElapsedTime(&ms, a, b) == hipSuccess) {
+         std::lock_guard<std::mutex> lock(g_prof_mutex);
+         auto &e = g_prof[name];
+         e.first += ms;
+         e.second += 1;
+      }
+      give_events(a, b);
+      a = b = nullptr;
    }
    on = false;
 }

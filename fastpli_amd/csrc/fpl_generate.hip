@@ -214,6 +214,7 @@ __global__ void k_bundle_boxes(Params pr, const BundleDev *__restrict__ bd,
    if (b == 0 && threadIdx.x == 0) {
       flags[0] = 0; // warnings
       flags[1] = 0; // voxelizer: a tile list too long for 16-bit positions
+      flags[2] = 0; // tile pairs beyond the capacity the host assumed (run_once)
    }
    double lo[3], hi[3];
    box_empty(lo, hi);
@@ -457,7 +458,7 @@ __global__ void k_segments(Params pr, const double *__restrict__ P,
 __global__ void k_emit(Params pr, const SegRec *__restrict__ segs,
                        const float4 *__restrict__ seg_dir,
                        const int64_t *__restrict__ seg_count, const int64_t *__restrict__ seg_off,
-                       uint32_t *keys, uint32_t *vals) {
+                       int64_t cap, uint32_t *keys, uint32_t *vals) {
    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
    if (i >= pr.n_points) return;
    if (seg_count[i] == 0) return;
@@ -477,11 +478,32 @@ __global__ void k_emit(Params pr, const SegRec *__restrict__ segs,
                keep = b.za < b.zb;
             }
             if (keep) {
-               keys[o] = (uint32_t)(((int64_t)tx * pr.nty + ty) * pr.ntz + tz);
-               vals[o] = (uint32_t)i;
+               if (o < cap) { // pairs past the capacity: k_pad reports them
+                  keys[o] = (uint32_t)(((int64_t)tx * pr.nty + ty) * pr.ntz + tz);
+                  vals[o] = (uint32_t)i;
+               }
                o++;
             }
          }
+}
+
+// The pair count (the scan's total) into flags[4..5] for the host's next
+// call; when the host sized the pair arrays without reading it (cap, the
+// previous call's count), pads [count, cap) with keys past the last tile, or
+// sets flags[2] when the count exceeds cap.
+__global__ void k_pad(const int64_t *__restrict__ count, int64_t cap, uint32_t pad_key,
+                      uint32_t *keys, uint32_t *vals, uint32_t *flags) {
+   const int64_t n = *count;
+   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (p == 0) {
+      flags[4] = (uint32_t)n;
+      flags[5] = (uint32_t)((uint64_t)n >> 32);
+      if (n > cap) flags[2] = 1;
+   }
+   if (p >= n && p < cap) {
+      keys[p] = pad_key;
+      vals[p] = 0;
+   }
 }
 
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
@@ -492,11 +514,12 @@ __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
 // Each tile's [begin, end) in the sorted list, and the bundles of its first
 // and last segment (the list is in global segment order, bundle-major)
 __global__ void k_ranges(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
-                         int64_t n, const SegRec *__restrict__ segs, uint32_t *range,
-                         int2 *tile_bundle) {
+                         int64_t n, uint32_t n_tiles, const SegRec *__restrict__ segs,
+                         uint32_t *range, int2 *tile_bundle) {
    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
    if (p >= n) return;
    const uint32_t t = keys[p];
+   if (t >= n_tiles) return; // padding (k_pad)
    if (p == 0 || keys[p - 1] != t) {
       range[2 * (int64_t)t] = (uint32_t)p;
       tile_bundle[t].x = segs[vals[p]].bundle;
@@ -1264,8 +1287,15 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
-Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, int32_t *label,
-           float *vector, double *props, uint32_t *warn, hipStream_t st) {
+// One generation.  use_hint: size the (tile, segment) pair arrays by the
+// previous call's pair count on this stream instead of reading the count back
+// in the middle of the call (a host round trip that leaves the GPU idle);
+// `overflow` reports a count above it (the outputs are then incomplete and
+// the caller runs again without the hint).
+Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label,
+                int32_t *label, float *vector, double *props, uint32_t *warn, hipStream_t st,
+                bool use_hint, bool &overflow) {
+   overflow = false;
    if (!vol || !fbs) return make_status(FPL_E_INVALID, "null argument");
    const int64_t nx = vol->dim[0], ny = vol->dim[1], nz = vol->dim[2];
    if (nx <= 0 || ny <= 0 || nz <= 0)
@@ -1406,6 +1436,7 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
                          fiber_box, bundle_box, (uint8_t *)fok, d_flags);
    }
    int64_t total = 0;
+   const int64_t hint = use_hint ? pair_hint(st) : 0;
    if (np > 0) {
       hipLaunchKernelGGL(k_segments, dim3(grid_for(np, B)), dim3(B), 0, st, pr, fbs->points,
                          fbs->fiber_ptr, d_bd, (const uint8_t *)fok, (SegRec *)segs,
@@ -1419,10 +1450,14 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
       FPL_TRY(ws_get(st, WS_CUB_TEMP, tmp_bytes, &tmp));
       FPL_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, (int64_t *)cnt, (int64_t *)off,
                                                (int)(np + 1), st));
-      FPL_HIP(hipMemcpyAsync(stage, (int64_t *)off + np, sizeof(int64_t), hipMemcpyDeviceToHost,
-                             st));
-      FPL_HIP(hipStreamSynchronize(st));
-      total = *(int64_t *)stage;
+      if (hint > 0) {
+         total = hint; // k_pad pads to it or reports more
+      } else {
+         FPL_HIP(hipMemcpyAsync(stage, (int64_t *)off + np, sizeof(int64_t),
+                                hipMemcpyDeviceToHost, st));
+         FPL_HIP(hipStreamSynchronize(st));
+         total = *(int64_t *)stage;
+      }
    }
    if (total >= ((int64_t)1 << 31))
       return make_status(FPL_E_INVALID, "too many segment/tile pairs: %lld", (long long)total);
@@ -1436,12 +1471,17 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
    int2 *tile_bundle = (int2 *)((uint32_t *)rng + 2 * n_tiles); // set with the ranges
    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(2 * n_tiles, B)), dim3(B), 0, st,
                       (uint32_t *)rng, 2 * n_tiles, 0u);
+   if (np > 0)
+      hipLaunchKernelGGL(k_pad, dim3(grid_for(total > 0 ? total : 1, B)), dim3(B), 0, st,
+                         (const int64_t *)off + np, total, (uint32_t)n_tiles, (uint32_t *)ka,
+                         (uint32_t *)va, d_flags);
    if (total > 0) {
       hipLaunchKernelGGL(k_emit, dim3(grid_for(np, B)), dim3(B), 0, st, pr, (const SegRec *)segs,
                          (const float4 *)seg_dir, (const int64_t *)cnt, (const int64_t *)off,
-                         (uint32_t *)ka, (uint32_t *)va);
+                         total, (uint32_t *)ka, (uint32_t *)va);
+      // keys up to n_tiles (the padding key) take part in the sort
       int nbits = 1;
-      while (((int64_t)1 << nbits) < n_tiles) nbits++;
+      while (((int64_t)1 << nbits) <= n_tiles) nbits++;
       size_t tmp_bytes = 0;
       FPL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (uint32_t *)ka,
                                                  (uint32_t *)kb, (uint32_t *)va, (uint32_t *)vb,
@@ -1452,8 +1492,8 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
                                                  (uint32_t *)va, (uint32_t *)vb, (int)total, 0,
                                                  nbits, st));
       hipLaunchKernelGGL(k_ranges, dim3(grid_for(total, B)), dim3(B), 0, st, (uint32_t *)kb,
-                         (const uint32_t *)vb, total, (const SegRec *)segs, (uint32_t *)rng,
-                         tile_bundle);
+                         (const uint32_t *)vb, total, (uint32_t)n_tiles, (const SegRec *)segs,
+                         (uint32_t *)rng, tile_bundle);
    }
    const bool lsq_lds = pr.n_lsq <= LSQ_LDS;
    // the 16-bit build packs a winner's label into 14 bits of its key (labels
@@ -1474,10 +1514,18 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
                             label, vector, d_flags);
       kt.stop();
       FPL_HIP(hipGetLastError());
-      FPL_HIP(hipMemcpyAsync(stage, d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      FPL_HIP(hipMemcpyAsync(stage, d_flags, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
       FPL_HIP(hipStreamSynchronize(st));
       kt.collect();
-      rerun = (((uint32_t *)stage)[1] & 1u) != 0;
+      const uint32_t *f = (const uint32_t *)stage;
+      if (np > 0) {
+         if (f[2]) {
+            overflow = true;
+            return Status();
+         }
+         set_pair_hint(st, (int64_t)((uint64_t)f[4] | (uint64_t)f[5] << 32));
+      }
+      rerun = (f[1] & 1u) != 0;
    }
    if (rerun) {
       // a tile list longer than 65535 (or labels beyond 15 bits): the whole
@@ -1496,11 +1544,28 @@ Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, 
                             label, vector, d_flags);
       kt2.stop();
       FPL_HIP(hipGetLastError());
-      FPL_HIP(hipMemcpyAsync(stage, d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      FPL_HIP(hipMemcpyAsync(stage, d_flags, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
       FPL_HIP(hipStreamSynchronize(st));
       kt2.collect();
+      const uint32_t *f = (const uint32_t *)stage;
+      if (np > 0) {
+         if (f[2]) {
+            overflow = true;
+            return Status();
+         }
+         set_pair_hint(st, (int64_t)((uint64_t)f[4] | (uint64_t)f[5] << 32));
+      }
    }
    if (warn) *warn = *(uint32_t *)stage;
+   return Status();
+}
+
+Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, int32_t *label,
+           float *vector, double *props, uint32_t *warn, hipStream_t st) {
+   bool overflow = false;
+   FPL_TRY(run_once(vol, fbs, only_label, label, vector, props, warn, st, true, overflow));
+   if (overflow)
+      FPL_TRY(run_once(vol, fbs, only_label, label, vector, props, warn, st, false, overflow));
    return Status();
 }
 

@@ -81,6 +81,11 @@ enum Slot : int {
 
 Status ws_get(void *stream, int slot, size_t bytes, void **ptr);
 
+// The (tile, segment) pair count of the stream's last generation (0: none),
+// which the next one assumes instead of reading its own back mid-call.
+int64_t pair_hint(void *stream);
+void set_pair_hint(void *stream, int64_t n);
+
 // Pinned host staging for the small device->host readbacks of a call.
 Status host_stage(void *stream, size_t bytes, void **ptr);
 
