@@ -17,8 +17,12 @@ Parity pins (see tests/test_oracle_*.py):
   * run_simulation: golden images of all 5 tilts (Slerp) at 1e-5 relative;
     closed forms for uniform columns; rho-rotation identity.
   * Lerp, flip_z, step_size != 1, tilted sensor (untilt_sensor_view=False)
-    and refraction are pinned only by invariants (parity partially unpinned:
-    no reference vectors exist for them).
+    and refraction have no reference vectors; they are pinned by closed
+    forms, invariants, a library routine (Lerp on same-hemisphere fields =
+    scipy's trilinear ``RegularGridInterpolator``) and geometry (each tilted
+    entry point, moved up the light direction and turned by scipy's ZYZ
+    rotation, lies on its pixel's vertical line): ``light_starts`` and
+    ``get_vec`` expose those two pieces of the march.
 
 ``oracle.adjoint`` (torch float64) restates the NN march differentiably: the
 FP64 reference gradient of the adjoint kernel, its forward pinned to
@@ -72,6 +76,10 @@ def lib():
         ]
         L.oracle_count_steps.restype = _i64
         L.oracle_count_steps.argtypes = [_i64, _i64, _i64, _d, _d, _d]
+        L.oracle_light_starts.restype = None
+        L.oracle_light_starts.argtypes = [_i64, _i64, _i64, _d, _d, _d, _int, _p]
+        L.oracle_get_vec.restype = None
+        L.oracle_get_vec.argtypes = [_i64, _i64, _i64, _p, _p, _int, _int, _d, _d, _d, _p]
         _LIB = L
     return _LIB
 
@@ -142,3 +150,24 @@ def count_steps(dim, step_size, theta, phi):
     nx, ny, nz = (int(d) for d in dim)
     return int(lib().oracle_count_steps(nx, ny, nz, float(step_size),
                                         float(theta), float(phi)))
+
+
+def light_starts(dim, theta, phi, tissue_refrection=1.0, untilt_sensor_view=False):
+    """Each pixel's ray entry point on the bottom plane, (nx, ny, 2) x, y (before the half step)."""
+    nx, ny, nz = (int(d) for d in dim)
+    out = np.empty((nx, ny, 2), dtype=np.float64)
+    lib().oracle_light_starts(nx, ny, nz, float(theta), float(phi), float(tissue_refrection),
+                              int(bool(untilt_sensor_view)), _ptr(out))
+    return out
+
+
+def get_vec(label, vector, pos, interpolate='Lerp', flip_z=False):
+    """GetVec at one position (x, y, z) -> float64 (3,)."""
+    label = np.ascontiguousarray(label, dtype=np.int32)
+    vector = np.ascontiguousarray(vector, dtype=np.float32)
+    nx, ny, nz = label.shape
+    out = np.empty(3, dtype=np.float64)
+    lib().oracle_get_vec(nx, ny, nz, _ptr(label), _ptr(vector), int(bool(flip_z)),
+                         INTERP[interpolate], float(pos[0]), float(pos[1]), float(pos[2]),
+                         _ptr(out))
+    return out

@@ -25,11 +25,14 @@
 //
 // Pins (tests/test_oracle*.py): the reference's golden fixture (labels, axes,
 // 5 Slerp tilts), closed forms and brute force for generation and for NN /
-// Slerp / flip_z / step_size / background absorption.  Two branches have no
-// reference vector and only partial checks -- parity unpinned: the Lerp blend
-// of corners with differing axes (checked only on uniform fields, where it
-// reduces to NN) and the tilted-sensor start positions at theta != 0 (checked
-// only at normal incidence and through their step count).
+// Slerp / flip_z / step_size / background absorption.  The Lerp blend and the
+// tilted-sensor start positions have no reference vectors; they are pinned by
+// a library routine and by geometry: Lerp on fields whose blends keep
+// non-negative dot products equals scipy's trilinear interpolation, negating
+// corners changes the result by its sign only, corners of other labels hand
+// the blend to the other corner; each tilted-sensor entry point, moved up the
+// light direction and turned by scipy's ZYZ rotation, lies on its pixel's
+// vertical line (oracle_light_starts / oracle_get_vec expose the two pieces).
 //
 // Parallelism mirrors the reference so that the timing is a fair CPU
 // baseline: generation lets every thread walk all segments and own the
@@ -503,9 +506,71 @@ struct Ray {
    int64_t cx, cy;
 };
 
+// CalcStartingLightPositions (simulator.cpp:604-773): the entry point of each
+// pixel's ray on the bottom plane z = 0 (before the half step)
+std::vector<Ray> light_starts(int64_t nx, int64_t ny, int64_t nz, double theta, double phi,
+                              double tissue_refrection, int untilt_sensor_view) {
+   const V3 dir = {pi_cos(phi) * pi_sin(theta), pi_sin(phi) * pi_sin(theta), pi_cos(theta)};
+   std::vector<Ray> rays;
+   rays.reserve(nx * ny);
+   const V3 shift = scale(V3{dir.x / std::cos(theta), dir.y / std::cos(theta), dir.z / std::cos(theta)},
+                          (double)nz);
+   if (untilt_sensor_view) {
+      for (int64_t i = 0; i < nx; i++)
+         for (int64_t j = 0; j < ny; j++)
+            rays.push_back({{(i + 0.5) - 0.5 * shift.x, (j + 0.5) - 0.5 * shift.y, 0.0}, i, j});
+   } else {
+      const double ang = std::asin(std::sin(-theta) * tissue_refrection);
+      const M3 r1 = rot_zyz(phi, ang, -phi);
+      const M3 r1_inv = rot_zyz(phi, -ang, -phi);
+      const V3 pc = {nx * 0.5, ny * 0.5, nz * 0.5};
+      V3 pt = {-pc.x, -pc.y, pc.z};
+      V3 e1 = {1, 0, 0}, e2 = {0, 1, 0};
+      pt = mat_vec(r1, pt);
+      e1 = mat_vec(r1, e1);
+      e2 = mat_vec(r1, e2);
+      const double a = e1.x, b = e2.x, c = e1.y, d = e2.y;
+      for (int64_t i = 0; i < nx; i++)
+         for (int64_t j = 0; j < ny; j++) {
+            const double e = i + 0.5 - pc.x - pt.x;
+            const double ff = j + 0.5 - pc.y - pt.y;
+            const double D = a * d - c * b;
+            const double Ds = e * d - ff * b;
+            const double Dt = a * ff - c * e;
+            const double s = Ds / D, t = Dt / D;
+            const V3 p = add(add(add(pc, pt), scale(e1, s)), scale(e2, t));
+            const V3 tis = add(mat_vec(r1_inv, sub(p, pc)), pc);
+            rays.push_back({{tis.x - shift.x, tis.y - shift.y, 0.0}, i, j});
+         }
+   }
+   return rays;
+}
+
 } // namespace
 
 extern "C" {
+
+// Test access to two pieces of the march: the rays' entry points on the bottom
+// plane (out: (nx, ny, 2) x, y before the half step) and GetVec at one
+// position (out: 3 doubles; interp 0 NN, 1 Lerp, 2 Slerp).
+void oracle_light_starts(int64_t nx, int64_t ny, int64_t nz, double theta, double phi,
+                         double tissue_refrection, int untilt_sensor_view, double *out) {
+   const std::vector<Ray> rays =
+       light_starts(nx, ny, nz, theta, phi, tissue_refrection, untilt_sensor_view);
+   for (const Ray &r : rays) {
+      out[2 * (r.cx * ny + r.cy) + 0] = r.pos.x;
+      out[2 * (r.cx * ny + r.cy) + 1] = r.pos.y;
+   }
+}
+
+void oracle_get_vec(int64_t nx, int64_t ny, int64_t nz, const int32_t *label, const float *vec,
+                    int flip_z, int interp, double x, double y, double z, double *out) {
+   Sim S{nx, ny, nz, label, vec, flip_z, interp};
+   const V3 v = S.get_vec(V3{x, y, z});
+   out[0] = v.x;
+   out[1] = v.y;
+   out[2] = v.z;
+}
 
 // interp: 0 NN, 1 Lerp, 2 Slerp.  props: (n_props, 2) dn, mu.
 // out: (nx, ny, n_rho) float64, C order.  Returns 0, or a negative code:
@@ -570,38 +635,8 @@ int oracle_simulate(int64_t nx, int64_t ny, int64_t nz, const int32_t *label,
    const V3 step = scale(dir, step_size);
 
    // starting positions (simulator.cpp:604-773)
-   std::vector<Ray> rays;
-   rays.reserve(nx * ny);
-   const V3 shift = scale(V3{dir.x / std::cos(theta), dir.y / std::cos(theta), dir.z / std::cos(theta)},
-                          (double)nz);
-   if (untilt_sensor_view) {
-      for (int64_t i = 0; i < nx; i++)
-         for (int64_t j = 0; j < ny; j++)
-            rays.push_back({{(i + 0.5) - 0.5 * shift.x, (j + 0.5) - 0.5 * shift.y, 0.0}, i, j});
-   } else {
-      const double ang = std::asin(std::sin(-theta) * tissue_refrection);
-      const M3 r1 = rot_zyz(phi, ang, -phi);
-      const M3 r1_inv = rot_zyz(phi, -ang, -phi);
-      const V3 pc = {nx * 0.5, ny * 0.5, nz * 0.5};
-      V3 pt = {-pc.x, -pc.y, pc.z};
-      V3 e1 = {1, 0, 0}, e2 = {0, 1, 0};
-      pt = mat_vec(r1, pt);
-      e1 = mat_vec(r1, e1);
-      e2 = mat_vec(r1, e2);
-      const double a = e1.x, b = e2.x, c = e1.y, d = e2.y;
-      for (int64_t i = 0; i < nx; i++)
-         for (int64_t j = 0; j < ny; j++) {
-            const double e = i + 0.5 - pc.x - pt.x;
-            const double ff = j + 0.5 - pc.y - pt.y;
-            const double D = a * d - c * b;
-            const double Ds = e * d - ff * b;
-            const double Dt = a * ff - c * e;
-            const double s = Ds / D, t = Dt / D;
-            const V3 p = add(add(add(pc, pt), scale(e1, s)), scale(e2, t));
-            const V3 tis = add(mat_vec(r1_inv, sub(p, pc)), pc);
-            rays.push_back({{tis.x - shift.x, tis.y - shift.y, 0.0}, i, j});
-         }
-   }
+   std::vector<Ray> rays = light_starts(nx, ny, nz, theta, phi, tissue_refrection,
+                                        untilt_sensor_view);
    if (rays.empty()) warn |= ORACLE_WARN_NO_LIGHT;
    for (auto &r : rays) r.pos = add(r.pos, scale(step, 0.5));
 

@@ -267,3 +267,110 @@ def test_oracle_tilted_sensor_at_normal_incidence():
                                untilt_sensor_view=False, tissue_refrection=1.4)
         b, _ = oracle.simulate(T, V, P, 0.0, 0.0, interpolate=interp, filter_rotations=rho)
         assert rel_err(a, b).max() < 1e-12
+
+
+def test_oracle_tilted_sensor_start_geometry():
+    """The tilted sensor's ray entry points (CalcStartingLightPositionsTilted,
+    simulator.cpp:618-703) by their geometric definition: a pixel's vertical
+    line in the laboratory meets the tissue's top plane turned by
+    R = Rz(phi) Ry(a) Rz(-phi), a = asin(sin(-theta) n), about the volume
+    centre; R^-1 of that point, moved down along the light direction to
+    z = 0, is the entry point.  Checked backwards with scipy's rotation: from
+    each entry point up the light direction to z = nz, then R, the point lies
+    on its pixel's vertical line (this fails for R and R^-1 swapped, a missing
+    refraction, a sign of theta or phi, or a transposed pixel grid)."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(3)
+    cases = [(np.deg2rad(5.5), p, 1.0) for p in (0.0, np.pi / 2, np.pi, -np.pi / 2)]
+    cases += [(rng.uniform(0.01, 0.5), rng.uniform(-np.pi, np.pi), n) for n in (1.0, 1.3, 1.0)]
+    for dim in [(10, 8, 6), (33, 17, 40)]:
+        nx, ny, nz = dim
+        pc = np.array(dim, dtype=float) / 2
+        for theta, phi, n in cases:
+            st = oracle.light_starts(dim, theta, phi, n)
+            d = np.array([np.cos(phi) * np.sin(theta), np.sin(phi) * np.sin(theta),
+                          np.cos(theta)])
+            bottom = np.concatenate([st, np.zeros(st.shape[:2] + (1,))], axis=-1)
+            top = bottom + d / np.cos(theta) * nz
+            R = Rotation.from_euler('ZYZ', [phi, np.arcsin(np.sin(-theta) * n), -phi])
+            lab = (top - pc) @ R.as_matrix().T + pc
+            ii, jj = np.meshgrid(np.arange(nx) + 0.5, np.arange(ny) + 0.5, indexing='ij')
+            assert np.allclose(lab[..., 0], ii, rtol=0, atol=1e-9), (dim, theta, phi, n)
+            assert np.allclose(lab[..., 1], jj, rtol=0, atol=1e-9), (dim, theta, phi, n)
+
+
+def _cone_field(rng, shape, half_angle_deg=40.0):
+    """Unit axes within a cone about a random axis (every pair, and every
+    positive combination of them, has a positive dot product)."""
+    axis = rng.normal(size=3)
+    axis /= np.linalg.norm(axis)
+    v = rng.normal(size=shape + (3,))
+    v -= (v @ axis)[..., None] * axis
+    v /= np.linalg.norm(v, axis=-1, keepdims=True)
+    ang = np.deg2rad(rng.uniform(0, half_angle_deg, size=shape))[..., None]
+    return (np.cos(ang) * axis + np.sin(ang) * v).astype(np.float32)
+
+
+def test_oracle_lerp_blend_is_trilinear_interpolation():
+    """GetVec with Lerp (LinearOrientationInterpolation, simulator.cpp:457-471)
+    on one label, where every blend has a non-negative dot product: the x, y, z
+    cascade of (1 - t) a + t b is trilinear interpolation of the components,
+    checked against scipy's RegularGridInterpolator (voxel centres at i + 0.5;
+    beyond the outer centres the clamped corner indices equal interpolation at
+    the clipped position).  Wrong weights, a transposed corner or a missing
+    clamp fail it."""
+    from scipy.interpolate import RegularGridInterpolator
+    rng = np.random.default_rng(11)
+    shape = (5, 4, 6)
+    vector = _cone_field(rng, shape)
+    label = np.ones(shape, dtype=np.int32)
+    grid = tuple(np.arange(n) + 0.5 for n in shape)
+    ref = RegularGridInterpolator(grid, vector.astype(np.float64), method='linear')
+    hi = np.array(shape, dtype=float)
+    for _ in range(400):
+        pos = rng.uniform(0.0, hi)
+        got = oracle.get_vec(label, vector, pos, 'Lerp')
+        want = ref(np.clip(pos, 0.5, hi - 0.5)[None])[0]
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-12), (pos, got, want)
+
+
+@pytest.mark.parametrize('interp', ['Lerp', 'Slerp'])
+def test_oracle_interpolation_ignores_axis_sign(interp):
+    """An axis and its negative are one fiber orientation: negating any corners
+    leaves the interpolated axis unchanged up to its sign (the Lerp and Slerp
+    hemisphere rules, simulator.cpp:449-494); without them the blend of a
+    flipped corner would shrink or turn the result."""
+    rng = np.random.default_rng(12)
+    shape = (4, 5, 3)
+    vector = _cone_field(rng, shape, 60.0)
+    label = np.ones(shape, dtype=np.int32)
+    flipped = vector * rng.choice([-1.0, 1.0], size=shape + (1,)).astype(np.float32)
+    for _ in range(300):
+        pos = rng.uniform(0.0, np.array(shape, dtype=float))
+        a = oracle.get_vec(label, vector, pos, interp)
+        b = oracle.get_vec(label, flipped, pos, interp)
+        s = 1.0 if a @ b >= 0 else -1.0
+        assert np.allclose(b, s * a, rtol=1e-9, atol=1e-12), (pos, a, b)
+
+
+def test_oracle_lerp_corners_of_other_labels():
+    """Corners whose label differs from the sample's voxel count as zero, and a
+    zero corner leaves the blend to the other one (v == 0 -> u,
+    simulator.cpp:461-464, 533-540): between a label-1 voxel with axis a and a
+    label-2 voxel with axis b, every sample in the first voxel returns a
+    exactly and every sample in the second b; a label-0 voxel returns zero.
+    Known answer of the subtraction branch: a = (1, 0, 0), b = (-0.6, 0.8, 0)
+    at weight 1/4 give 3/4 a - 1/4 b = (0.9, -0.2, 0)."""
+    a = np.array([0.6, 0.0, 0.8], dtype=np.float32)
+    b = np.array([0.0, 1.0, 0.0], dtype=np.float32)
+    vector = np.stack([a, b, a]).reshape(3, 1, 1, 3)
+    label = np.array([1, 2, 0], dtype=np.int32).reshape(3, 1, 1)
+    for x in np.linspace(0.05, 0.95, 7):
+        assert np.array_equal(oracle.get_vec(label, vector, (x, 0.5, 0.5)), a.astype(float))
+        assert np.array_equal(oracle.get_vec(label, vector, (1 + x, 0.5, 0.5)),
+                              b.astype(float))
+        assert np.array_equal(oracle.get_vec(label, vector, (2 + x, 0.5, 0.5)), np.zeros(3))
+    vector = np.array([[1.0, 0.0, 0.0], [-0.6, 0.8, 0.0]], dtype=np.float32).reshape(2, 1, 1, 3)
+    label = np.ones((2, 1, 1), dtype=np.int32)
+    got = oracle.get_vec(label, vector, (0.75, 0.5, 0.5), 'Lerp')
+    assert np.allclose(got, [0.9, -0.2, 0.0], rtol=0, atol=1e-7), got
