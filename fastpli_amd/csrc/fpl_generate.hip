@@ -14,10 +14,14 @@
 //                 (fiber_bundle.cpp:147-154); min/max is exact in any order
 //   k_fibers      one thread per fiber: bundle/fiber culling (generator.cpp:208-221)
 //   k_segments    one thread per segment: FP64 record + integer bbox
-//                 (generator.cpp:293-305) + number of 8x8x32 voxel tiles it touches
+//                 (generator.cpp:293-305) + the 8x8x32 voxel tiles of the bbox that
+//                 can hold a candidate (tile_box): count and mask
 //   scan          hipcub exclusive sum of per-segment tile counts
+//   k_pad         the pair count for the next call; padding up to the count this
+//                 call assumed (the previous call's), or a flag when it is larger
 //   k_emit        (tile, segment) pairs in segment order
 //   radix sort    stable by tile -> per tile list in global segment order
+//   k_ranges      each tile's list range and the bundles at its two ends
 //   k_voxelize    one workgroup per tile, the tile's segment records staged in
 //                 LDS; each wave owns every fourth z layer of the tile (8x8x8
 //                 voxels), whose state lives in LDS.
