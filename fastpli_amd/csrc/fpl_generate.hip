@@ -1440,7 +1440,10 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_la
                          fiber_box, bundle_box, (uint8_t *)fok, d_flags);
    }
    int64_t total = 0;
-   const int64_t hint = use_hint ? pair_hint(st) : 0;
+   // a hint above 64 tiles per point (a much larger previous scene) would make
+   // this call pad and sort mostly padding: it reads its count back instead
+   int64_t hint = use_hint ? pair_hint(st) : 0;
+   if (hint > 64 * fbs->n_points) hint = 0;
    if (np > 0) {
       hipLaunchKernelGGL(k_segments, dim3(grid_for(np, B)), dim3(B), 0, st, pr, fbs->points,
                          fbs->fiber_ptr, d_bd, (const uint8_t *)fok, (SegRec *)segs,
