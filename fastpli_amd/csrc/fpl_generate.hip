@@ -974,6 +974,17 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
       if (threadIdx.x == 0) atomicOr(&flags[1], 1u);
       return;
    }
+   // the list entries of this thread's (at most two) staging pieces of the
+   // next chunk, read one chunk ahead (the first chunk's before the set-up
+   // below): staging then waits for the record loads only
+   static_assert(CHUNK * 8 <= 2 * WG, "two staging pieces per thread");
+   uint32_t ahead0 = 0, ahead1 = 0;
+   auto read_ahead = [&](uint32_t nb) {
+      const int nn = nb < end ? (int)min(end - nb, (uint32_t)CHUNK) : 0;
+      if ((int)threadIdx.x < nn * 8) ahead0 = list[nb + (threadIdx.x >> 3)];
+      if ((int)threadIdx.x + WG < nn * 8) ahead1 = list[nb + ((threadIdx.x + WG) >> 3)];
+   };
+   read_ahead(beg);
    WaveState<SegT> &W = wstate[wave];
    for (int s = lane; s < SLOTS; s += 64) {
       W.key[s] = KEY_EMPTY;
@@ -996,26 +1007,17 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
    int q_head = 0, q_tail = 0;
    wave_sync();
 
-   // the list entries of this thread's (at most two) staging pieces of the
-   // next chunk, read one chunk ahead: staging then waits for one load only
-   static_assert(CHUNK * 8 <= 2 * WG, "two staging pieces per thread");
-   uint32_t ahead0 = 0, ahead1 = 0;
    for (uint32_t base = beg; base < end; base += CHUNK) {
       const int n = (int)((end - base) < (uint32_t)CHUNK ? (end - base) : (uint32_t)CHUNK);
       const uint32_t pos1 = base - beg + 1u; // list position + 1 of the chunk's first record
       __syncthreads(); // every wave is done with the previous chunk
       for (int e = threadIdx.x, r = 0; e < n * 8; e += WG, r++) {
          const int c = e >> 3, piece = e & 7;
-         const uint32_t sidx = base == beg ? list[base + c] : (r == 0 ? ahead0 : ahead1);
+         const uint32_t sidx = r == 0 ? ahead0 : ahead1;
          ((int4 *)&chunk[c].r)[piece] = ((const int4 *)&segs[sidx])[piece];
          if (piece == 0) chunk_gidx[c] = sidx;
       }
-      {
-         const uint32_t nb = base + CHUNK;
-         const int nn = nb < end ? (int)min(end - nb, (uint32_t)CHUNK) : 0;
-         if ((int)threadIdx.x < nn * 8) ahead0 = list[nb + (threadIdx.x >> 3)];
-         if ((int)threadIdx.x + WG < nn * 8) ahead1 = list[nb + ((threadIdx.x + WG) >> 3)];
-      }
+      read_ahead(base + CHUNK);
       __syncthreads();
 
       // Stage A set-up, one lane per record of the chunk: the record's candidate
