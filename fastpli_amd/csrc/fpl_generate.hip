@@ -1177,7 +1177,7 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
                    flags, ov, free_mode);
          q_head += nb;
       }
-      if (!ov)
+      if (!ov && !free_mode) // (the free mode does not look at the warning)
          ov = uniform((int)((__atomic_load_n(&flags[0], __ATOMIC_RELAXED) & FPL_WARN_OVERLAP) !=
                             0)) != 0;
    }
