@@ -1072,6 +1072,13 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
          int cnt = 0;
          uint32_t geo = 0, magic = 0;
          if (p < n_pairs) {
+            // the slab of the record, requested before the rectangle's arithmetic
+            float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0;
+            if (free_mode) {
+               const uint32_t gi = chunk_gidx[rc];
+               c0 = seg_clip[2 * gi];
+               c1 = seg_clip[2 * gi + 1];
+            }
             const SegRec &R = chunk[rc].r;
             const int l = (ri & 7) + (p - (ri >> 3));
             const int cx0 = max(R.lo[0], xt), cx1 = min(R.hi[0], xt + TX);
@@ -1097,8 +1104,6 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
                if (free_mode && xa < xb && ya < yb) {
                   // the columns and rows whose voxels can lie in the segment's slab
                   // a in [a_lo, a_hi] (above); a = u.x x + u.y y + a0 is linear
-                  const uint32_t gi = chunk_gidx[rc];
-                  const float4 c0 = seg_clip[2 * gi], c1 = seg_clip[2 * gi + 1];
                   const float a0 = (zf - c1.w) * c0.z - c1.y * c0.x - c1.z * c0.y;
                   clip_axis(c0.x, c0.y, a0, c0.w, c1.x, xa, xb, ya, yb);
                   clip_axis(c0.y, c0.x, a0, c0.w, c1.x, ya, yb, xa, xb);
