@@ -301,12 +301,17 @@ def main():
     vector = torch.empty(dim + [3], dtype=torch.float32, device=dev)
     images = torch.empty((len(TILTS), nx, ny, n_rho), dtype=torch.float64, device=dev)
     setup, _keep = s._setup()
+    fields_bytes, _ = _lib.fields_layout(dim, args.field_storage)
+    fields_buf = torch.empty(fields_bytes, dtype=torch.uint8, device=dev)
+    fields = _lib.fields_struct(fields_buf, dim, args.field_storage)
 
     def step():
-        props, _ = _lib.generate(dim, s.dim_origin, VOXEL, points_d, fiber_ptr_d, bundle_ptr,
-                                 layer_ptr, table, False, label, vector)
+        # generate_tissue: label + optical axis (the API's arrays) and the
+        # march's records, written by the voxelizer; run_simulation: 5 tilts
+        props, _ = _lib.generate_fields(dim, s.dim_origin, VOXEL, points_d, fiber_ptr_d,
+                                        bundle_ptr, layer_ptr, table, label, vector, fields)
         props[0, 1] = s.background_absorption
-        _lib.simulate(setup, dim, label, vector, props, TILTS, images)
+        _lib.simulate_fields(setup, fields, props, TILTS, images)
 
     log(f'warmup {args.warmup}')
     for _ in range(args.warmup):

@@ -65,6 +65,12 @@ class FplSetup(ctypes.Structure):
                 ('field_storage', ctypes.c_int32)]
 
 
+class FplFields(ctypes.Structure):
+    _fields_ = [('dim', _c_i64 * 3), ('storage', ctypes.c_int32), ('label_min', ctypes.c_int32),
+                ('label_max', ctypes.c_int32), ('pad_', ctypes.c_int32), ('records', _c_p),
+                ('codes', _c_p)]
+
+
 class FplRequest(ctypes.Structure):
     _fields_ = [('tissue', _c_i64), ('setup', ctypes.POINTER(FplSetup)), ('props', _c_p),
                 ('n_props', _c_i64), ('tilts', _c_p), ('n_tilts', _c_i64), ('images', _c_p)]
@@ -77,7 +83,9 @@ EXPORTS = ('fpl_num_properties', 'fpl_generate', 'fpl_simulate', 'fpl_optic_resa
            'fpl_count_voxel_steps', 'fpl_profile_enable', 'fpl_profile_read',
            'fpl_release_workspace', 'fpl_version', 'fpl_server_create', 'fpl_server_add_tissue',
            'fpl_server_remove_tissue', 'fpl_server_submit', 'fpl_server_wait',
-           'fpl_server_stats', 'fpl_server_destroy', 'fpl_rofl', 'fpl_simulate_adjoint')
+           'fpl_server_stats', 'fpl_server_destroy', 'fpl_rofl', 'fpl_simulate_adjoint',
+           'fpl_record_bytes', 'fpl_fields_bytes', 'fpl_fields_codes_offset',
+           'fpl_generate_fields', 'fpl_pack_fields', 'fpl_simulate_fields')
 
 
 def lib():
@@ -99,6 +107,26 @@ def lib():
         L.fpl_simulate.argtypes = [
             ctypes.POINTER(FplSetup), _c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_i64, _c_p,
             ctypes.POINTER(ctypes.c_uint32), _c_p, ctypes.c_char_p, ctypes.c_size_t
+        ]
+        L.fpl_record_bytes.restype = ctypes.c_size_t
+        L.fpl_record_bytes.argtypes = [ctypes.c_int32]
+        L.fpl_fields_bytes.restype = ctypes.c_size_t
+        L.fpl_fields_bytes.argtypes = [_c_p, ctypes.c_int32]
+        L.fpl_fields_codes_offset.restype = ctypes.c_size_t
+        L.fpl_fields_codes_offset.argtypes = [_c_p, ctypes.c_int32]
+        L.fpl_generate_fields.restype = ctypes.c_int
+        L.fpl_generate_fields.argtypes = [
+            ctypes.POINTER(FplVolume), ctypes.POINTER(FplFiberBundles), _c_p, _c_p, _c_p,
+            ctypes.POINTER(FplFields), ctypes.POINTER(ctypes.c_uint32), _c_p, ctypes.c_char_p,
+            ctypes.c_size_t
+        ]
+        L.fpl_pack_fields.restype = ctypes.c_int
+        L.fpl_pack_fields.argtypes = [_c_p, _c_p, _c_p, ctypes.POINTER(FplFields), _c_p,
+                                      ctypes.c_char_p, ctypes.c_size_t]
+        L.fpl_simulate_fields.restype = ctypes.c_int
+        L.fpl_simulate_fields.argtypes = [
+            ctypes.POINTER(FplSetup), ctypes.POINTER(FplFields), _c_p, _c_i64, _c_p, _c_i64,
+            _c_p, ctypes.POINTER(ctypes.c_uint32), _c_p, ctypes.c_char_p, ctypes.c_size_t
         ]
         L.fpl_simulate_adjoint.restype = ctypes.c_int
         L.fpl_simulate_adjoint.argtypes = [
@@ -194,6 +222,88 @@ def make_setup(step_size, light_intensity, voxel_size, wavelength, tissue_refrec
                  int(bool(flip_z)), int(rho.size), rho.ctypes.data if rho.size else None,
                  STORAGE[field_storage])
     return s, rho
+
+
+def _bundles(points, fiber_ptr, bundle_ptr, layer_ptr, layer_table):
+    """FplFiberBundles over device points / fiber offsets and host tables; returns
+    (struct, keep-alive arrays)."""
+    scale, dn, mu, model = (np.ascontiguousarray(a) for a in layer_table)
+    bundle_ptr = np.ascontiguousarray(bundle_ptr, dtype=np.int64)
+    layer_ptr = np.ascontiguousarray(layer_ptr, dtype=np.int64)
+    fbs = FplFiberBundles(
+        len(bundle_ptr) - 1,
+        int(fiber_ptr.shape[0]) - 1, int(points.shape[0]), _ptr(points), _ptr(fiber_ptr),
+        bundle_ptr.ctypes.data, layer_ptr.ctypes.data,
+        scale.ctypes.data if scale.size else None, dn.ctypes.data if dn.size else None,
+        mu.ctypes.data if mu.size else None, model.ctypes.data if model.size else None)
+    return fbs, (scale, dn, mu, model, bundle_ptr, layer_ptr)
+
+
+def _volume(dim, origin, voxel_size):
+    return FplVolume((_c_i64 * 3)(*[int(d) for d in dim]),
+                     (ctypes.c_double * 3)(*[float(o) for o in origin]), float(voxel_size))
+
+
+def record_bytes(storage):
+    """Bytes per voxel of the record array of a field storage ('f32', 'f16', 'angle16')."""
+    return int(lib().fpl_record_bytes(STORAGE[storage]))
+
+
+def fields_layout(dim, storage):
+    """(total bytes, byte offset of the angle codes) of one allocation of march fields."""
+    d = np.ascontiguousarray(dim, dtype=np.int64)
+    L = lib()
+    return (int(L.fpl_fields_bytes(d.ctypes.data, STORAGE[storage])),
+            int(L.fpl_fields_codes_offset(d.ctypes.data, STORAGE[storage])))
+
+
+def fields_struct(buffer, dim, storage, label_min=0, label_max=0):
+    """FplFields over a device byte buffer laid out as fields_layout says."""
+    _, off = fields_layout(dim, storage)
+    base = _ptr(buffer)
+    codes = base + off if storage == 'angle16' else None
+    return FplFields((_c_i64 * 3)(*[int(d) for d in dim]), STORAGE[storage], int(label_min),
+                     int(label_max), 0, base, codes)
+
+
+def generate_fields(dim, origin, voxel_size, points, fiber_ptr, bundle_ptr, layer_ptr,
+                    layer_table, label, vector, fields):
+    """fpl_generate_fields: label / vector (device tensors or None) and the march
+    fields (FplFields, filled in place); returns (props ndarray, warn mask)."""
+    fbs, _keep = _bundles(points, fiber_ptr, bundle_ptr, layer_ptr, layer_table)
+    vol = _volume(dim, origin, voxel_size)
+    L = lib()
+    n_props = L.fpl_num_properties(ctypes.byref(fbs))
+    props = np.zeros((n_props, 2), dtype=np.float64)
+    warn = ctypes.c_uint32(0)
+    msg = ctypes.create_string_buffer(512)
+    rc = L.fpl_generate_fields(ctypes.byref(vol), ctypes.byref(fbs), _ptr(label), _ptr(vector),
+                               props.ctypes.data, ctypes.byref(fields), ctypes.byref(warn),
+                               _stream(), msg, 512)
+    check(rc, msg)
+    return props, warn.value
+
+
+def pack_fields(dim, label, vector, fields):
+    """fpl_pack_fields: C-order device fields -> march fields (filled in place)."""
+    d = np.ascontiguousarray(dim, dtype=np.int64)
+    msg = ctypes.create_string_buffer(512)
+    rc = lib().fpl_pack_fields(d.ctypes.data, _ptr(label), _ptr(vector), ctypes.byref(fields),
+                               _stream(), msg, 512)
+    check(rc, msg)
+
+
+def simulate_fields(setup, fields, props, tilts, images):
+    """fpl_simulate_fields on march fields; returns the warn mask."""
+    props = np.ascontiguousarray(props, dtype=np.float64)
+    tilts = np.ascontiguousarray(tilts, dtype=np.float64).reshape(-1, 2)
+    warn = ctypes.c_uint32(0)
+    msg = ctypes.create_string_buffer(512)
+    rc = lib().fpl_simulate_fields(ctypes.byref(setup), ctypes.byref(fields), props.ctypes.data,
+                                   props.shape[0], tilts.ctypes.data, tilts.shape[0],
+                                   _ptr(images), ctypes.byref(warn), _stream(), msg, 512)
+    check(rc, msg)
+    return warn.value
 
 
 def generate(dim, origin, voxel_size, points, fiber_ptr, bundle_ptr, layer_ptr, layer_table,

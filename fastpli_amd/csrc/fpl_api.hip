@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "fpl_internal.h"
+#include "fpl_records.h"
 
 namespace fpl {
 namespace {
@@ -185,6 +186,26 @@ void fpl_release_workspace(void) {
       if (it->second.host) (void)hipHostFree(it->second.host);
       it = fpl::g_ws.erase(it);
    }
+}
+
+size_t fpl_record_bytes(int32_t field_storage) {
+   return field_storage < 0 || field_storage > 2 ? 0 : fpl::rec::record_bytes(field_storage);
+}
+
+size_t fpl_fields_codes_offset(const int64_t dim[3], int32_t field_storage) {
+   if (!dim || dim[0] <= 0 || dim[1] <= 0 || dim[2] <= 0 || field_storage < 0 ||
+       field_storage > 2)
+      return 0;
+   const size_t n = (size_t)dim[0] * (size_t)dim[1] * (size_t)dim[2];
+   return (n * fpl::rec::record_bytes(field_storage) + 255) & ~(size_t)255;
+}
+
+size_t fpl_fields_bytes(const int64_t dim[3], int32_t field_storage) {
+   const size_t off = fpl_fields_codes_offset(dim, field_storage);
+   if (off == 0) return 0;
+   const size_t n = (size_t)dim[0] * (size_t)dim[1] * (size_t)dim[2];
+   const size_t sb = fpl::rec::second_bytes(field_storage);
+   return sb ? off + n * sb : n * fpl::rec::record_bytes(field_storage);
 }
 
 int64_t fpl_num_properties(const fpl_fiber_bundles *fbs) {

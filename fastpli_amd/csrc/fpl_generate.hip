@@ -39,6 +39,7 @@
 #include <vector>
 
 #include "fpl_internal.h"
+#include "fpl_records.h"
 
 namespace fpl {
 namespace gen {
@@ -87,6 +88,7 @@ struct Params {
    int32_t max_layer;
    int32_t ntx, nty, ntz;
    int32_t n_lsq; // rows of the layer tables
+   int32_t n_props;
 };
 
 // aabb::AABB(p, q) unsorted constructor (aabb.hpp:54-71)
@@ -219,6 +221,7 @@ __global__ void k_bundle_boxes(Params pr, const BundleDev *__restrict__ bd,
       flags[0] = 0; // warnings
       flags[1] = 0; // voxelizer: a tile list too long for 16-bit positions
       flags[2] = 0; // tile pairs beyond the capacity the host assumed (run_once)
+      flags[3] = 0; // voxelizer: a label >= n_props was written
    }
    double lo[3], hi[3];
    box_empty(lo, hi);
@@ -914,6 +917,61 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
    wave_sync();
 }
 
+// Where the voxelizer's final pass writes a voxel: the API's C-order fields
+// (label[x][y][z], vector[x][y][z][3]) and / or the march's [x][z][y] records
+// (fpl_records.h), each optional.
+struct Outputs {
+   int32_t *label;
+   float *vector;
+   void *rec_a;
+   uint32_t *rec_b;
+   int32_t rec_storage;
+};
+
+// the 16-bit storages (a double atan2 for the angle codes) out of line
+__device__ __forceinline__ void store_record_short(const Outputs &o, int64_t r, int32_t lab,
+                                                float x, float y, float z) {
+   int4 e;
+   rec::encode(o.rec_storage, lab, x, y, z, e);
+   rec::store(o.rec_storage, o.rec_a, o.rec_b, r, e);
+}
+
+// bit 1 of flags[3]: some label is not 0 (the "All labels are 0" warning of
+// Simpli.generate_tissue without a pass over the labels); one load per wave
+__device__ __forceinline__ void note_nonzero(uint64_t any, uint32_t *flags) {
+   if (any != 0 && (threadIdx.x & 63) == 0 &&
+       (__atomic_load_n(&flags[3], __ATOMIC_RELAXED) & 2u) == 0)
+      atomicOr(&flags[3], 2u);
+}
+
+// REC16: the records are in a 16-bit storage (a kernel variant of its own:
+// the encoders' registers would make the f32 build spill)
+template <bool REC16>
+__device__ __forceinline__ void write_voxel(const Outputs &o, const Params &pr, int vx, int vy,
+                                            int vz, int32_t lab, const float vo[3],
+                                            uint32_t *flags) {
+   // the past-the-end layer index of the last bundle gives label n_props
+   // (beyond the property table, which CheckInput rejects): bit 0 of flags[3]
+   if (lab >= pr.n_props) atomicOr(&flags[3], 1u);
+   if (o.label || o.vector) {
+      const int64_t ind = ((int64_t)vx * pr.ny + vy) * pr.nz + vz;
+      if (o.label) o.label[ind] = lab;
+      if (o.vector) {
+         o.vector[3 * ind + 0] = vo[0];
+         o.vector[3 * ind + 1] = vo[1];
+         o.vector[3 * ind + 2] = vo[2];
+      }
+   }
+   if (o.rec_a) {
+      const int64_t r = ((int64_t)vx * pr.nz + vz) * pr.ny + vy;
+      if constexpr (REC16)
+         store_record_short(o, r, lab, vo[0], vo[1], vo[2]);
+      else
+         ((int4 *)o.rec_a)[r] =
+             make_int4(lab, __float_as_int(vo[0]), __float_as_int(vo[1]), __float_as_int(vo[2]));
+   }
+}
+
 // One workgroup per 8x8x32 tile, wave w owning the layers z = 4 l + w, l < 8
 // (slot = l * 64 + column, column = x * 8 + y within the tile): a segment's
 // candidates spread evenly over the waves, which meet at every chunk.  The tile's
@@ -927,7 +985,7 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
 // (run_batch) runs the exact FP64 test on 64 queued candidates at a time.
 // LSQ_IN_LDS: the layer tables fit the LDS copy; a compile-time choice keeps
 // the layer searches plain LDS reads
-template <typename SegT, bool LSQ_IN_LDS>
+template <typename SegT, bool LSQ_IN_LDS, bool REC16>
 __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
     k_voxelize(Params pr, const SegRec *__restrict__ segs, const float4 *__restrict__ seg_dir,
                const float4 *__restrict__ seg_clip,
@@ -935,8 +993,7 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
                const uint32_t *__restrict__ range, const int2 *__restrict__ tile_bundle,
                const BundleDev *__restrict__ bd,
                const double *__restrict__ layer_sq, const uint8_t *__restrict__ layer_model,
-               int only_label, int32_t *__restrict__ label, float *__restrict__ vector,
-               uint32_t *flags) {
+               Outputs out, uint32_t *flags) {
    __shared__ WaveState<SegT> wstate[WG / 64];
    __shared__ LdsSeg chunk[CHUNK];
    __shared__ uint32_t chunk_gidx[CHUNK]; // global segment index of each record
@@ -1188,6 +1245,8 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
    }
 
    wave_sync();
+   const bool need_vec = out.vector != nullptr || out.rec_a != nullptr;
+   uint64_t nonzero = 0; // wave-uniform: some voxel of the wave got a label != 0
    if (short_seg && free_mode) {
       // Results of the free mode, eight columns of eight voxels per pass: the
       // label and layer model come from the key; a 'p' voxel reads its
@@ -1200,7 +1259,7 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
          const bool hit = (uint32_t)(key >> 32) != 0x7F800000u;
          los[j] = hit ? (uint32_t)key : 0u;
          sgs[j] = -1;
-         if (!only_label && ((los[j] >> 14) & 3u) != 0u)
+         if (need_vec && ((los[j] >> 14) & 3u) != 0u)
             sgs[j] = (int32_t)list[beg + (0x10000u - (los[j] >> 16)) - 1u];
       }
 #pragma unroll
@@ -1208,11 +1267,10 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
          const int col = j * 8 + (lane >> 3), zz = lane & 7;
          const int vx = xt + (col >> 3), vy = yt + (col & 7), vz = zw + 4 * zz;
          if (vx < pr.nx && vy < pr.ny && vz < pr.nz) {
-            const int64_t ind = ((int64_t)vx * pr.ny + vy) * pr.nz + vz;
             const uint32_t lo = los[j];
-            label[ind] = (int32_t)(lo & 0x3FFFu);
-            if (!only_label) {
-               float vo[3] = {0.0f, 0.0f, 0.0f};
+            const int32_t lab = (int32_t)(lo & 0x3FFFu);
+            float vo[3] = {0.0f, 0.0f, 0.0f};
+            if (need_vec) {
                const uint32_t m2 = (lo >> 14) & 3u;
                if (m2 == 1u) {
                   const float4 d = seg_dir[sgs[j]];
@@ -1228,12 +1286,12 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
                   const Closest cp = closest(pt, E);
                   unit_to_float(pt[0] - cp.c[0], pt[1] - cp.c[1], pt[2] - cp.c[2], vo);
                }
-               vector[3 * ind + 0] = vo[0];
-               vector[3 * ind + 1] = vo[1];
-               vector[3 * ind + 2] = vo[2];
             }
+            write_voxel<REC16>(out, pr, vx, vy, vz, lab, vo, flags);
+            nonzero |= __ballot(lab != 0);
          }
       }
+      note_nonzero(nonzero, flags);
       return;
    }
    // Results of the ordered mode, eight columns of eight voxels per pass.  The
@@ -1260,7 +1318,6 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
       if (j < 7 && sg1 >= 0) nxt = load_winner(segs[sg1], seg_dir[sg1]);
       const int vx = xt + (col >> 3), vy = yt + (col & 7), vz = zw + 4 * zz;
       if (vx < pr.nx && vy < pr.ny && vz < pr.nz) {
-         const int64_t ind = ((int64_t)vx * pr.ny + vy) * pr.nz + vz;
          const unsigned long long key = W.key[slot];
          int32_t lab = (int32_t)(uint32_t)(key & 0x7FFFFFFFull);
          if (short_seg) {
@@ -1276,9 +1333,8 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
             lab = layer_label(__uint_as_float((uint32_t)(key >> 32)), (float)(rr * rr), lsq,
                               cur.layer_off, cur.bundle, max_layer);
          }
-         label[ind] = lab;
-         if (!only_label) {
-            float vo[3] = {0.0f, 0.0f, 0.0f};
+         float vo[3] = {0.0f, 0.0f, 0.0f};
+         if (need_vec) {
             if (sg >= 0) {
                // a float ratio just above the last scale^2 gives the past-the-end
                // layer index; the padded model table repeats the outermost
@@ -1295,15 +1351,15 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
                   unit_to_float(pt[0] - cp.c[0], pt[1] - cp.c[1], pt[2] - cp.c[2], vo);
                }
             }
-            vector[3 * ind + 0] = vo[0];
-            vector[3 * ind + 1] = vo[1];
-            vector[3 * ind + 2] = vo[2];
          }
+         write_voxel<REC16>(out, pr, vx, vy, vz, lab, vo, flags);
+         nonzero |= __ballot(lab != 0);
       }
       cur = nxt;
       sg = sg1;
       sg1 = sg2;
    }
+   note_nonzero(nonzero, flags);
 }
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
@@ -1313,9 +1369,9 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
 // in the middle of the call (a host round trip that leaves the GPU idle);
 // `overflow` reports a count above it (the outputs are then incomplete and
 // the caller runs again without the hint).
-Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label,
-                int32_t *label, float *vector, double *props, uint32_t *warn, hipStream_t st,
-                bool use_hint, bool &overflow) {
+Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outputs &out,
+                double *props, uint32_t *warn, int32_t *label_max, hipStream_t st, bool use_hint,
+                bool &overflow) {
    overflow = false;
    if (!vol || !fbs) return make_status(FPL_E_INVALID, "null argument");
    const int64_t nx = vol->dim[0], ny = vol->dim[1], nz = vol->dim[2];
@@ -1326,8 +1382,7 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_la
       return make_status(FPL_E_INVALID, "volume dimension >= 2^24 is not supported");
    if (!(vol->voxel_size > 0))
       return make_status(FPL_E_INVALID, "voxel_size <= 0: %f", vol->voxel_size);
-   if (!label || (!only_label && !vector))
-      return make_status(FPL_E_INVALID, "output pointers are null");
+   if (!out.label && !out.rec_a) return make_status(FPL_E_INVALID, "output pointers are null");
    if (fbs->n_bundles < 0 || fbs->n_fibers < 0 || fbs->n_points < 0)
       return make_status(FPL_E_INVALID, "negative sizes");
    if (fbs->n_points >= ((int64_t)1 << 31))
@@ -1406,6 +1461,7 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_la
    pr.scale = 1.0 / vol->voxel_size;
    pr.max_layer = max_layer;
    pr.n_lsq = (int32_t)lsq.size();
+   pr.n_props = (int32_t)n_props;
    pr.ntx = (int32_t)((nx + TX - 1) / TX);
    pr.nty = (int32_t)((ny + TY - 1) / TY);
    pr.ntz = (int32_t)((nz + TZ - 1) / TZ);
@@ -1520,22 +1576,33 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_la
                          (uint32_t *)rng, tile_bundle);
    }
    const bool lsq_lds = pr.n_lsq <= LSQ_LDS;
+   const bool rec16 = out.rec_a && out.rec_storage != rec::ST_F32;
+   // the voxelizer variant: 16- or 32-bit winner positions, layer tables in LDS
+   // or not, 16-bit record storage or not
+   auto voxelize = [&](bool short_pos) {
+#define FPL_VOX(SEG, LDS, R16)                                                                    \
+   hipLaunchKernelGGL((k_voxelize<SEG, LDS, R16>), dim3((unsigned)n_tiles), dim3(WG), 0, st, pr,  \
+                      (const SegRec *)segs, seg_dir, seg_clip, (const uint32_t *)vb,               \
+                      (const uint32_t *)rng, tile_bundle, d_bd, d_lsq, d_lm, out, d_flags)
+#define FPL_VOX2(SEG, LDS)                                                                        \
+   do {                                                                                           \
+      if (rec16) FPL_VOX(SEG, LDS, true); else FPL_VOX(SEG, LDS, false);                          \
+   } while (0)
+      if (short_pos) {
+         if (lsq_lds) FPL_VOX2(uint16_t, true); else FPL_VOX2(uint16_t, false);
+      } else {
+         if (lsq_lds) FPL_VOX2(int32_t, true); else FPL_VOX2(int32_t, false);
+      }
+#undef FPL_VOX2
+#undef FPL_VOX
+   };
    // the 16-bit build packs a winner's label into 14 bits of its key (labels
    // reach n_props when a float ratio rounds past the last layer)
    const bool short_labels = n_props <= 0x3FFF;
    bool rerun = !short_labels;
    if (short_labels) {
       KernelTimer kt("voxelize", st);
-      if (lsq_lds)
-         hipLaunchKernelGGL((k_voxelize<uint16_t, true>), dim3((unsigned)n_tiles), dim3(WG), 0,
-                            st, pr, (const SegRec *)segs, seg_dir, seg_clip, (const uint32_t *)vb,
-                            (const uint32_t *)rng, tile_bundle, d_bd, d_lsq, d_lm, only_label,
-                            label, vector, d_flags);
-      else
-         hipLaunchKernelGGL((k_voxelize<uint16_t, false>), dim3((unsigned)n_tiles), dim3(WG), 0,
-                            st, pr, (const SegRec *)segs, seg_dir, seg_clip, (const uint32_t *)vb,
-                            (const uint32_t *)rng, tile_bundle, d_bd, d_lsq, d_lm, only_label,
-                            label, vector, d_flags);
+      voxelize(true);
       kt.stop();
       FPL_HIP(hipGetLastError());
       FPL_HIP(hipMemcpyAsync(stage, d_flags, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -1556,16 +1623,7 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_la
       // volume with 32-bit winner indices (every voxel is rewritten; warnings
       // are an OR)
       KernelTimer kt2("voxelize", st);
-      if (lsq_lds)
-         hipLaunchKernelGGL((k_voxelize<int32_t, true>), dim3((unsigned)n_tiles), dim3(WG), 0, st,
-                            pr, (const SegRec *)segs, seg_dir, seg_clip, (const uint32_t *)vb,
-                            (const uint32_t *)rng, tile_bundle, d_bd, d_lsq, d_lm, only_label,
-                            label, vector, d_flags);
-      else
-         hipLaunchKernelGGL((k_voxelize<int32_t, false>), dim3((unsigned)n_tiles), dim3(WG), 0,
-                            st, pr, (const SegRec *)segs, seg_dir, seg_clip, (const uint32_t *)vb,
-                            (const uint32_t *)rng, tile_bundle, d_bd, d_lsq, d_lm, only_label,
-                            label, vector, d_flags);
+      voxelize(false);
       kt2.stop();
       FPL_HIP(hipGetLastError());
       FPL_HIP(hipMemcpyAsync(stage, d_flags, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -1581,15 +1639,20 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_la
       }
    }
    if (warn) *warn = *(uint32_t *)stage;
+   if (label_max) {
+      // an upper bound of the labels written: n_props only in the past-the-end
+      // case, 0 when every voxel is 0
+      const uint32_t f3 = ((const uint32_t *)stage)[3];
+      *label_max = (int32_t)((f3 & 1u) ? n_props : ((f3 & 2u) ? n_props - 1 : 0));
+   }
    return Status();
 }
 
-Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, int only_label, int32_t *label,
-           float *vector, double *props, uint32_t *warn, hipStream_t st) {
+Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outputs &out,
+           double *props, uint32_t *warn, int32_t *label_max, hipStream_t st) {
    bool overflow = false;
-   FPL_TRY(run_once(vol, fbs, only_label, label, vector, props, warn, st, true, overflow));
-   if (overflow)
-      FPL_TRY(run_once(vol, fbs, only_label, label, vector, props, warn, st, false, overflow));
+   FPL_TRY(run_once(vol, fbs, out, props, warn, label_max, st, true, overflow));
+   if (overflow) FPL_TRY(run_once(vol, fbs, out, props, warn, label_max, st, false, overflow));
    return Status();
 }
 
@@ -1600,7 +1663,52 @@ extern "C" int fpl_generate(const fpl_volume *vol, const fpl_fiber_bundles *fbs,
                             int32_t *label, float *vector, double *props, uint32_t *warn,
                             void *stream, char *msg, size_t msg_len) {
    if (warn) *warn = 0;
-   fpl::Status s = fpl::gen::run(vol, fbs, only_label, label, vector, props, warn,
-                                 (hipStream_t)stream);
+   fpl::Status s;
+   if (!label || (!only_label && !vector)) {
+      s = fpl::make_status(FPL_E_INVALID, "output pointers are null");
+   } else {
+      const fpl::gen::Outputs out{label, only_label ? nullptr : vector, nullptr, nullptr, 0};
+      s = fpl::gen::run(vol, fbs, out, props, warn, nullptr, (hipStream_t)stream);
+   }
+   return fpl::finish(s, msg, msg_len);
+}
+
+extern "C" int fpl_generate_fields(const fpl_volume *vol, const fpl_fiber_bundles *fbs,
+                                   int32_t *label, float *vector, double *props,
+                                   fpl_fields *fields, uint32_t *warn, void *stream, char *msg,
+                                   size_t msg_len) {
+   if (warn) *warn = 0;
+   fpl::Status s;
+   do {
+      if (!vol || !fields) {
+         s = fpl::make_status(FPL_E_INVALID, "null argument");
+         break;
+      }
+      const int32_t st = fields->storage;
+      if (st < 0 || st > 2) {
+         s = fpl::make_status(FPL_E_INVALID,
+                              "field_storage must be 0 (f32), 1 (f16) or 2 (angle16)");
+         break;
+      }
+      if (fields->records && st == fpl::rec::ST_ANGLE16 && !fields->codes) {
+         s = fpl::make_status(FPL_E_INVALID, "the angle16 storage needs the codes array");
+         break;
+      }
+      if (fbs && fields->records) {
+         // labels reach n_props when a float ratio rounds past the last layer
+         const int64_t np = fpl_num_properties(fbs);
+         if (np > fpl::rec::label_limit(st)) {
+            s = fpl::make_status(FPL_E_INVALID, "label %lld does not fit the %s field storage",
+                                 (long long)np, st == fpl::rec::ST_F16 ? "16-bit" : "15-bit");
+            break;
+         }
+      }
+      for (int k = 0; k < 3; k++) fields->dim[k] = vol->dim[k];
+      const fpl::gen::Outputs out{label, vector, fields->records, fields->codes, st};
+      int32_t lmax = 0;
+      s = fpl::gen::run(vol, fbs, out, props, warn, &lmax, (hipStream_t)stream);
+      fields->label_min = 0;
+      fields->label_max = lmax;
+   } while (false);
    return fpl::finish(s, msg, msg_len);
 }

@@ -28,6 +28,7 @@
 #include <hip/hip_fp16.h>
 
 #include "fpl_internal.h"
+#include "fpl_records.h"
 #include "fpl_sim.h"
 
 namespace fpl {
@@ -80,19 +81,13 @@ __device__ __forceinline__ void two_sum(float &hi, float &lo, float b) {
    lo += err;
 }
 
-// The march reads a packed copy of the two fields in [x][z][y] order, so that
-// adjacent lanes (adjacent rays in y) read adjacent records: every load of a
-// wave is one contiguous access instead of 64 scattered ones.  Three storages:
-//   ST_F32      16-byte record: int32 label, float axis x, y, z
-//   ST_F16       8-byte record: uint16 label, fp16 axis x, y, z; the axis is
-//                canonicalised to z >= 0 (the retardance and the interpolation
-//                are invariant under v -> -v) and the sign bit of z carries the
-//                reference's "not normalized" test of the source axis; axes that
-//                pass it are renormalised after decoding
-//   ST_ANGLE16   6 bytes per voxel in two arrays: uint16 label (bit 15 = "not
-//                normalized") and the axis as 16-bit polar / azimuth codes on the
-//                upper hemisphere, polar code 0xffff = zero axis
-enum Storage { ST_F32 = 0, ST_F16 = 1, ST_ANGLE16 = 2 };
+// The march reads the fields as [x][z][y] records (the storages of
+// fpl_records.h): the 64 rays of a wave read one contiguous run per z-step.
+using rec::ST_F32;
+using rec::ST_F16;
+using rec::ST_ANGLE16;
+using rec::ANG_THETA;
+using rec::ANG_PHI;
 
 struct Vox {
    int32_t lab;
@@ -100,8 +95,6 @@ struct Vox {
    bool nn; // source axis failed |v|^2 - 1 <= 1e-5 (precision storages only)
 };
 
-constexpr double ANG_THETA = 1.5707963267948966 / 65534.0; // polar step
-constexpr double ANG_PHI = 6.283185307179586 / 65536.0;    // azimuth step
 
 template <typename Idx, int ST> struct Field {
    const void *a; // ST_F32: int4[], ST_F16: uint2[], ST_ANGLE16: uint16 labels
@@ -951,42 +944,6 @@ __global__ void __launch_bounds__(WG)
 // Packs label[x][y][z] + vector[x][y][z][3] into the march's [x][z][y]
 // storage through a 32 x 32 (y, z) LDS tile, and reduces the label min / max
 // of CheckInput (simulator.cpp:149-167) on the way.
-__device__ __forceinline__ void encode(int ST, int32_t l, float x, float y, float z, int4 &out) {
-   if (ST == ST_F32) {
-      out = make_int4(l, __float_as_int(x), __float_as_int(y), __float_as_int(z));
-      return;
-   }
-   // "optical_axis not normalized" test of the reference, in double
-   // (simulator.cpp:298): length2 of the float components minus one > 1e-5
-   const double n2 = (double)x * x + (double)y * y + (double)z * z;
-   const bool nn = n2 - 1 > 1e-5;
-   if (z < 0.0f) { // canonical orientation: z >= 0
-      x = -x;
-      y = -y;
-      z = -z;
-   }
-   z = fabsf(z);
-   if (ST == ST_F16) {
-      const uint32_t hx = __half_as_ushort(__float2half_rn(x));
-      const uint32_t hy = __half_as_ushort(__float2half_rn(y));
-      const uint32_t hz = __half_as_ushort(__float2half_rn(z)) | (nn ? 0x8000u : 0u);
-      out = make_int4((int)(((uint32_t)l & 0xffffu) | (hx << 16)), (int)(hy | (hz << 16)), 0, 0);
-      return;
-   }
-   uint32_t code;
-   if (x == 0.0f && y == 0.0f && z == 0.0f) {
-      code = 0xffffu;
-   } else {
-      const double th = atan2(sqrt((double)x * x + (double)y * y), (double)z); // [0, pi/2]
-      const double ph = atan2((double)y, (double)x);                          // [-pi, pi]
-      uint32_t tc = (uint32_t)rint(th / ANG_THETA);
-      tc = tc > 65534u ? 65534u : tc;
-      const uint32_t pcode = ((uint32_t)rint((ph + 3.141592653589793) / ANG_PHI)) & 0xffffu;
-      code = tc | (pcode << 16);
-   }
-   out = make_int4((int)(((uint32_t)l & 0x7fffu) | (nn ? 0x8000u : 0u)), (int)code, 0, 0);
-}
-
 template <int ST>
 __global__ void __launch_bounds__(256)
     k_pack(const int32_t *__restrict__ label, const float *__restrict__ vec, int64_t nx,
@@ -1012,7 +969,7 @@ __global__ void __launch_bounds__(256)
             lo = l < lo ? l : lo;
             hi = l > hi ? l : hi;
             const float3 v = *(const float3 *)(vec + 3 * i);
-            encode(ST, l, v.x, v.y, v.z, tile[zz][yy]);
+            rec::encode(ST, l, v.x, v.y, v.z, tile[zz][yy]);
          }
       }
       __syncthreads();
@@ -1021,15 +978,7 @@ __global__ void __launch_bounds__(256)
          const int64_t y = y0 + yy, z = z0 + zz;
          if (y < ny && z < nz) {
             const int64_t o = (x * nz + z) * ny + y;
-            const int4 e = tile[zz][yy];
-            if (ST == ST_F32) {
-               ((int4 *)rec_a)[o] = e;
-            } else if (ST == ST_F16) {
-               ((uint2 *)rec_a)[o] = make_uint2((uint32_t)e.x, (uint32_t)e.y);
-            } else {
-               ((uint16_t *)rec_a)[o] = (uint16_t)e.x;
-               rec_b[o] = (uint32_t)e.y;
-            }
+            rec::store(ST, rec_a, rec_b, o, tile[zz][yy]);
          }
       }
       __syncthreads();
@@ -1148,9 +1097,7 @@ TiltDev make_tilt(const fpl_setup &S, const int64_t dim[3], double theta, double
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
-size_t record_bytes(int storage) {
-   return storage == ST_F32 ? 16 : (storage == ST_F16 ? 8 : 2);
-}
+size_t record_bytes(int storage) { return rec::record_bytes(storage); }
 
 Status prepare(const fpl_setup *setup, const int64_t dim[3], const double *props, int64_t n_props,
                const double *tilts, int64_t n_tilts, Block &B) {
@@ -1351,6 +1298,75 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
    return Status();
 }
 
+// fpl_simulate on march fields (fpl_records.h layout): the setup checks, then
+// the label range against the property table (CheckInput before the march, as
+// the reference), then the march.
+Status run_fields(const fpl_setup *setup, const fpl_fields *Fd, const double *props,
+                  int64_t n_props, const double *tilts, int64_t n_tilts, double *images,
+                  uint32_t *warn, hipStream_t st) {
+   if (!setup || !Fd) return make_status(FPL_E_INVALID, "null argument");
+   fpl_setup S = *setup;
+   S.field_storage = Fd->storage;
+   Block B;
+   FPL_TRY(prepare(&S, Fd->dim, props, n_props, tilts, n_tilts, B));
+   if (!Fd->records || !images || (B.storage == ST_ANGLE16 && !Fd->codes))
+      return make_status(FPL_E_INVALID, "null array");
+   FPL_TRY(check_labels(Fd->label_min, Fd->label_max, n_props, B.storage));
+
+   void *small, *stage, *flags;
+   FPL_TRY(ws_get(st, WS_SMALL, B.bytes.size(), &small));
+   FPL_TRY(ws_get(st, WS_FLAGS, 64, &flags));
+   FPL_TRY(host_stage(st, B.bytes.size() + 64, &stage));
+   FPL_HIP(hipStreamSynchronize(st));
+   memcpy(stage, B.bytes.data(), B.bytes.size());
+   FPL_HIP(hipMemcpyAsync(small, stage, B.bytes.size(), hipMemcpyHostToDevice, st));
+   uint32_t *d_flags = (uint32_t *)flags;
+   clear_flags(d_flags, (int32_t *)((char *)flags + 16), st);
+   KernelTimer kt("march", st);
+   march(B, small, Fd->records, Fd->codes, images, d_flags, st);
+   kt.stop();
+   FPL_HIP(hipGetLastError());
+   FPL_HIP(hipMemcpyAsync(stage, flags, 4, hipMemcpyDeviceToHost, st));
+   FPL_HIP(hipStreamSynchronize(st));
+   kt.collect();
+   const uint32_t f = *(uint32_t *)stage;
+   // the range was checked: a label outside it means the fields changed since
+   if (f & 0x80000000u) return make_status(FPL_E_LABEL_MAX, "label exceed properties.size()");
+   if (warn) *warn = f & 0x7fffffffu;
+   return Status();
+}
+
+// Packs caller-owned C-order fields into march fields and records their
+// label range (the min / max of CheckInput, simulator.cpp:149-167).
+Status pack_fields(const int64_t dim[3], const int32_t *label, const float *vector,
+                   fpl_fields *Fd, hipStream_t st) {
+   if (!dim || !Fd) return make_status(FPL_E_INVALID, "null argument");
+   if (Fd->storage < 0 || Fd->storage > 2)
+      return make_status(FPL_E_INVALID, "field_storage must be 0 (f32), 1 (f16) or 2 (angle16)");
+   for (int k = 0; k < 3; k++)
+      if (dim[k] <= 0 || dim[k] >= (1 << 30))
+         return make_status(FPL_E_INVALID, "bad field dimensions");
+   if (!label || !vector || !Fd->records || (Fd->storage == ST_ANGLE16 && !Fd->codes))
+      return make_status(FPL_E_INVALID, "null array");
+   void *stage, *flags;
+   FPL_TRY(ws_get(st, WS_FLAGS, 64, &flags));
+   FPL_TRY(host_stage(st, 64, &stage));
+   uint32_t *d_flags = (uint32_t *)flags;
+   int32_t *d_mm = (int32_t *)((char *)flags + 16);
+   clear_flags(d_flags, d_mm, st);
+   KernelTimer kp("pack", st);
+   pack(Fd->storage, dim, label, vector, Fd->records, Fd->codes, d_mm, st);
+   kp.stop();
+   FPL_HIP(hipGetLastError());
+   FPL_HIP(hipMemcpyAsync(stage, d_mm, 8, hipMemcpyDeviceToHost, st));
+   FPL_HIP(hipStreamSynchronize(st));
+   kp.collect();
+   for (int k = 0; k < 3; k++) Fd->dim[k] = dim[k];
+   Fd->label_min = ((int32_t *)stage)[0];
+   Fd->label_max = ((int32_t *)stage)[1];
+   return Status();
+}
+
 // z-march trip count of one tilt (simulator.cpp:259-261): the same for every
 // ray, which all start at z = 0 + step_z / 2
 int64_t tilt_steps(const TiltDev &T, int64_t nz) {
@@ -1464,5 +1480,21 @@ extern "C" int fpl_simulate_adjoint(const fpl_setup *setup, const int64_t dim[3]
                                     char *msg, size_t msg_len) {
    fpl::Status s = fpl::sim::adjoint(setup, dim, label, vector, props, n_props, tilts, n_tilts,
                                      grad_images, grad_props, grad_vector, (hipStream_t)stream);
+   return fpl::finish(s, msg, msg_len);
+}
+
+extern "C" int fpl_simulate_fields(const fpl_setup *setup, const fpl_fields *fields,
+                                   const double *props, int64_t n_props, const double *tilts,
+                                   int64_t n_tilts, double *images, uint32_t *warn, void *stream,
+                                   char *msg, size_t msg_len) {
+   if (warn) *warn = 0;
+   fpl::Status s = fpl::sim::run_fields(setup, fields, props, n_props, tilts, n_tilts, images,
+                                        warn, (hipStream_t)stream);
+   return fpl::finish(s, msg, msg_len);
+}
+
+extern "C" int fpl_pack_fields(const int64_t dim[3], const int32_t *label, const float *vector,
+                               fpl_fields *fields, void *stream, char *msg, size_t msg_len) {
+   fpl::Status s = fpl::sim::pack_fields(dim, label, vector, fields, (hipStream_t)stream);
    return fpl::finish(s, msg, msg_len);
 }

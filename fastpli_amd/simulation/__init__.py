@@ -1,5 +1,5 @@
 from .autograd import simulate_stack
 from .serving import SimulationServer
-from .simpli import Simpli
+from .simpli import MarchFields, Simpli
 
-__all__ = ['Simpli', 'SimulationServer', 'simulate_stack']
+__all__ = ['MarchFields', 'Simpli', 'SimulationServer', 'simulate_stack']

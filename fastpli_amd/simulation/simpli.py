@@ -9,6 +9,9 @@ of the pybind11/OpenMP core; there is no CPU fallback.
 Extensions of this package (not in the reference):
   * ``generate_tissue_device`` / ``run_simulation`` with torch tensors on the
     GPU keep the fields resident in HBM (no host round trip);
+  * ``generate_tissue_fields`` / ``pack_fields`` give ``MarchFields``, the
+    fields in the march's own record layout, and ``run_simulation_fields``
+    simulates them without repacking (generation writes the records itself);
   * ``run_simulation_tilts`` runs several tilts in one device call;
   * ``count_voxel_steps`` gives the exact Mueller voxel-step count of a run.
 """
@@ -34,6 +37,45 @@ def _is_tensor(x):
     except ImportError:  # pragma: no cover
         return False
     return isinstance(x, torch.Tensor)
+
+
+class MarchFields:
+    """Label and optical-axis fields in the ray march's record layout, on the GPU.
+
+    Made by ``Simpli.generate_tissue_fields`` (the voxelizer writes the records
+    as it writes the tissue) or ``Simpli.pack_fields`` (from C-order fields),
+    consumed by ``Simpli.run_simulation_fields``.  A snapshot: later changes of
+    the arrays it was made from do not reach it.
+
+    storage: 'f32' (16 B/voxel, exact), 'f16' (8 B) or 'angle16' (6 B).
+    label_min / label_max: bounds of the labels it holds, checked against the
+    property table before every simulation (the reference's CheckInput).
+    """
+
+    def __init__(self, dim, storage, buffer):
+        self.dim = tuple(int(d) for d in dim)
+        self.storage = storage
+        self.buffer = buffer
+        self.label_min = 0
+        self.label_max = 0
+
+    @classmethod
+    def empty(cls, dim, storage, device):
+        torch = _torch()
+        nbytes, _ = _lib.fields_layout(dim, storage)
+        return cls(dim, storage, torch.empty(nbytes, dtype=torch.uint8, device=device))
+
+    def struct(self):
+        return _lib.fields_struct(self.buffer, self.dim, self.storage, self.label_min,
+                                  self.label_max)
+
+    def update(self, st):
+        self.label_min = int(st.label_min)
+        self.label_max = int(st.label_max)
+
+    @property
+    def nbytes(self):
+        return int(self.buffer.numel())
 
 
 class Simpli(_SimpliBase):
@@ -79,33 +121,77 @@ class Simpli(_SimpliBase):
             raise ValueError(f'dim.global[any] <= 0: [{dim[0]},{dim[1]},{dim[2]}]')
         return dim
 
+    def _device_fiber_model(self, dev):
+        torch = _torch()
+        points, fiber_ptr, bundle_ptr, layer_ptr, table = self._fiber_bundles.to_flat()
+        points_d = torch.from_numpy(points).to(dev)
+        fiber_ptr_d = torch.from_numpy(fiber_ptr).to(dev)
+        return points_d, fiber_ptr_d, bundle_ptr, layer_ptr, table
+
+    def _generate(self, only_tissue, fields, with_arrays=True):
+        self._check_volume_input()
+        self._check_generation_input()
+        dim = self._dim_checked()
+        torch = _torch()
+        dev = torch.device('cuda', torch.cuda.current_device())
+        model = self._device_fiber_model(dev)
+        tissue = optical_axis = None
+        if with_arrays:
+            tissue = torch.empty(dim, dtype=torch.int32, device=dev)
+            if not only_tissue:
+                optical_axis = torch.empty(dim + [3], dtype=torch.float32, device=dev)
+        st = fields.struct() if fields is not None else _lib.FplFields()
+        props, warn = _lib.generate_fields(dim, self._dim_origin, self._voxel_size, *model,
+                                           tissue, optical_axis, st)
+        if fields is not None:
+            fields.update(st)
+        props[0, 1] = self._background_absorption
+        _lib.raise_warnings(warn)
+        if st.label_max == 0:
+            warnings.warn(
+                'All labels are 0. Usually this means, that the VOI contains ' +
+                'no fiber_bundles or that the voxel_size is to large.', UserWarning)
+        return tissue, optical_axis, props
+
     def generate_tissue_device(self, only_tissue=False):
         """Voxelize the fiber bundles into device tensors.
 
         Returns (tissue int32 (nx, ny, nz), optical_axis float32 (nx, ny, nz, 3)
         or None, tissue_properties float64 ndarray (n, 2)).
         """
+        return self._generate(only_tissue, None)
+
+    def generate_tissue_fields(self, field_storage=None, with_arrays=True):
+        """Voxelize the fiber bundles into march fields (and the device arrays).
+
+        The voxelizer writes every voxel's march record as it writes the tissue,
+        so ``run_simulation_fields`` needs no repacking pass.  field_storage
+        defaults to the ``field_storage`` property.  Returns (tissue, optical_axis,
+        tissue_properties, fields); tissue and optical_axis are None when
+        with_arrays is False (only the records are written).
+        """
         self._check_volume_input()
         self._check_generation_input()
-        dim = self._dim_checked()
+        storage = self._field_storage if field_storage is None else field_storage
+        if storage not in _lib.STORAGE:
+            raise ValueError('field_storage must be \'f32\', \'f16\' or \'angle16\'')
         torch = _torch()
         dev = torch.device('cuda', torch.cuda.current_device())
-        points, fiber_ptr, bundle_ptr, layer_ptr, table = self._fiber_bundles.to_flat()
-        points_d = torch.from_numpy(points).to(dev)
-        fiber_ptr_d = torch.from_numpy(fiber_ptr).to(dev)
-        tissue = torch.empty(dim, dtype=torch.int32, device=dev)
-        optical_axis = None if only_tissue else torch.empty(dim + [3], dtype=torch.float32,
-                                                            device=dev)
-        props, warn = _lib.generate(dim, self._dim_origin, self._voxel_size, points_d,
-                                    fiber_ptr_d, bundle_ptr, layer_ptr, table, only_tissue,
-                                    tissue, optical_axis)
-        props[0, 1] = self._background_absorption
-        _lib.raise_warnings(warn)
-        if not bool(torch.any(tissue != 0)):
-            warnings.warn(
-                'All labels are 0. Usually this means, that the VOI contains ' +
-                'no fiber_bundles or that the voxel_size is to large.', UserWarning)
-        return tissue, optical_axis, props
+        fields = MarchFields.empty(self._dim_checked(), storage, dev)
+        tissue, optical_axis, props = self._generate(False, fields, with_arrays)
+        return tissue, optical_axis, props, fields
+
+    def pack_fields(self, tissue, optical_axis, field_storage=None):
+        """March fields from C-order fields (numpy arrays or device tensors)."""
+        storage = self._field_storage if field_storage is None else field_storage
+        if storage not in _lib.STORAGE:
+            raise ValueError('field_storage must be \'f32\', \'f16\' or \'angle16\'')
+        dim, tissue_, axis_, _ = self._prepare_fields(tissue, optical_axis, np.zeros((1, 2)))
+        fields = MarchFields.empty(dim, storage, tissue_.device)
+        st = fields.struct()
+        _lib.pack_fields(dim, tissue_, axis_, st)
+        fields.update(st)
+        return fields
 
     def generate_tissue(self, only_tissue=False):
         """Generation of the discretized tissue.
@@ -204,6 +290,36 @@ class Simpli(_SimpliBase):
         if as_tensor:
             return images
         return images.cpu().numpy()
+
+    def run_simulation_fields(self, fields, tissue_properties, tilts):
+        """Simulate several tilts of march fields in one device call.
+
+        Returns images float64 (n_tilts, nx, ny, n_rho) as a torch tensor on the
+        GPU.  The fields' storage applies (the field_storage property does not).
+        """
+        self._print('Run simulation')
+        self._check_volume_input()
+        if not isinstance(fields, MarchFields):
+            raise TypeError('fields must be MarchFields')
+        if tuple(self._dim_checked()) != fields.dim:
+            raise ValueError('dim_.local.x() * dim_.local.y() * dim_.local.z() != '
+                             'label_field_.size()')
+        tissue_properties = np.array(tissue_properties)
+        if tissue_properties.ndim != 2:
+            raise ValueError('tissue_properties.ndim !=2')
+        if tissue_properties.shape[1] != 2:
+            raise ValueError('tissue_properties.shape[1] != 2')
+        setup, _keep = self._setup()
+        torch = _torch()
+        tilts = np.array(tilts, dtype=np.float64, ndmin=2).reshape(-1, 2)
+        dim = fields.dim
+        images = torch.empty((tilts.shape[0], dim[0], dim[1], setup.n_rho), dtype=torch.float64,
+                             device=fields.buffer.device)
+        warn = _lib.simulate_fields(setup, fields.struct(), tissue_properties, tilts, images)
+        _lib.raise_warnings(warn)
+        if bool(torch.any(images < 0)):
+            raise ValueError('intensity < 0 detected')
+        return images
 
     def run_simulation(self, tissue, optical_axis, tissue_properties, theta, phi):
         """Simulate the PLI images of one tilt (theta, phi).

@@ -131,6 +131,60 @@ int fpl_simulate(const fpl_setup *setup, const int64_t dim[3],
                  int64_t n_tilts, double *images, uint32_t *warn, void *stream,
                  char *msg, size_t msg_len);
 
+/* ------------------------------------------------------- march fields ---
+ * The label and optical-axis fields in the march's own layout: [x][z][y]
+ * records, so that the rays of a wave (adjacent y) read one contiguous run per
+ * z-step.  No counterpart in the reference, whose simulator reads the C-order
+ * arrays of run_simulation (simulator_module.cpp:56-93); this handle lets a
+ * caller that generates a tissue and then simulates it skip the repacking pass
+ * (fpl_generate_fields writes the records from the voxelizer), and lets a
+ * caller that simulates one tissue many times pack it once (fpl_pack_fields).
+ *
+ * Storage (field_storage):
+ *   0 f32      int32 label + float32 axis, 16 B/voxel (exact input)
+ *   1 f16      uint16 label + fp16 axis, 8 B/voxel
+ *   2 angle16  uint16 label + 16-bit polar/azimuth codes, 2 + 4 B/voxel
+ * The caller allocates the arrays: `records` takes fpl_record_bytes(storage)
+ * bytes per voxel, `codes` (angle16 only) 4 bytes per voxel.
+ * fpl_fields_bytes(dim, storage) is the size of one allocation holding both,
+ * `codes` at byte offset fpl_fields_codes_offset(dim, storage). */
+typedef struct fpl_fields {
+   int64_t dim[3];
+   int32_t storage;
+   int32_t label_min; /* label range of the fields: set by the call that fills */
+   int32_t label_max; /* them, checked against the property table by          */
+   int32_t pad_;      /* fpl_simulate_fields (CheckInput, simulator.cpp:149-167) */
+   void *records;     /* DEVICE */
+   uint32_t *codes;   /* DEVICE, angle16 only, else NULL */
+} fpl_fields;
+
+size_t fpl_record_bytes(int32_t field_storage);
+size_t fpl_fields_bytes(const int64_t dim[3], int32_t field_storage);
+size_t fpl_fields_codes_offset(const int64_t dim[3], int32_t field_storage);
+
+/* Tissue generation straight into march fields: fpl_generate whose voxelizer
+ * also writes every voxel's record (same labels and axes, bit for bit).
+ * label / vector (DEVICE, C order) may be NULL: then only the records are
+ * written.  fields->records / codes / storage are inputs; dim and the label
+ * range are set.  Labels must fit the storage (n_props <= 65535 for f16,
+ * <= 32767 for angle16). */
+int fpl_generate_fields(const fpl_volume *vol, const fpl_fiber_bundles *fbs,
+                        int32_t *label, float *vector, double *props, fpl_fields *fields,
+                        uint32_t *warn, void *stream, char *msg, size_t msg_len);
+
+/* Packs caller-owned C-order fields (label DEVICE int32 (dim...), vector
+ * DEVICE float (dim..., 3)) into march fields and records their label range. */
+int fpl_pack_fields(const int64_t dim[3], const int32_t *label, const float *vector,
+                    fpl_fields *fields, void *stream, char *msg, size_t msg_len);
+
+/* fpl_simulate on march fields: the same checks (the label range against the
+ * property table before the march), the same kernels and results; the
+ * setup's field_storage is ignored (the fields' storage applies). */
+int fpl_simulate_fields(const fpl_setup *setup, const fpl_fields *fields,
+                        const double *props, int64_t n_props, const double *tilts,
+                        int64_t n_tilts, double *images, uint32_t *warn, void *stream,
+                        char *msg, size_t msg_len);
+
 /* Reverse mode of fpl_simulate (no counterpart in the reference, which has no
  * derivatives): for L = sum(grad_images * images), with images what
  * fpl_simulate returns for the same arguments,
