@@ -78,9 +78,6 @@ def measured_traffic(kernel, storage):
     return None
 
 
-SERVE_SIZES = [(64, 64, 60), (128, 128, 120), (256, 256, 200), (512, 512, 300)]
-
-
 def adjoint_bench(reps=5):
     """C4 (one vMF bundle, 300 x 300 x 150 @ 0.2 um), NN, 5 tilts x 9 rotations:
     the forward march and the adjoint kernel (gradients w.r.t. the (dn, mu) table
@@ -118,87 +115,108 @@ def adjoint_bench(reps=5):
             'gradients': 'dn, mu per label; optical axis per voxel'}
 
 
-def serving_bench(n_requests, load, seed=0):
-    """Mixed-size request stream through SimulationServer (p50 / p99 latency).
+def serving_bench(n_requests, load, seed=0, pool=4):
+    """Serving stream: each request a whole simulation of its own fiber model.
 
-    Four registered tissues (crossing bundles at 0.2 um, fiber count scaled to
-    the cross-section): 64^2x60, 128^2x120, 256^2x200, 512^2x300 voxels.  Each
-    request picks a size uniformly, the 5-tilt set or the untilted view alone
-    (1:1), Slerp or NN (1:1), and scales the layers' dn and mu by U(0.5, 1.5);
-    9 rotations.  Arrivals are Poisson at `load` x the rate one request at a
-    time would sustain (mean service time measured first, each shape alone).
-    Latency is submit -> completion as the server measures it.
+    Volumes uniform over 64^2x60, 128^2x120, 256^2x200 and 512^2x300 voxels
+    (0.2 um); each request a seeded 1-3-bundle fiber model (synthetic.
+    serving_model: radii lognormal(1.0, 0.2) um, a geometry from a pool of
+    `pool` seeded models per size) with its layers' dn drawn from U[-0.006,
+    -0.001] and mu from U[1, 10] per request; 5 tilts or the untilted view
+    alone (1:1), 9 rotations, Slerp.  The server generates every request's
+    tissue in its slot and marches it (SimulationServer.submit_model).  Arrivals
+    are Poisson at `load` x the rate one request at a time would sustain (mean
+    service time of every (size, tilt set) measured alone first).  Latency is
+    submit -> completion as the server measures it.
     """
     import torch
     from fastpli_amd import synthetic
     from fastpli_amd.simulation import SimulationServer
     rng = np.random.default_rng(seed)
-    srv = SimulationServer(max_inflight=4)
-    tissues = []
-    for (nx, ny, nz) in SERVE_SIZES:
-        L, Z = nx * VOXEL, nz * VOXEL
-        n_f = max(8, int(round(2000 / (120.0 * 60.0) * L * Z)))
-        s = make_simpli([nx, ny, nz], synthetic.crossing_bundles(seed=seed + nx, n_fibers=n_f,
-                                                                  size_xy=L, size_z=Z))
-        with warnings.catch_warnings():
-            warnings.simplefilter('ignore')
-            lab, vec, props = s.generate_tissue_device()
-        tissues.append((srv.add_tissue(lab, vec), props, s))
-        del lab, vec
-    simplis = {}
-    for interp in ('Slerp', 'NN'):
-        s = make_simpli([8, 8, 8])
-        s.interpolate = interp
-        simplis[interp] = s
-    work_of = {(k, nt): sum(tissues[k][2].count_voxel_steps(t, p) for t, p in TILTS[:nt])
-               for k in range(len(SERVE_SIZES)) for nt in (1, 5)}
+    sizes = synthetic.SERVE_SIZES
+    geoms, simplis = [], []
+    for k, dim in enumerate(sizes):
+        entries = []
+        for p in range(pool):
+            fbs = synthetic.serving_model(1000 * seed + 10 * k + p, dim)
+            points, fptr, bptr, lptr, _ = fbs.to_flat()
+            pinned = torch.from_numpy(points).pin_memory().numpy()
+            entries.append((pinned, fptr, bptr, lptr))
+            if p == 0:
+                simplis.append(make_simpli(list(dim), fbs))
+        geoms.append(entries)
+    work_of = {(k, nt): sum(simplis[k].count_voxel_steps(t, p) for t, p in TILTS[:nt])
+               for k in range(len(sizes)) for nt in (1, 5)}
+
+    def layers(nb):
+        scale = np.tile([0.75, 1.0], nb)
+        dn = np.zeros(2 * nb)
+        dn[1::2] = rng.uniform(-0.006, -0.001, size=nb)
+        mu = rng.uniform(1.0, 10.0, size=2 * nb)
+        model = np.tile(np.array([ord('b'), ord('p')], dtype=np.uint8), nb)
+        return scale, dn, mu, model
 
     def make_request():
-        k = int(rng.integers(0, len(SERVE_SIZES)))
+        k = int(rng.integers(0, len(sizes)))
+        pts, fptr, bptr, lptr = geoms[k][int(rng.integers(0, pool))]
         tilts = TILTS if rng.uniform() < 0.5 else TILTS[:1]
-        interp = 'Slerp' if rng.uniform() < 0.5 else 'NN'
-        props = np.array(tissues[k][1], dtype=np.float64)
-        props[1:, 0] *= rng.uniform(0.5, 1.5)
-        props[1:, 1] *= rng.uniform(0.5, 1.5)
-        return k, tilts, interp, props
+        nb = len(bptr) - 1
+        return k, tilts, (pts, fptr, bptr, 2 * np.arange(nb + 1, dtype=np.int64), layers(nb))
 
-    # service time of each shape alone (warms the graph cache of every slot)
+    srv = SimulationServer(max_inflight=4)
+    # service time of each (size, tilt set) alone (warms every slot's buffers and graphs)
     service = {}
-    for k in range(len(SERVE_SIZES)):
-        for nt, tilts in ((5, TILTS), (1, TILTS[:1])):
-            for interp in ('Slerp', 'NN'):
-                times = []
-                for _ in range(6):
-                    t = srv.submit(tissues[k][0], simplis[interp], tissues[k][1], tilts)
+    for k in range(len(sizes)):
+        for nt in (5, 1):
+            times = []
+            for _ in range(6):
+                _, tilts, model = make_request()
+                t = srv.submit_model(simplis[k], TILTS[:nt], model=model)
+                with warnings.catch_warnings():
+                    warnings.simplefilter('ignore')
                     _, lat = srv.wait(t, return_latency=True)
-                    times.append(lat)
-                service[(k, nt, interp)] = float(np.median(times[2:]))
+                times.append(lat)
+            service[(k, nt)] = float(np.median(times[2:]))
     mean_service = float(np.mean(list(service.values())))
     rate = load / mean_service
     reqs = [make_request() for _ in range(n_requests)]
     gaps = rng.exponential(1.0 / rate, size=n_requests)
     arrivals = np.cumsum(gaps)
     outstanding, lats, work = [], [], 0
+    st0 = srv.stats()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     i = 0
-    while i < n_requests or outstanding:
-        now = time.perf_counter() - t0
-        while i < n_requests and arrivals[i] <= now:
-            k, tilts, interp, props = reqs[i]
-            outstanding.append(srv.submit(tissues[k][0], simplis[interp], props, tilts))
-            work += work_of[(k, len(tilts))]
-            i += 1
-        still = []
-        for t in outstanding:
-            try:
-                _, lat = srv.wait(t, timeout=0.0, return_latency=True)
-                lats.append(lat)
-            except TimeoutError:
-                still.append(t)
-        outstanding = still
-        if i < n_requests and not outstanding:
-            time.sleep(max(0.0, min(arrivals[i] - (time.perf_counter() - t0), 0.001)))
+    with warnings.catch_warnings():
+        warnings.simplefilter('ignore')  # interpenetrating bundles: "objects overlap"
+        while i < n_requests or outstanding:
+            now = time.perf_counter() - t0
+            while i < n_requests and arrivals[i] <= now:
+                k, tilts, model = reqs[i]
+                outstanding.append(srv.submit_model(simplis[k], tilts, model=model))
+                work += work_of[(k, len(tilts))]
+                i += 1
+            # block briefly on the oldest request (the wait releases the
+            # server's lock), then collect whatever else has finished
+            nxt = arrivals[i] - (time.perf_counter() - t0) if i < n_requests else 1e-3
+            if outstanding:
+                try:
+                    _, lat = srv.wait(outstanding[0], timeout=min(max(nxt, 0.0), 2e-4),
+                                      return_latency=True)
+                    lats.append(lat)
+                    outstanding.pop(0)
+                except TimeoutError:
+                    pass
+            elif nxt > 0:
+                time.sleep(min(nxt, 1e-3))
+            still = []
+            for t in outstanding:
+                try:
+                    _, lat = srv.wait(t, timeout=0.0, return_latency=True)
+                    lats.append(lat)
+                except TimeoutError:
+                    still.append(t)
+            outstanding = still
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
@@ -212,13 +230,19 @@ def serving_bench(n_requests, load, seed=0):
         'arrival_rate_per_s': rate,
         'offered_load': load,
         'mean_service_alone_ms': mean_service * 1e3,
+        'service_alone_ms': {f'{sizes[k][0]}^2x{sizes[k][2]}/{nt}t': v * 1e3
+                             for (k, nt), v in service.items()},
         'throughput_requests_per_s': n_requests / wall,
         'throughput_voxel_steps_per_s': work / wall,
         'max_inflight': 4,
+        'tissues_generated': st['generated'] - st0['generated'],
         'graphs_captured': st['graphs'],
-        'graph_replays': st['replays'],
-        'mix': 'sizes 64^2x60 / 128^2x120 / 256^2x200 / 512^2x300 uniform; 5 tilts or 1 (1:1); '
-               'Slerp or NN (1:1); dn, mu x U(0.5, 1.5); 9 rotations; Poisson arrivals',
+        'graph_replays': st['replays'] - st0['replays'],
+        'mix': 'each request generates its own tissue: seeded 1-3-bundle fiber model '
+               '(radius lognormal(1.0, 0.2) um, dn U[-0.006,-0.001], mu U[1,10] per request; '
+               f'geometry from {pool} seeded models per size), sizes 64^2x60 / 128^2x120 / '
+               '256^2x200 / 512^2x300 uniform; 5 tilts or 1 (1:1); 9 rotations; Slerp; '
+               'Poisson arrivals',
     }
 
 

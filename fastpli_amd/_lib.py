@@ -73,7 +73,14 @@ class FplFields(ctypes.Structure):
 
 class FplRequest(ctypes.Structure):
     _fields_ = [('tissue', _c_i64), ('setup', ctypes.POINTER(FplSetup)), ('props', _c_p),
-                ('n_props', _c_i64), ('tilts', _c_p), ('n_tilts', _c_i64), ('images', _c_p)]
+                ('n_props', _c_i64), ('tilts', _c_p), ('n_tilts', _c_i64), ('images', _c_p),
+                ('stream', _c_p)]
+
+
+class FplModelRequest(ctypes.Structure):
+    _fields_ = [('volume', FplVolume), ('bundles', FplFiberBundles),
+                ('background_absorption', ctypes.c_double), ('setup', ctypes.POINTER(FplSetup)),
+                ('tilts', _c_p), ('n_tilts', _c_i64), ('images', _c_p), ('stream', _c_p)]
 
 
 _LIB = None
@@ -85,7 +92,8 @@ EXPORTS = ('fpl_num_properties', 'fpl_generate', 'fpl_simulate', 'fpl_optic_resa
            'fpl_server_remove_tissue', 'fpl_server_submit', 'fpl_server_wait',
            'fpl_server_stats', 'fpl_server_destroy', 'fpl_rofl', 'fpl_simulate_adjoint',
            'fpl_record_bytes', 'fpl_fields_bytes', 'fpl_fields_codes_offset',
-           'fpl_generate_fields', 'fpl_pack_fields', 'fpl_simulate_fields')
+           'fpl_generate_fields', 'fpl_pack_fields', 'fpl_simulate_fields',
+           'fpl_server_submit_model')
 
 
 def lib():
@@ -172,6 +180,9 @@ def lib():
         L.fpl_server_submit.restype = ctypes.c_int
         L.fpl_server_submit.argtypes = [_c_p, ctypes.POINTER(FplRequest),
                                         ctypes.POINTER(_c_i64)] + msgargs
+        L.fpl_server_submit_model.restype = ctypes.c_int
+        L.fpl_server_submit_model.argtypes = [_c_p, ctypes.POINTER(FplModelRequest),
+                                              ctypes.POINTER(_c_i64)] + msgargs
         L.fpl_server_wait.restype = ctypes.c_int
         L.fpl_server_wait.argtypes = [_c_p, _c_i64, ctypes.c_double,
                                       ctypes.POINTER(ctypes.c_uint32),

@@ -1,4 +1,5 @@
 // Library-wide entry points and the per-stream device workspace.
+#include <array>
 #include <cmath>
 #include <map>
 #include <string>
@@ -17,7 +18,8 @@ struct Workspace {
    size_t cap[WS_NUM_SLOTS] = {};
    void *host = nullptr;
    size_t host_cap = 0;
-   int64_t pair_hint = 0; // generation: the last call's (tile, segment) pair count
+   // generation: the (tile, segment) pair count of the last call per scene shape
+   std::map<std::array<int64_t, 5>, int64_t> pair_hints;
 };
 
 std::mutex g_mutex;
@@ -32,9 +34,17 @@ Workspace &ws_for(void *stream) {
 
 } // namespace
 
-int64_t pair_hint(void *stream) { return ws_for(stream).pair_hint; }
+int64_t pair_hint(void *stream, const std::array<int64_t, 5> &key) {
+   Workspace &w = ws_for(stream);
+   auto it = w.pair_hints.find(key);
+   return it == w.pair_hints.end() ? 0 : it->second;
+}
 
-void set_pair_hint(void *stream, int64_t n) { ws_for(stream).pair_hint = n; }
+void set_pair_hint(void *stream, const std::array<int64_t, 5> &key, int64_t n) {
+   Workspace &w = ws_for(stream);
+   if (w.pair_hints.size() >= 256) w.pair_hints.clear();
+   w.pair_hints[key] = n;
+}
 
 Status ws_get(void *stream, int slot, size_t bytes, void **ptr) {
    Workspace &w = ws_for(stream);
@@ -61,6 +71,23 @@ Status ws_get(void *stream, int slot, size_t bytes, void **ptr) {
    }
    *ptr = w.ptr[slot];
    return Status();
+}
+
+size_t ws_bytes(void *stream) {
+   Workspace &w = ws_for(stream);
+   size_t n = 0;
+   for (int k = 0; k < WS_NUM_SLOTS; k++) n += w.cap[k];
+   return n;
+}
+
+void ws_free(void *stream) {
+   Workspace &w = ws_for(stream);
+   (void)hipStreamSynchronize((hipStream_t)stream);
+   for (int k = 0; k < WS_NUM_SLOTS; k++) {
+      if (w.ptr[k]) (void)hipFree(w.ptr[k]);
+      w.ptr[k] = nullptr;
+      w.cap[k] = 0;
+   }
 }
 
 Status host_stage(void *stream, size_t bytes, void **ptr) {

@@ -35,9 +35,11 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <vector>
 
+#include "fpl_gen.h"
 #include "fpl_internal.h"
 #include "fpl_records.h"
 
@@ -917,17 +919,6 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
    wave_sync();
 }
 
-// Where the voxelizer's final pass writes a voxel: the API's C-order fields
-// (label[x][y][z], vector[x][y][z][3]) and / or the march's [x][z][y] records
-// (fpl_records.h), each optional.
-struct Outputs {
-   int32_t *label;
-   float *vector;
-   void *rec_a;
-   uint32_t *rec_b;
-   int32_t rec_storage;
-};
-
 // the 16-bit storages (a double atan2 for the angle codes) out of line
 __device__ __forceinline__ void store_record_short(const Outputs &o, int64_t r, int32_t lab,
                                                 float x, float y, float z) {
@@ -1515,7 +1506,8 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
    int64_t total = 0;
    // a hint above 64 tiles per point (a much larger previous scene) would make
    // this call pad and sort mostly padding: it reads its count back instead
-   int64_t hint = use_hint ? pair_hint(st) : 0;
+   const std::array<int64_t, 5> scene{fbs->n_points, fbs->n_fibers, nx, ny, nz};
+   int64_t hint = use_hint ? pair_hint(st, scene) : 0;
    if (hint > 64 * fbs->n_points) hint = 0;
    if (np > 0) {
       hipLaunchKernelGGL(k_segments, dim3(grid_for(np, B)), dim3(B), 0, st, pr, fbs->points,
@@ -1614,7 +1606,7 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
             overflow = true;
             return Status();
          }
-         set_pair_hint(st, (int64_t)((uint64_t)f[4] | (uint64_t)f[5] << 32));
+         set_pair_hint(st, scene, (int64_t)((uint64_t)f[4] | (uint64_t)f[5] << 32));
       }
       rerun = (f[1] & 1u) != 0;
    }
@@ -1635,7 +1627,7 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
             overflow = true;
             return Status();
          }
-         set_pair_hint(st, (int64_t)((uint64_t)f[4] | (uint64_t)f[5] << 32));
+         set_pair_hint(st, scene, (int64_t)((uint64_t)f[4] | (uint64_t)f[5] << 32));
       }
    }
    if (warn) *warn = *(uint32_t *)stage;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -81,10 +82,17 @@ enum Slot : int {
 
 Status ws_get(void *stream, int slot, size_t bytes, void **ptr);
 
-// The (tile, segment) pair count of the stream's last generation (0: none),
-// which the next one assumes instead of reading its own back mid-call.
-int64_t pair_hint(void *stream);
-void set_pair_hint(void *stream, int64_t n);
+// Device bytes the workspace of `stream` holds, and its release (the stream
+// must be idle).
+size_t ws_bytes(void *stream);
+void ws_free(void *stream);
+
+// The (tile, segment) pair count of the stream's last generation of a scene
+// shape (points, fibers, nx, ny, nz; 0: none), which the next generation of
+// that shape assumes instead of reading its own back mid-call (a serving slot
+// alternates between scenes: one hint per stream would mostly be wrong).
+int64_t pair_hint(void *stream, const std::array<int64_t, 5> &scene);
+void set_pair_hint(void *stream, const std::array<int64_t, 5> &scene, int64_t n);
 
 // Pinned host staging for the small device->host readbacks of a call.
 Status host_stage(void *stream, size_t bytes, void **ptr);

@@ -38,12 +38,20 @@ void clear_flags(uint32_t *d_flags, int32_t *d_mm, hipStream_t st);
 void pack(int storage, const int64_t dim[3], const int32_t *label, const float *vector,
           void *packed, uint32_t *packed_b, int32_t *d_mm, hipStream_t st);
 
+// Whether the march holds B's property table in LDS (a kernel variant).
+bool props_in_lds(const Block &B);
+
 // All tilts of B in one launch; d_block holds B.bytes on the device.
 void march(const Block &B, const void *d_block, const void *packed, const uint32_t *packed_b,
            double *images, uint32_t *d_flags, hipStream_t st);
 
 // The reference's label checks (simulator.cpp:131-168) on a reduced range.
 Status check_labels(int32_t lmin, int32_t lmax, int64_t n_props, int storage);
+
+// Packs C-order fields into march fields on `st` and records their label
+// range (returns after the packing is complete).
+Status pack_fields(const int64_t dim[3], const int32_t *label, const float *vector,
+                   fpl_fields *Fd, hipStream_t st);
 
 } // namespace sim
 } // namespace fpl

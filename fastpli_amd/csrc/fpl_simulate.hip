@@ -1220,6 +1220,8 @@ void pack(int storage, const int64_t dim[3], const int32_t *label, const float *
                          dim[1], dim[2], packed, packed_b, d_mm);
 }
 
+bool props_in_lds(const Block &B) { return B.n_props <= MAX_LDS_PROPS; }
+
 void march(const Block &B, const void *d_block, const void *packed, const uint32_t *packed_b,
            double *images, uint32_t *d_flags, hipStream_t st) {
    const int64_t n_tiles = ray_tiles(B.dim[0], B.dim[1]);
@@ -1228,7 +1230,7 @@ void march(const Block &B, const void *d_block, const void *packed, const uint32
    const PropDev *d_pr = (const PropDev *)((const char *)d_block + B.off_pr);
    const dim3 mgrid((unsigned)(n_tiles * B.n_tilts));
    const bool small_idx = B.dim[0] * B.dim[1] * B.dim[2] < ((int64_t)1 << 32);
-   const bool props_lds = B.n_props <= MAX_LDS_PROPS;
+   const bool props_lds = props_in_lds(B);
 #define FPL_MARCH(IDX, STV)                                                                       \
    do {                                                                                           \
       if (props_lds)                                                                              \

@@ -13,6 +13,11 @@
                       50 x 40 grid of the bundle's cross-section.
 ``single_bundle``     one bundle with per-fiber directions drawn from a von
                       Mises-Fisher distribution about a random axis.
+``serving_model``     a request of the serving stream: 1-3 bundles of straight
+                      fibers, each bundle along a random direction (azimuth
+                      uniform, inclination within +-30 deg), radii lognormal
+                      (mean 1.0 um, sigma 0.2), its 'p' layer's dn uniform in
+                      [-0.006, -0.001] and both layers' mu uniform in [1, 10].
 
 Layers of every model: [(0.75, 0, 5, 'b'), (1.0, -0.004, 1, 'p')] (myelin
 model of the benchmark), unless given.
@@ -116,3 +121,51 @@ def _vmf_sample(rng, mu, kappa):
     v -= v.dot(mu) * mu
     v /= np.linalg.norm(v)
     return w * mu + np.sqrt(max(0.0, 1.0 - w * w)) * v
+
+
+SERVE_SIZES = ((64, 64, 60), (128, 128, 120), (256, 256, 200), (512, 512, 300))
+
+
+def serving_model(seed, dim, voxel_size=0.2, density=0.28, seg=0.5, sigma=0.2):
+    """Fiber model of one serving request in a volume of `dim` voxels.
+
+    Every bundle gets ``density`` fibers per um^2 of the volume's cross-section
+    normal to its direction (the crossing bundles' 2000 fibers over
+    120 x 60 um^2), centres uniform in the volume, straight polylines with
+    `seg` um segments reaching 2 um beyond the volume.  Returns FiberBundles
+    with their layers [(0.75, 0, mu_b, 'b'), (1.0, dn, mu_p, 'p')].
+    """
+    rng = np.random.default_rng(seed)
+    size = np.asarray(dim, dtype=float) * voxel_size
+    n_bundles = int(rng.integers(1, 4))
+    data, layers = [], []
+    half = 0.5 * np.linalg.norm(size) + 2.0
+    t = np.arange(-half, half + 1e-9, seg)
+    for _ in range(n_bundles):
+        az = rng.uniform(0, 2 * np.pi)
+        inc = np.deg2rad(rng.uniform(-30.0, 30.0))
+        d = np.array([np.cos(inc) * np.cos(az), np.cos(inc) * np.sin(az), np.sin(inc)])
+        # area of the box's projection along d
+        area = abs(d[0]) * size[1] * size[2] + abs(d[1]) * size[0] * size[2] + \
+            abs(d[2]) * size[0] * size[1]
+        n_f = max(1, int(round(density * area)))
+        c = rng.uniform(0, 1, size=(n_f, 3)) * size
+        r = np.exp(rng.normal(np.log(1.0) - sigma**2 / 2.0, sigma, size=n_f))
+        pts = c[:, None, :] + t[None, :, None] * d[None, None, :]
+        # the part of every fiber within 2 um of the volume (a convex region:
+        # one run of points per fiber)
+        inside = np.all((pts > -2.0) & (pts < size + 2.0), axis=2)
+        first = np.argmax(inside, axis=1)
+        last = t.size - 1 - np.argmax(inside[:, ::-1], axis=1)
+        fibers = []
+        for k in range(n_f):
+            if inside[k, first[k]] and last[k] > first[k]:
+                f = np.empty((last[k] + 1 - first[k], 4))
+                f[:, :3] = pts[k, first[k]:last[k] + 1]
+                f[:, 3] = r[k]
+                fibers.append(f)
+        dn = rng.uniform(-0.006, -0.001)
+        mu_b, mu_p = rng.uniform(1.0, 10.0, size=2)
+        data.append(fibers)
+        layers.append([(0.75, 0.0, float(mu_b), 'b'), (1.0, float(dn), float(mu_p), 'p')])
+    return objects.FiberBundles(data, layers)

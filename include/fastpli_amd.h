@@ -249,24 +249,36 @@ int fpl_profile_read(const char *name, double *total_ms, int64_t *launches);
 /* ------------------------------------------------------------ serving
  * A device-resident request loop for many simulations at once.  No
  * counterpart in the reference (it runs one simulation per process call, MPI
- * ranks sharing one volume); it serves callers that stream run_simulation
- * requests of different volume sizes, tilt sets and layer parameters.
+ * ranks sharing one volume); it serves callers that stream simulation
+ * requests of different volume sizes, fiber models, tilt sets and layer
+ * parameters.
  *
- * Tissues are registered once: their label/axis fields are packed into the
- * march's record layout on the device and stay resident.  A request names a
- * tissue, a setup (fpl_setup; the tissue's field_storage applies), a property table
- * and a tilt list, and a DEVICE output buffer.  A worker thread admits pending
- * requests in arrival order whenever a stream slot is free and the resident
- * tissues plus the device footprint of the requests in flight plus the new one
- * fit the HBM budget; admitted requests run concurrently, each on its slot's
- * stream as a captured hipGraph (parameter upload, flag reset, march, warning
- * readback) cached per (tissue, tilt count, property count, rotation count)
- * and replayed, followed by a copy of the slot's images into the request's
- * buffer.  Results are identical to fpl_simulate's for the same inputs.
+ * Two kinds of request:
+ *   fpl_request        the march of a registered tissue (its fields packed
+ *                      once by fpl_server_add_tissue and resident) under the
+ *                      request's setup, property table and tilt list;
+ *   fpl_model_request  a fiber model: its tissue is generated in the request's
+ *                      slot straight into march fields (fpl_generate_fields),
+ *                      then marched: Simpli.generate_tissue + run_simulation of
+ *                      every tilt (generator.cpp:165-261, simulator.cpp:182-411).
+ * The server runs max_inflight slots, each with its own stream, device buffers
+ * (grown to the largest request the slot ran) and host thread.  Pending
+ * requests are admitted in arrival order: a free slot takes the oldest one when
+ * the registered tissues, the memory all slots hold and what the slot must grow
+ * by fit the HBM budget; otherwise the request waits until a slot finishes.
+ * When every slot is idle and the request still does not fit, the slots
+ * release their buffers; a request that exceeds the budget on its own fails
+ * with FPL_E_NOMEM.  The march of a request is the replay of a hipGraph
+ * (parameter upload, flag reset, march, warning readback) captured once per
+ * slot and request shape, followed by a copy of the slot's images into the
+ * request's buffer.  Requests in different slots overlap on the device; results
+ * are identical to fpl_simulate's / fpl_generate + fpl_simulate's for the same
+ * inputs.
  *
- * The server owns its streams, slot buffers and graphs; the caller owns the
- * request's output buffer, which must stay valid until fpl_server_wait has
- * returned FPL_OK (or an error) for the ticket.  All functions are thread safe. */
+ * A request's work starts after the work queued on `stream` (the caller's
+ * stream, may be NULL) at submit, so its output buffer may still be in use
+ * there.  The caller owns the output buffer, which must stay valid until
+ * fpl_server_wait has returned for the ticket.  All functions are thread safe. */
 typedef struct fpl_server fpl_server;
 
 typedef struct fpl_request {
@@ -277,7 +289,23 @@ typedef struct fpl_request {
    const double *tilts;     /* HOST float64 (n_tilts, 2) = (theta, phi); copied */
    int64_t n_tilts;
    double *images;          /* DEVICE float64 (n_tilts, nx, ny, n_rho) */
+   void *stream;            /* the caller's hipStream_t (see above) */
 } fpl_request;
+
+typedef struct fpl_model_request {
+   fpl_volume volume;
+   /* the fiber model as fpl_generate takes it, except that points and
+    * fiber_ptr are HOST arrays here, which the server reads until
+    * fpl_server_wait has returned for the ticket (pinned memory copies
+    * fastest); the layer tables are copied at submit */
+   fpl_fiber_bundles bundles;
+   double background_absorption; /* row 0 of the property table (Simpli sets it) */
+   const fpl_setup *setup;  /* HOST; copied; its field_storage is the fields' */
+   const double *tilts;     /* HOST float64 (n_tilts, 2); copied */
+   int64_t n_tilts;
+   double *images;          /* DEVICE float64 (n_tilts, dim[0], dim[1], n_rho) */
+   void *stream;            /* the caller's hipStream_t (see above) */
+} fpl_model_request;
 
 /* hbm_budget in bytes (<= 0: 90% of the device's free memory at creation);
  * max_inflight concurrent requests (stream slots, 1..16). */
@@ -293,15 +321,22 @@ int fpl_server_remove_tissue(fpl_server *server, int64_t tissue, char *msg, size
 /* Validates the request (the checks of fpl_simulate) and queues it. */
 int fpl_server_submit(fpl_server *server, const fpl_request *request, int64_t *ticket,
                       char *msg, size_t msg_len);
+/* Validates a fiber-model request (the setup checks of fpl_simulate, the
+ * storage's label width) and queues it; the generated property table is
+ * checked in the slot before the march. */
+int fpl_server_submit_model(fpl_server *server, const fpl_model_request *request,
+                            int64_t *ticket, char *msg, size_t msg_len);
 /* Waits up to timeout_s (< 0: no limit) for a ticket.  FPL_OK: the images are
- * written, warn receives the FPL_WARN_* mask and latency_s the time from
- * submit to completion; the ticket is then retired.  FPL_E_PENDING: not
- * finished yet.  Any other code: the request failed (retired). */
+ * written, warn receives the FPL_WARN_* mask (generation and march) and
+ * latency_s the time from submit to completion; the ticket is then retired.
+ * FPL_E_PENDING: not finished yet.  Any other code: the request failed
+ * (retired). */
 int fpl_server_wait(fpl_server *server, int64_t ticket, double timeout_s, uint32_t *warn,
                     double *latency_s, char *msg, size_t msg_len);
-/* stats[0..5] = submitted, completed, in flight, pending, graphs captured,
- * graph replays. */
-int fpl_server_stats(fpl_server *server, int64_t stats[6]);
+/* stats[0..8] = submitted, completed, in flight, pending, graphs captured,
+ * graph replays, tissues generated, bytes the slots hold, bytes of the
+ * registered tissues. */
+int fpl_server_stats(fpl_server *server, int64_t stats[9]);
 /* Finishes the requests in flight, fails the pending ones, frees everything. */
 int fpl_server_destroy(fpl_server *server);
 

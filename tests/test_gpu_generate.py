@@ -240,17 +240,21 @@ def test_generate_bent_fibers_slab_clip(seed):
 
 
 def test_generate_pair_count_from_previous_call():
-    """A generation sizes its (tile, segment) pair arrays by the previous
-    call's pair count on the same stream: a larger scene after a smaller one
-    (more pairs than assumed: the call runs again with its own count), the
-    smaller one after it (padding), and an empty one in between; every result
-    is bit-exact vs the oracle and the warnings match."""
+    """A generation sizes its (tile, segment) pair arrays by the pair count of
+    the previous call of the same scene shape (points, fibers, volume) on the
+    same stream: the same fibers at a finer voxel size (more pairs than assumed:
+    the call runs again with its own count), then the coarser one again
+    (padding); other scenes in between (no hint); every result is bit-exact vs
+    the oracle and the warnings match."""
     _say('generate: pair arrays sized by the previous call (larger, smaller, empty scenes)')
     small = synthetic.single_bundle(seed=5, n_fibers=6, size=(8.0, 8.0, 6.0))
     large = synthetic.crossing_bundles(seed=6, n_fibers=60, size_xy=12.0, size_z=6.0)
     empty = objects.FiberBundles([[np.array([[100.0, 100.0, 100.0, 1.0],
                                              [101.0, 100.0, 100.0, 1.0]])]],
                                  [[(1.0, 0.0, 1, 'p')]])
-    for fbs, dim in ((small, (40, 40, 30)), (large, (60, 60, 30)), (small, (40, 40, 30)),
-                     (empty, (20, 20, 20)), (large, (60, 60, 30))):
+    # the same points with three times the radii: the same scene shape, more pairs
+    thick = objects.FiberBundles([[f.as_array() * np.array([1.0, 1.0, 1.0, 3.0])
+                                   for f in small[0]]], small.layers)
+    for fbs, dim in ((small, (40, 40, 30)), (large, (60, 60, 30)), (thick, (40, 40, 30)),
+                     (small, (40, 40, 30)), (empty, (20, 20, 20)), (large, (60, 60, 30))):
         _compare(fbs, dim, [0.0, 0.0, 0.0], 0.2)
