@@ -93,7 +93,7 @@ EXPORTS = ('fpl_num_properties', 'fpl_generate', 'fpl_simulate', 'fpl_optic_resa
            'fpl_server_stats', 'fpl_server_destroy', 'fpl_rofl', 'fpl_simulate_adjoint',
            'fpl_record_bytes', 'fpl_fields_bytes', 'fpl_fields_codes_offset',
            'fpl_generate_fields', 'fpl_pack_fields', 'fpl_simulate_fields',
-           'fpl_server_submit_model')
+           'fpl_server_submit_model', 'fpl_simulate_noalloc')
 
 
 def lib():
@@ -136,6 +136,8 @@ def lib():
             ctypes.POINTER(FplSetup), ctypes.POINTER(FplFields), _c_p, _c_i64, _c_p, _c_i64,
             _c_p, ctypes.POINTER(ctypes.c_uint32), _c_p, ctypes.c_char_p, ctypes.c_size_t
         ]
+        L.fpl_simulate_noalloc.restype = ctypes.c_int
+        L.fpl_simulate_noalloc.argtypes = L.fpl_simulate.argtypes
         L.fpl_simulate_adjoint.restype = ctypes.c_int
         L.fpl_simulate_adjoint.argtypes = [
             ctypes.POINTER(FplSetup), _c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_i64, _c_p, _c_p,
@@ -354,6 +356,21 @@ def simulate(setup, dim, label, vector, props, tilts, images):
                             props.ctypes.data, props.shape[0], tilts.ctypes.data,
                             tilts.shape[0], _ptr(images), ctypes.byref(warn), _stream(), msg,
                             512)
+    check(rc, msg)
+    return warn.value
+
+
+def simulate_noalloc(setup, dim, label, vector, props, tilts, images):
+    """fpl_simulate_noalloc (static device scratch, no allocation); returns the warn mask."""
+    props = np.ascontiguousarray(props, dtype=np.float64)
+    tilts = np.ascontiguousarray(tilts, dtype=np.float64).reshape(-1, 2)
+    dim_arr = np.ascontiguousarray(dim, dtype=np.int64)
+    warn = ctypes.c_uint32(0)
+    msg = ctypes.create_string_buffer(512)
+    rc = lib().fpl_simulate_noalloc(ctypes.byref(setup), dim_arr.ctypes.data, _ptr(label),
+                                    _ptr(vector), props.ctypes.data, props.shape[0],
+                                    tilts.ctypes.data, tilts.shape[0], _ptr(images),
+                                    ctypes.byref(warn), _stream(), msg, 512)
     check(rc, msg)
     return warn.value
 

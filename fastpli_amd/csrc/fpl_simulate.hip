@@ -21,6 +21,8 @@
 // are stepped in FP64 exactly as the reference so that the visited voxels are
 // identical.
 #include <algorithm>
+#include <mutex>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -1300,6 +1302,80 @@ Status run(const fpl_setup *setup, const int64_t dim[3], const int32_t *label, c
    return Status();
 }
 
+// ------------------------------------------------------------ no allocation
+// fpl_simulate_noalloc: fpl_simulate whose scratch (the packed records, the
+// parameter block and the flag words) is a static device array instead of the
+// per-stream workspace, for callers that cannot allocate device memory.  Each
+// device holds its own copy from the library's load until the process ends;
+// every call on a device shares it (keeping the last call's contents), so
+// calls that overlap on one device race.
+constexpr size_t NOALLOC_FIELDS = (size_t)64 << 20; // records (+ angle codes)
+constexpr size_t NOALLOC_BLOCK = (size_t)64 << 10;  // parameter block, then 64 B of flags
+__device__ __align__(256) unsigned char scratch[NOALLOC_FIELDS + NOALLOC_BLOCK];
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the array's name in host code is only a stand-in for the symbol).
+Status scratch_address(void **out) {
+   static std::mutex mu;
+   static std::map<int, void *> addr;
+   int dev = 0;
+   FPL_HIP(hipGetDevice(&dev));
+   std::lock_guard<std::mutex> lk(mu);
+   auto it = addr.find(dev);
+   if (it == addr.end()) {
+      void *p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+         return make_status(FPL_E_DEVICE, "hipGetSymbolAddress(scratch) failed: %s",
+                            hipGetErrorString(e));
+      it = addr.emplace(dev, p).first;
+   }
+   *out = it->second;
+   return Status();
+}
+
+Status run_noalloc(const fpl_setup *setup, const int64_t dim[3], const int32_t *label,
+                   const float *vector, const double *props, int64_t n_props,
+                   const double *tilts, int64_t n_tilts, double *images, uint32_t *warn,
+                   hipStream_t st) {
+   Block B;
+   FPL_TRY(prepare(setup, dim, props, n_props, tilts, n_tilts, B));
+   if (!label || !vector || !images) return make_status(FPL_E_INVALID, "null array");
+   const size_t n_vox = (size_t)(dim[0] * dim[1] * dim[2]);
+   const size_t off_b = (n_vox * rec::record_bytes(B.storage) + 255) & ~(size_t)255;
+   const size_t fields = off_b + n_vox * rec::second_bytes(B.storage);
+   if (fields > NOALLOC_FIELDS)
+      return make_status(FPL_E_INVALID,
+                         "fpl_simulate_noalloc: the packed fields need %zu bytes, the static "
+                         "scratch holds %zu (at most %zu voxels in this storage)",
+                         fields, NOALLOC_FIELDS,
+                         NOALLOC_FIELDS /
+                             (rec::record_bytes(B.storage) + rec::second_bytes(B.storage)));
+   if (B.bytes.size() > NOALLOC_BLOCK - 64)
+      return make_status(FPL_E_INVALID,
+                         "fpl_simulate_noalloc: the parameter block needs %zu bytes, the static "
+                         "scratch holds %zu (fewer tilts or labels)",
+                         B.bytes.size(), NOALLOC_BLOCK - 64);
+   void *base = nullptr;
+   FPL_TRY(scratch_address(&base));
+   char *records = (char *)base;
+   uint32_t *codes = B.storage == ST_ANGLE16 ? (uint32_t *)(records + off_b) : nullptr;
+   char *block = records + NOALLOC_FIELDS;
+   uint32_t *d_flags = (uint32_t *)(block + NOALLOC_BLOCK - 64);
+   int32_t *d_mm = (int32_t *)(d_flags + 4);
+   FPL_HIP(hipMemcpyAsync(block, B.bytes.data(), B.bytes.size(), hipMemcpyHostToDevice, st));
+   clear_flags(d_flags, d_mm, st); // a kernel: the words start from the last call's values
+   pack(B.storage, dim, label, vector, records, codes, d_mm, st);
+   march(B, block, records, codes, images, d_flags, st);
+   FPL_HIP(hipGetLastError());
+   uint32_t host[8] = {};
+   FPL_HIP(hipMemcpyAsync(host, d_flags, sizeof(host), hipMemcpyDeviceToHost, st));
+   FPL_HIP(hipStreamSynchronize(st));
+   FPL_TRY(check_labels((int32_t)host[4], (int32_t)host[5], n_props, B.storage));
+   if (warn) *warn = host[0] & 0x7fffffffu;
+   return Status();
+}
+
 // fpl_simulate on march fields (fpl_records.h layout): the setup checks, then
 // the label range against the property table (CheckInput before the march, as
 // the reference), then the march.
@@ -1498,5 +1574,16 @@ extern "C" int fpl_simulate_fields(const fpl_setup *setup, const fpl_fields *fie
 extern "C" int fpl_pack_fields(const int64_t dim[3], const int32_t *label, const float *vector,
                                fpl_fields *fields, void *stream, char *msg, size_t msg_len) {
    fpl::Status s = fpl::sim::pack_fields(dim, label, vector, fields, (hipStream_t)stream);
+   return fpl::finish(s, msg, msg_len);
+}
+
+extern "C" int fpl_simulate_noalloc(const fpl_setup *setup, const int64_t dim[3],
+                                    const int32_t *label, const float *vector,
+                                    const double *props, int64_t n_props, const double *tilts,
+                                    int64_t n_tilts, double *images, uint32_t *warn, void *stream,
+                                    char *msg, size_t msg_len) {
+   if (warn) *warn = 0;
+   fpl::Status s = fpl::sim::run_noalloc(setup, dim, label, vector, props, n_props, tilts,
+                                         n_tilts, images, warn, (hipStream_t)stream);
    return fpl::finish(s, msg, msg_len);
 }

@@ -185,6 +185,22 @@ int fpl_simulate_fields(const fpl_setup *setup, const fpl_fields *fields,
                         int64_t n_tilts, double *images, uint32_t *warn, void *stream,
                         char *msg, size_t msg_len);
 
+/* fpl_simulate that allocates nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * checks, kernels and results as fpl_simulate; its scratch (the packed fields
+ * and the parameter block) is a static device array of 64 MiB + 64 KiB that
+ * every device the library runs on holds until the process ends.  Volumes
+ * whose packed fields exceed 64 MiB (4 Mi voxels in f32, 8 Mi in f16, 11 Mi in
+ * angle16) or parameter blocks above 64 KiB are rejected with FPL_E_INVALID.
+ * All calls on one device share the array: calls that overlap on a device
+ * race and must be serialised by the caller. */
+int fpl_simulate_noalloc(const fpl_setup *setup, const int64_t dim[3],
+                         const int32_t *label, const float *vector,
+                         const double *props, int64_t n_props, const double *tilts,
+                         int64_t n_tilts, double *images, uint32_t *warn, void *stream,
+                         char *msg, size_t msg_len);
+
 /* Reverse mode of fpl_simulate (no counterpart in the reference, which has no
  * derivatives): for L = sum(grad_images * images), with images what
  * fpl_simulate returns for the same arguments,
