@@ -375,11 +375,31 @@ def simulate_noalloc(setup, dim, label, vector, props, tilts, images):
     return warn.value
 
 
+def _check_array(name, a, shape, dtype):
+    """The shape, element type and contiguity a raw-pointer argument must have."""
+    if tuple(a.shape) != tuple(int(d) for d in shape):
+        raise ValueError(f'{name} has shape {tuple(a.shape)}, expected {tuple(shape)}')
+    if str(a.dtype).split('.')[-1] != dtype:
+        raise ValueError(f'{name} has dtype {a.dtype}, expected {dtype}')
+    contiguous = a.is_contiguous() if hasattr(a, 'is_contiguous') else a.flags['C_CONTIGUOUS']
+    if not contiguous:
+        raise ValueError(f'{name} must be contiguous')
+
+
 def simulate_adjoint(setup, dim, label, vector, props, tilts, grad_images, grad_vector=None):
-    """fpl_simulate_adjoint on device tensors; returns dL/d props as (n_props, 2) float64."""
+    """fpl_simulate_adjoint on device tensors; returns dL/d props as (n_props, 2) float64.
+
+    The kernel reads and writes through raw pointers: every array's shape,
+    element type and contiguity is checked against the setup and dim first."""
     props = np.ascontiguousarray(props, dtype=np.float64)
     tilts = np.ascontiguousarray(tilts, dtype=np.float64).reshape(-1, 2)
     dim_arr = np.ascontiguousarray(dim, dtype=np.int64)
+    nx, ny, nz = (int(d) for d in dim_arr)
+    _check_array('tissue', label, (nx, ny, nz), 'int32')
+    _check_array('optical_axis', vector, (nx, ny, nz, 3), 'float32')
+    _check_array('grad_images', grad_images, (tilts.shape[0], nx, ny, setup.n_rho), 'float64')
+    if grad_vector is not None:
+        _check_array('grad_vector', grad_vector, (nx, ny, nz, 3), 'float32')
     grad_props = np.zeros(props.shape, dtype=np.float64)
     msg = ctypes.create_string_buffer(512)
     rc = lib().fpl_simulate_adjoint(ctypes.byref(setup), dim_arr.ctypes.data, _ptr(label),
@@ -393,8 +413,11 @@ def simulate_adjoint(setup, dim, label, vector, props, tilts, grad_images, grad_
 
 def count_voxel_steps(setup, dim, theta, phi):
     dim_arr = np.ascontiguousarray(dim, dtype=np.int64)
-    return int(lib().fpl_count_voxel_steps(ctypes.byref(setup), dim_arr.ctypes.data,
-                                           float(theta), float(phi)))
+    n = int(lib().fpl_count_voxel_steps(ctypes.byref(setup), dim_arr.ctypes.data, float(theta),
+                                        float(phi)))
+    if n < 0:
+        raise ValueError(f'tilt theta = {theta} needs more than 2^24 z-steps per ray')
+    return n
 
 
 def profile_enable(on=True):

@@ -265,6 +265,8 @@ int64_t fpl_count_voxel_steps(const fpl_setup *setup, const int64_t dim[3], doub
          st = c;
    }
    const double step_z = st * setup->step_size;
+   // the bound fpl_simulate puts on one ray's z-steps (-1: the tilt is rejected)
+   if (!((double)dim[2] / std::fabs(step_z) <= (double)((int64_t)1 << 24))) return -1;
    double z = 0.0 + step_z * 0.5;
    int64_t n = 0;
    for (; std::floor(z) >= 0 && std::floor(z) < (double)dim[2]; z = z + step_z) {

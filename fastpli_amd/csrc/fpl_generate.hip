@@ -91,6 +91,7 @@ struct Params {
    int32_t ntx, nty, ntz;
    int32_t n_lsq; // rows of the layer tables
    int32_t n_props;
+   int32_t single_free; // tiles of one bundle use the free mode (see k_voxelize)
 };
 
 // aabb::AABB(p, q) unsorted constructor (aabb.hpp:54-71)
@@ -836,7 +837,14 @@ __device__ __forceinline__ void run_batch(WaveState<SegT> &W, const LdsSeg *chun
          // | model << 14 | label (model 0 'b', 1 'p', 2 'r'; labels < 2^14)
          if (acc) {
             const int32_t lab = layer_label(dist_sq, r_sq, lsq, R.layer_off, R.bundle, max_layer);
-            const uint8_t m = lmodel[R.layer_off + (lab - 1 - R.bundle * max_layer)];
+            const int32_t layer = lab - 1 - R.bundle * max_layer;
+            // the past-the-end layer of a bundle with max_layer layers is
+            // max_layer labels from its layer 0: replacing one by the other is
+            // an overlap for the reference (generator.cpp:353-359), which the
+            // order-free single-bundle mode does not follow; the host reruns
+            // the volume in order (bit 2 of flags[3])
+            if (!ov && layer == max_layer) atomicOr(&flags[3], 4u);
+            const uint8_t m = lmodel[R.layer_off + layer];
             const uint32_t m2 = m == 'p' ? 1u : (m == 'r' ? 2u : 0u);
             free_lo = (0x10000u - widx) << 16 | m2 << 14 | (uint32_t)lab;
          }
@@ -1042,7 +1050,9 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
    ov = uniform((int)ov) != 0;
    // free mode (see the key layout above): the warning is raised already, or
    // the list (global segment order, bundle-major) starts and ends in one
-   // bundle, so no update here can replace another bundle's label
+   // bundle, so no update here can replace another bundle's label (but for
+   // the bundle's past-the-end label: stage B flags it, and the host reruns
+   // the volume with pr.single_free = 0)
    bool free_mode = false;
    if (short_seg) {
       bool single = true;
@@ -1050,7 +1060,7 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
          const int2 tb = tile_bundle[t];
          single = tb.x == tb.y;
       }
-      free_mode = uniform((int)(ov || single)) != 0;
+      free_mode = uniform((int)(ov || (single && pr.single_free))) != 0;
    }
    int q_head = 0, q_tail = 0;
    wave_sync();
@@ -1453,6 +1463,7 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
    pr.max_layer = max_layer;
    pr.n_lsq = (int32_t)lsq.size();
    pr.n_props = (int32_t)n_props;
+   pr.single_free = 1;
    pr.ntx = (int32_t)((nx + TX - 1) / TX);
    pr.nty = (int32_t)((ny + TY - 1) / TY);
    pr.ntz = (int32_t)((nz + TZ - 1) / TZ);
@@ -1609,6 +1620,19 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
          set_pair_hint(st, scene, (int64_t)((uint64_t)f[4] | (uint64_t)f[5] << 32));
       }
       rerun = (f[1] & 1u) != 0;
+      if (!rerun && (f[3] & 4u) && !(f[0] & FPL_WARN_OVERLAP)) {
+         // a past-the-end label in a single-bundle tile run without order:
+         // the overlap warning needs the update order, so the volume runs
+         // again with those tiles in order (same voxels; warnings are an OR)
+         pr.single_free = 0;
+         KernelTimer kt3("voxelize", st);
+         voxelize(true);
+         kt3.stop();
+         FPL_HIP(hipGetLastError());
+         FPL_HIP(hipMemcpyAsync(stage, d_flags, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+         FPL_HIP(hipStreamSynchronize(st));
+         kt3.collect();
+      }
    }
    if (rerun) {
       // a tile list longer than 65535 (or labels beyond 15 bits): the whole

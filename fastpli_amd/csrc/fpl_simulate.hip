@@ -37,6 +37,7 @@ namespace fpl {
 namespace sim {
 
 constexpr int WG = 256;
+constexpr int64_t MAX_RAY_STEPS = (int64_t)1 << 24; // z-steps of one ray (prepare rejects more)
 constexpr int MAX_LDS_PROPS = 128; // property tables up to this size are held in LDS
 constexpr int MAX_RHO = 64;
 
@@ -1178,6 +1179,14 @@ Status prepare(const fpl_setup *setup, const int64_t dim[3], const double *props
       if (tdev[t].step[2] == 0.0)
          return make_status(FPL_E_INVALID, "tilt theta = %f gives no progress along z",
                             tilts[2 * t]);
+      // a direction just short of parallel (theta within ~1e-7 of 90 degrees)
+      // ends, but after up to ~1e10 steps per ray: an effectively endless
+      // kernel.  Reject trip counts above MAX_RAY_STEPS (nz / |step_z| bounds it)
+      const double trips = (double)dim[2] / std::fabs(tdev[t].step[2]);
+      if (!(trips <= (double)MAX_RAY_STEPS))
+         return make_status(FPL_E_INVALID,
+                            "tilt theta = %f needs %.3g z-steps per ray, more than %lld",
+                            tilts[2 * t], trips, (long long)MAX_RAY_STEPS);
    }
    // tilts that interpolate (long rays of work) first, so that the short
    // nearest-neighbour workgroups fill the end of the launch

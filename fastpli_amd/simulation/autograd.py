@@ -48,16 +48,17 @@ def _function():
             if bool(torch.any(images < 0)):
                 raise ValueError('intensity < 0 detected')
             ctx.save_for_backward(props, optical_axis, tissue)
-            ctx.simpli, ctx.tilts, ctx.dim, ctx.keep = simpli, tilts, dim, keep
+            # the backward differentiates this forward: its setup (and the
+            # rotations it points to), not the simpli's state at backward time
+            ctx.setup, ctx.keep, ctx.tilts, ctx.dim = setup, keep, tilts, dim
             return images
 
         @staticmethod
         def backward(ctx, grad_images):
             props, axis, tissue = ctx.saved_tensors
-            setup, keep = ctx.simpli._setup()
             g = grad_images.to(torch.float64).contiguous()
             gvec = torch.empty_like(axis) if ctx.needs_input_grad[1] else None
-            gp = _lib.simulate_adjoint(setup, ctx.dim, tissue, axis.detach(),
+            gp = _lib.simulate_adjoint(ctx.setup, ctx.dim, tissue, axis.detach(),
                                        props.detach().to('cpu', torch.float64).numpy(),
                                        ctx.tilts, g, gvec)
             gprops = None

@@ -249,7 +249,8 @@ int fpl_rofl(const double *in, int64_t n_pix, int64_t n_rho, double tilt_angle, 
              size_t msg_len);
 
 /* Exact count of Mueller voxel-steps of one tilt: n_rho x rays x z-march
- * iterations (simulator.cpp:259-261).  Host only. */
+ * iterations (simulator.cpp:259-261).  Host only.  -1 for a tilt whose rays
+ * need more than 2^24 z-steps, which fpl_simulate rejects. */
 int64_t fpl_count_voxel_steps(const fpl_setup *setup, const int64_t dim[3],
                               double theta, double phi);
 

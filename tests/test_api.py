@@ -152,3 +152,43 @@ def test_field_storage_property():
     assert s.get_dict()['field_storage'] == 'angle16'
     with pytest.raises(ValueError):
         s.field_storage = 'bf16'
+
+
+def test_adjoint_binding_checks_array_shapes():
+    """The adjoint's raw pointers are checked against setup and dim before the call."""
+    import torch
+    setup, _keep = _lib.make_setup(1.0, 26000, 1.0, 525, 1.0, 'NN', True, False,
+                                   np.linspace(0, np.pi, 6, endpoint=False))
+    dim = [4, 5, 6]
+    lab = torch.zeros(dim, dtype=torch.int32)
+    vec = torch.zeros(dim + [3], dtype=torch.float32)
+    props = np.zeros((2, 2))
+    tilts = [(0.0, 0.0)]
+    good = torch.zeros((1, 4, 5, 6), dtype=torch.float64)
+    with pytest.raises(ValueError, match='grad_images has shape'):
+        _lib.simulate_adjoint(setup, dim, lab, vec, props, tilts, torch.zeros((1, 4, 5, 9),
+                                                                              dtype=torch.float64))
+    with pytest.raises(ValueError, match='grad_images has dtype'):
+        _lib.simulate_adjoint(setup, dim, lab, vec, props, tilts, good.float())
+    with pytest.raises(ValueError, match='grad_vector has shape'):
+        _lib.simulate_adjoint(setup, dim, lab, vec, props, tilts, good,
+                              torch.zeros((4, 5, 5, 3), dtype=torch.float32))
+    with pytest.raises(ValueError, match='optical_axis has dtype'):
+        _lib.simulate_adjoint(setup, dim, lab, vec.double(), props, tilts, good)
+    with pytest.raises(ValueError, match='must be contiguous'):
+        _lib.simulate_adjoint(setup, dim, lab, vec, props, tilts,
+                              torch.zeros((1, 5, 4, 6), dtype=torch.float64).transpose(1, 2))
+
+
+def test_near_parallel_tilt_is_rejected_on_the_host():
+    """A tilt just short of 90 degrees has a z-step of ~1e-8 voxel: rejected
+    (by trip count) before any device work, like the exactly parallel one."""
+    s = Simpli()
+    s.voxel_size = 1.0
+    s.dim = [4, 4, 300]
+    s.light_intensity = 26000
+    s.wavelength = 525
+    s.filter_rotations = np.linspace(0, np.pi, 6, endpoint=False)
+    with pytest.raises(ValueError, match='z-steps per ray'):
+        s.count_voxel_steps(1.5707963, 0.0)
+    assert s.count_voxel_steps(np.deg2rad(60), 0.0) > 0

@@ -258,3 +258,20 @@ def test_generate_pair_count_from_previous_call():
     for fbs, dim in ((small, (40, 40, 30)), (large, (60, 60, 30)), (thick, (40, 40, 30)),
                      (small, (40, 40, 30)), (empty, (20, 20, 20)), (large, (60, 60, 30))):
         _compare(fbs, dim, [0.0, 0.0, 0.0], 0.2)
+
+
+def test_generate_past_the_end_overlap_in_one_bundle():
+    """One bundle whose outer layer scale is 0.7 (float(0.49) > 0.49): a voxel
+    centre 0.7 from fiber A (r = 1) gets the past-the-end layer, label 3, which
+    is max_layer = 2 labels from layer 0.  Fiber B, listed after A, gives the
+    same voxels label 1: the reference warns "objects overlap" (the order A, B
+    replaces 3 by 1) but not for B, A (A never wins).  Labels bit-exact and the
+    same verdict as the oracle both ways."""
+    _say('generate: past-the-end label vs layer 0 in one bundle')
+    a = np.array([[-1.0, 1.2, 0.5, 1.0], [9.0, 1.2, 0.5, 1.0]])
+    b = np.array([[-1.0, 0.5, 0.6, 1.0], [9.0, 0.5, 0.6, 1.0]])
+    layers = [[(0.45, -0.004, 1, 'p'), (0.7, 0.004, 2, 'r')]]
+    for order, warns in (([a, b], True), ([b, a], False)):
+        fbs = objects.FiberBundles([order], layers)
+        _, _, ow = _compare(fbs, (8, 4, 4), [0.0] * 3, 1.0)
+        assert bool(ow & oracle.WARN_OVERLAP) == warns

@@ -153,6 +153,8 @@ def test_serving_errors(tissues):
             srv.submit(tid, _simpli(voxel), bad, [(0, 0)])
         with pytest.raises(ValueError, match='no progress'):
             srv.submit(tid, _simpli(voxel), props, [(np.pi / 2, 0)])
+        with pytest.raises(ValueError, match='z-steps per ray'):
+            srv.submit(tid, _simpli(voxel), props, [(1.5707963, 0)])
         neg = lab.astype(np.int32).copy()
         neg[0, 0, 0] = -1
         with pytest.raises(ValueError, match='label < 0'):

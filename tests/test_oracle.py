@@ -374,3 +374,24 @@ def test_oracle_lerp_corners_of_other_labels():
     label = np.ones((2, 1, 1), dtype=np.int32)
     got = oracle.get_vec(label, vector, (0.75, 0.5, 0.5), 'Lerp')
     assert np.allclose(got, [0.9, -0.2, 0.0], rtol=0, atol=1e-7), got
+
+
+def test_oracle_past_the_end_label_overlap_known_answer():
+    """Hand-derived case: with r = 1 and outer scale 0.7, a voxel centre at
+    distance 0.7 has float(d^2) / float(r^2) = float(0.49) > 0.49 (double), so
+    lower_bound passes the last layer: label 1 + 0 * 2 + 2 = 3 (generator.cpp:
+    345-351).  A second fiber of the bundle at distance 0.1 gives label 1 and
+    replaces it (0.01 <= 0.49), |3 - 1| = 2 >= max_layer: "objects overlap"
+    (generator.cpp:353-359).  Listed the other way round, nothing is replaced."""
+    a = np.array([[-1.0, 1.2, 0.5, 1.0], [9.0, 1.2, 0.5, 1.0]])
+    b = np.array([[-1.0, 0.5, 0.6, 1.0], [9.0, 0.5, 0.6, 1.0]])
+    layers = [[(0.45, -0.004, 1, 'p'), (0.7, 0.004, 2, 'r')]]
+    assert float(np.float32(0.7 * 0.7)) > 0.49
+    only_a = objects.FiberBundles([[a]], layers)
+    la, _, pa, _ = oracle.generate(only_a, (8, 4, 4), np.zeros(3), 1.0, n_threads=1)
+    assert np.all(la[:, 0, 0] == 3) and pa.shape[0] == 3
+    for order, warns in (([a, b], True), ([b, a], False)):
+        fbs = objects.FiberBundles([order], layers)
+        lab, _, _, w = oracle.generate(fbs, (8, 4, 4), np.zeros(3), 1.0, n_threads=1)
+        assert np.all(lab[:, 0, 0] == 1)
+        assert bool(w & oracle.WARN_OVERLAP) == warns
