@@ -1,153 +1,132 @@
-"""Layer containers of a fiber bundle: (scale, birefringence dn, absorption mu, model).
+"""Layers of a fiber bundle: (scale, birefringence dn, absorption mu [1/mm], model).
 
-Behaviour follows fastpli's ``src/fastpli/objects/layers.py:9-153``: a layer is
-a 4-tuple ``(float, float, float, 'b'|'p'|'r')``; a list of layers is sorted by
-scale (warning 'layers had to be sorted'), duplicate scales raise
-``ValueError('detected layers of same radii')``, and a background layer ('b')
-with non-zero birefringence raises ``ValueError``.
+Contract kept from the reference (src/fastpli/objects/layers.py): a layer is
+four values ``(scale, dn, mu, 'b'|'p'|'r')``; the messages and the order of the
+checks below; a bundle's layers are held sorted by scale.
+
+Design: a ``Layer`` is an immutable tuple subclass (nothing to copy or
+freeze), and the checks are data, in the order the reference applies them.
 """
 
-import copy
 import warnings
 
-_MODELS = ('b', 'p', 'r')
+from ._sealed import Sealed, edited
+
+# What one layer must satisfy, in the reference's order (layers.py:9-18):
+# (violated?, TypeError / ValueError raises, a Warning class warns, message)
+LAYER_RULES = (
+    (lambda ly: len(ly) != 4, TypeError, 'layer != (float, float, float, char)'),
+    (lambda ly: ly[1] < 0 and ly[3] == 'r', UserWarning, 'birefringence negative and radial'),
+    (lambda ly: ly[1] > 0 and ly[3] == 'p', UserWarning, 'birefringence positive and parallel'),
+    (lambda ly: ly[1] != 0 and ly[3] == 'b', ValueError, 'birefringence != 0 for background.'),
+)
 
 
-def _check_layer(ly):
-    """Validate one layer tuple (layers.py:9-18)."""
-    if len(ly) != 4:
-        raise TypeError('layer != (float, float, float, char)')
-    if ly[1] < 0 and ly[-1] == 'r':
-        warnings.warn('birefringence negative and radial')
-    if ly[1] > 0 and ly[-1] == 'p':
-        warnings.warn('birefringence positive and parallel')
-    if ly[1] != 0 and ly[-1] == 'b':
-        raise ValueError('birefringence != 0 for background.')
-    return (float(ly[0]), float(ly[1]), float(ly[2]), ly[3])
+def apply_rules(rules, value):
+    """Raises or warns for every violated rule of `rules`, in order."""
+    for violated, kind, message in rules:
+        if violated(value):
+            if issubclass(kind, Warning):
+                warnings.warn(message, kind, stacklevel=3)
+            else:
+                raise kind(message)
 
 
-class Layer:
-    """One layer: scale relative to the fiber radius, dn, mu [1/mm], model."""
+class Layer(tuple):
+    """One layer: ``Layer(scale, dn, mu, model)`` or ``Layer((scale, dn, mu, model))``."""
 
-    __is_frozen = False
+    __slots__ = ()
 
-    def __setattr__(self, key, value):
-        if self.__is_frozen and not hasattr(self, key):
+    def __new__(cls, *args):
+        raw = args[0] if len(args) == 1 else args
+        if isinstance(raw, Layer):
+            return raw
+        raw = tuple(raw)
+        apply_rules(LAYER_RULES, raw)
+        return tuple.__new__(cls, (float(raw[0]), float(raw[1]), float(raw[2]), raw[3]))
+
+    def __setattr__(self, name, value):
+        if not hasattr(self, name):
             raise TypeError(f'{self!r} is a frozen class')
-        object.__setattr__(self, key, value)
-
-    def __freeze(self):
-        self.__is_frozen = True
-
-    def __init__(self, *args):
-        if len(args) == 1:
-            args = args[0]
-        self._data = _check_layer(args)
-        self.__freeze()
-
-    def __getitem__(self, item):
-        return self._data[item]
-
-    def __str__(self):
-        return str(self._data)
-
-    def __repr__(self):
-        return repr(self._data)
+        object.__setattr__(self, name, value)  # read-only properties raise here
 
     def copy(self):
-        return copy.deepcopy(self)
+        return self  # immutable
 
     def data(self):
-        return self._data
+        return tuple(self)
 
-    @property
-    def scale(self):
-        return self._data[0]
-
-    @property
-    def birefringence(self):
-        return self._data[1]
-
-    @property
-    def absorption(self):
-        return self._data[2]
-
-    @property
-    def model(self):
-        return self._data[3]
+    scale = property(lambda self: self[0], doc='radius scale relative to the fiber radius')
+    birefringence = property(lambda self: self[1], doc='birefringence dn')
+    absorption = property(lambda self: self[2], doc='absorption coefficient mu [1/mm]')
+    model = property(lambda self: self[3], doc="'b' background, 'p' parallel, 'r' radial")
 
 
-def _sort_and_check(lys):
-    keys = sorted(range(len(lys)), key=lambda i: lys[i][0])
-    if any(keys[i] > keys[i + 1] for i in range(len(keys) - 1)):
-        warnings.warn('layers had to be sorted')
-        lys = [lys[i] for i in keys]
-    if any(lys[i][0] == lys[i + 1][0] for i in range(len(lys) - 1)):
+def _ordered(layers):
+    """`layers` sorted by scale (stable; warns when the order changes) with
+    distinct scales (layers.py:91-99)."""
+    order = sorted(range(len(layers)), key=lambda k: layers[k].scale)
+    if order != list(range(len(layers))):
+        warnings.warn('layers had to be sorted', stacklevel=3)
+        layers = [layers[k] for k in order]
+    scales = [ly.scale for ly in layers]
+    if any(a == b for a, b in zip(scales, scales[1:])):
         raise ValueError('detected layers of same radii')
-    return lys
+    return list(layers)
 
 
-def _check_layers(lys):
-    """Validate and sort a layer list (layers.py:76-99)."""
-    if lys is None:
+def as_layer_list(value):
+    """A list of Layer from a layer, a sequence of layers or Layers (layers.py:76-99);
+    None stays None."""
+    if value is None:
         return None
-    if isinstance(lys, Layers):
-        lys = lys.data()
-    if all(isinstance(ly, (list, tuple, Layer)) for ly in lys):
-        lys = [Layer(ly) for ly in lys]
-    elif isinstance(lys, (list, tuple, Layer)):
-        lys = [Layer(lys)]
+    items = list(value)
+    if all(isinstance(it, (list, tuple, Layer)) for it in items):
+        layers = [Layer(it) for it in items]
+    elif isinstance(value, (list, tuple, Layer)):
+        layers = [Layer(value)]
     else:
         raise TypeError('Expected "Layer" or "[Layer]"')
-    if any(ly.scale > 1 for ly in lys):
-        warnings.warn('layer scale > 1 detected')
-    return _sort_and_check(lys)
+    if any(ly.scale > 1 for ly in layers):
+        warnings.warn('layer scale > 1 detected', stacklevel=2)
+    return _ordered(layers)
 
 
-class Layers:
-    """Sorted list of :class:`Layer` of one fiber bundle."""
-
-    __is_frozen = False
-
-    def __setattr__(self, key, value):
-        if self.__is_frozen and not hasattr(self, key):
-            raise TypeError(f'{self!r} is a frozen class')
-        object.__setattr__(self, key, value)
-
-    def __freeze(self):
-        self.__is_frozen = True
+class Layers(Sealed):
+    """The layers of one fiber bundle, sorted by scale (None: not set)."""
 
     def __init__(self, data):
-        self._data = _check_layers(data)
-        self.__freeze()
+        self._layers = as_layer_list(data)
+        self._seal()
 
     def __getitem__(self, item):
-        return self._data[item]
+        return self._layers[item]
 
     def __setitem__(self, item, value):
-        lys = list(self._data)
-        lys[item] = Layer(value)
-        self._data = _sort_and_check(lys)
-
-    def __str__(self):
-        return str(self._data)
-
-    def __repr__(self):
-        return repr(self._data)
-
-    def copy(self):
-        return copy.deepcopy(self)
+        layers = list(self._layers)
+        layers[item] = Layer(value)
+        self._layers = _ordered(layers)
+        edited()
 
     def __iter__(self):
-        if self._data is None:
-            return iter(())
-        return iter(self._data)
+        return iter(self._layers if self._layers is not None else ())
 
     def __len__(self):
-        return 0 if self._data is None else len(self._data)
+        return len(self._layers) if self._layers is not None else 0
+
+    def __str__(self):
+        return str(self._layers)
+
+    def __repr__(self):
+        return repr(self._layers)
+
+    def copy(self):
+        new = Layers(None)
+        object.__setattr__(new, '_layers', None if self._layers is None else list(self._layers))
+        return new
 
     def is_none(self):
-        return self._data is None
+        return self._layers is None
 
     def data(self):
         return [ly.data() for ly in self]

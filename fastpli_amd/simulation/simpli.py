@@ -21,6 +21,7 @@ import warnings
 import numpy as np
 
 from .. import _lib
+from ..objects._sealed import edit_count
 from ._base import _SimpliBase
 
 
@@ -87,32 +88,8 @@ class Simpli(_SimpliBase):
             raise NotImplementedError(
                 'MPI domain decomposition is not supported: one GPU holds the whole volume')
         self.mpi = None
-        self._freeze()
-
-    # ------------------------------------------------------------ properties
-    @property
-    def interpolate(self):
-        """optical axis interpolation: 'NN', 'Lerp' or 'Slerp'"""
-        return self._interpolate
-
-    @interpolate.setter
-    def interpolate(self, interpolate):
-        if interpolate not in ('NN', 'Lerp', 'Slerp'):
-            raise ValueError('Only \'NN\', \'Lerp\' or \'Slerp\' are supported')
-        self._interpolate = interpolate
-
-    @property
-    def omp_num_threads(self):
-        """kept for API compatibility; the GPU path does not use OpenMP"""
-        return self._omp_num_threads
-
-    @omp_num_threads.setter
-    def omp_num_threads(self, num_threads):
-        if not isinstance(num_threads, int):
-            raise TypeError('num_threads != int')
-        if num_threads <= 0:
-            raise TypeError('num_threads <= 0')
-        self._omp_num_threads = num_threads
+        self._model_cache = None  # (fiber bundles, edit count, device, flattened model)
+        self._seal()
 
     # ------------------------------------------------------------ generation
     def _dim_checked(self):
@@ -122,11 +99,19 @@ class Simpli(_SimpliBase):
         return dim
 
     def _device_fiber_model(self, dev):
+        """The fiber model flattened and its points on the device, kept while the
+        model object is the same and unedited (objects' edit counter)."""
         torch = _torch()
+        c = self._model_cache
+        if (c is not None and c[0] is self._fiber_bundles and c[1] == edit_count() and
+                c[2] == dev):
+            return c[3]
         points, fiber_ptr, bundle_ptr, layer_ptr, table = self._fiber_bundles.to_flat()
         points_d = torch.from_numpy(points).to(dev)
         fiber_ptr_d = torch.from_numpy(fiber_ptr).to(dev)
-        return points_d, fiber_ptr_d, bundle_ptr, layer_ptr, table
+        model = (points_d, fiber_ptr_d, bundle_ptr, layer_ptr, table)
+        self._model_cache = (self._fiber_bundles, edit_count(), dev, model)
+        return model
 
     def _generate(self, only_tissue, fields, with_arrays=True):
         self._check_volume_input()

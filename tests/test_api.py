@@ -134,15 +134,64 @@ def test_fiber_containers_and_flat():
         objects.FiberBundles([[[[0, 0, 0, 1]]]], [[(1, 0, 0, 'b')], [(1, 0, 0, 'b')]])
 
 
-def test_dat_loader_round_trip(tmp_path):
-    fbs = fio.load(os.path.join(GOLDEN, 'cube.dat'))
-    assert len(fbs) == 1 and len(fbs[0]) == 100
-    assert sum(f.shape[0] for f in fbs[0]) == 1994
+def _dat_line_by_line(text):
+    """The .dat grammar read one line at a time (empty line: end of fiber;
+    a second one in a row: end of bundle), to check the run-based reader."""
+    fiber, bundles, empty = [], [[]], 0
+    for line in text.splitlines():
+        if line.strip():
+            empty = 0
+            fiber.append([float(v) for v in line.split()[:4]])
+            continue
+        empty += 1
+        if empty == 1 and fiber:
+            bundles[-1].append(np.array(fiber))
+            fiber = []
+        elif empty == 2 and bundles[-1]:
+            bundles.append([])
+    if fiber:
+        bundles[-1].append(np.array(fiber))
+    return bundles
+
+
+@pytest.mark.parametrize('text', [
+    '1 2 3 4\n5 6 7 8\n\n9 9 9 9\n\n\n1 1 1 1\n',        # two bundles
+    '\n\n\n1 2 3 4\n\n\n\n\n5 6 7 8 9 10\n  \n',        # leading / long blank runs, 6 numbers
+    '1 2 3 4\n\n\n',                                    # trailing empty bundle
+    '1 2 3 4\n2 3 4 5',                                   # no final newline
+    '',
+])
+def test_dat_reader_grammar(tmp_path, text):
+    p = tmp_path / 'g.dat'
+    p.write_text(text)
+    got = fio.load(str(p))
+    ref = _dat_line_by_line(text)
+    assert len(got) == len(ref)
+    for gb, rb in zip(got, ref):
+        assert len(gb) == len(rb)
+        for gf, rf in zip(gb, rb):
+            assert np.array_equal(gf.as_array(), rf)
+
+
+@pytest.mark.parametrize('name, bundles, n_points', [('cube.dat', [100], 1994),
+                                                     ('optic_chiasm.dat', [61, 61, 19, 19],
+                                                      6465)])
+def test_dat_models_and_exact_round_trip(tmp_path, name, bundles, n_points):
+    path = os.path.join(GOLDEN, name)
+    fbs = fio.load(path)
+    assert [len(b) for b in fbs] == bundles
+    assert sum(f.shape[0] for b in fbs for f in b) == n_points
+    ref = _dat_line_by_line(open(path).read())
+    assert all(np.array_equal(f.as_array(), r) for b, rb in zip(fbs, ref) for f, r in zip(b, rb))
     p = tmp_path / 'x.dat'
     fio.save(str(p), fbs)
+    with pytest.raises(FileExistsError):
+        fio.save(str(p), fbs)  # mode 'w-'
     again = fio.load(str(p))
-    for a, b in zip(fbs[0], again[0]):
-        np.testing.assert_allclose(a.as_array(), b.as_array(), atol=1e-6)
+    assert [len(b) for b in again] == bundles
+    for b, bb in zip(fbs, again):
+        for f, g in zip(b, bb):
+            assert np.array_equal(f.as_array(), g.as_array())
 
 
 def test_field_storage_property():
@@ -192,3 +241,46 @@ def test_near_parallel_tilt_is_rejected_on_the_host():
     with pytest.raises(ValueError, match='z-steps per ray'):
         s.count_voxel_steps(1.5707963, 0.0)
     assert s.count_voxel_steps(np.deg2rad(60), 0.0) > 0
+
+
+def test_fiber_model_edits_are_counted():
+    """Every edit of a fiber model object bumps the counter the device copy of
+    a model is keyed on; arrays handed out are read-only snapshots."""
+    from fastpli_amd.objects._sealed import edit_count
+    fbs = objects.FiberBundles([[np.array([[0.0, 0, 0, 1], [1.0, 0, 0, 1]])]],
+                               [[(1.0, -0.004, 1, 'p')]])
+    flat0 = fbs.to_flat()
+    edits = [
+        lambda: fbs[0][0].__setitem__((0, 0), 5.0),
+        lambda: setattr(fbs[0][0], 'radii', [2.0, 2.0]),
+        lambda: fbs[0].append(np.array([[0.0, 1, 0, 1], [1.0, 1, 0, 1]])),
+        lambda: fbs[0].layers.__setitem__(0, (1.0, -0.002, 1, 'p')),
+        lambda: setattr(fbs[0], 'layers', [(1.0, -0.001, 1, 'p')]),
+        lambda: fbs.append(fbs[0]),
+        lambda: fbs.__delitem__(1),
+    ]
+    for edit in edits:
+        n = edit_count()
+        edit()
+        assert edit_count() == n + 1
+    pts = fbs[0][0].points
+    with pytest.raises(ValueError):
+        pts[0, 0] = 7.0  # read-only snapshot
+    assert fbs.to_flat()[0][0, 0] == 5.0 and flat0[0][0, 0] == 0.0
+    assert fbs.to_flat()[4][1][0] == -0.001
+
+
+def test_install_alias_gives_the_reference_module_paths():
+    """fastpli_amd.compat.install_alias() makes `from fastpli.simulation import
+    Simpli` (and .objects, .io) this package's classes; run in a fresh process."""
+    import subprocess
+    import sys
+    code = ('import fastpli_amd, fastpli_amd.compat as c; c.install_alias();'
+            'from fastpli.simulation import Simpli; from fastpli.objects import FiberBundles;'
+            'import fastpli.io;'
+            'import fastpli_amd.simulation as s, fastpli_amd.objects as o;'
+            'assert Simpli is s.Simpli and FiberBundles is o.FiberBundles;'
+            'assert fastpli.io.fiber_bundles is fastpli_amd.io.fiber_bundles;'
+            'print("ok")')
+    out = subprocess.run([sys.executable, '-c', code], cwd=ROOT, capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip() == 'ok', out.stderr

@@ -72,3 +72,37 @@ def test_config_full_size(name):
         ref, _ = oracle.simulate(label, vector, props, theta, phi, voxel_size=0.2,
                                  filter_rotations=ROT9, interpolate='Slerp', n_threads=THREADS)
         assert_intensity_close(imgs[t], ref, 26000)
+
+
+def test_real_fiber_model_optic_chiasm():
+    """The reference's example model examples/optic_chiasm.dat (4 bundles, 160
+    fibers, radii 50-100 um) in the example's setup (optic_chiasm.py:80-101):
+    5 um voxels over [-2000, 2000]^2 x [-50, 50] um (800 x 800 x 20), layers
+    [(1.0, -0.0001, 5, 'p')], 6 rotations, the 5-tilt set, Slerp.  Labels and
+    axes bit-exact, intensities to 1e-4 against the oracle."""
+    from fastpli_amd.io import fiber_bundles as fio
+    from tests.conftest import GOLDEN, GOLDEN_ROT
+    _say('real model: optic_chiasm.dat 800x800x20 vs oracle')
+    fbs = fio.load(os.path.join(GOLDEN, 'optic_chiasm.dat'))
+    fbs.layers = [[(1.0, -0.0001, 5, 'p')]] * len(fbs)
+    s = Simpli()
+    s.voxel_size = 5
+    s.set_voi([-2000, -2000, -50], [2000, 2000, 50])
+    s.fiber_bundles = fbs
+    s.filter_rotations = GOLDEN_ROT
+    s.light_intensity = 26000
+    s.wavelength = 525
+    assert tuple(s.dim) == (800, 800, 20)
+    with warnings.catch_warnings():
+        warnings.simplefilter('ignore')
+        label, vector, props = s.generate_tissue()
+    ol, ov, op, _ = oracle.generate(fbs, tuple(s.dim), s.dim_origin, 5.0, n_threads=THREADS)
+    assert np.array_equal(label, ol), f'{np.count_nonzero(label != ol)} labels differ'
+    assert np.array_equal(vector, ov)
+    assert np.count_nonzero(label) > 0.2 * label.size
+    imgs = s.run_simulation_tilts(label, vector, props, TILTS5)
+    for t, (theta, phi) in enumerate(TILTS5):
+        ref, _ = oracle.simulate(label, vector, props, theta, phi, voxel_size=5.0,
+                                 filter_rotations=GOLDEN_ROT, interpolate='Slerp',
+                                 n_threads=THREADS)
+        assert_intensity_close(imgs[t], ref, 26000)
