@@ -170,7 +170,7 @@ def lib():
         msgargs = [ctypes.c_char_p, ctypes.c_size_t]
         L.fpl_rofl.restype = ctypes.c_int
         L.fpl_rofl.argtypes = [_c_p, _c_i64, _c_i64, ctypes.c_double, ctypes.c_double,
-                               ctypes.c_double, _c_p, _c_p, _c_p] + msgargs
+                               ctypes.c_double, ctypes.c_int, _c_p, _c_p, _c_p] + msgargs
         L.fpl_server_create.restype = ctypes.c_int
         L.fpl_server_create.argtypes = [ctypes.c_int, _c_i64, ctypes.c_int32,
                                         ctypes.POINTER(_c_p)] + msgargs
@@ -450,11 +450,12 @@ def epa(images, t, d, r):
     check(rc, msg)
 
 
-def rofl(images, tilt_angle, gain, dir_offset, mask, out):
+def rofl(images, tilt_angle, gain, dir_offset, mask, out, grad_mode=False):
     """fpl_rofl on a float64 device tensor (5, ..., n_rho) -> out (8, ...)."""
     nr = images.shape[-1]
     n_pix = images.numel() // (5 * nr)
     msg = ctypes.create_string_buffer(512)
     rc = lib().fpl_rofl(_ptr(images), n_pix, nr, float(tilt_angle), float(gain),
-                        float(dir_offset), _ptr(mask), _ptr(out), _stream(), msg, 512)
+                        float(dir_offset), int(bool(grad_mode)), _ptr(mask), _ptr(out),
+                        _stream(), msg, 512)
     check(rc, msg)

@@ -425,11 +425,11 @@ class Simpli(_SimpliBase):
         data: (5, x, y, rho) images of the tilts (0, 0), (tau, 0), (tau, 90 deg),
         (tau, 180 deg), (tau, 270 deg).  Returns (directionmap, inclmap, trelmap,
         (dirdevmap, incldevmap, treldevmap, funcmap, itermap)).  Runs one GPU
-        thread per pixel in float64 (Levenberg-Marquardt with the exact
-        Jacobian); the results agree with the reference's MINPACK fit to its
-        convergence tolerance, and itermap counts this fit's model
-        evaluations.  ``mp_pool`` and ``grad_mode`` are accepted for API
-        compatibility (the fit always uses the exact Jacobian).
+        thread per pixel in float64: the reference's fit as
+        scipy.optimize.leastsq runs it (maxfev 150), MINPACK lmdif with forward
+        differences, or with ``grad_mode=True`` lmder with the reference's
+        analytic per-view Jacobian; itermap is its nfev.  ``mp_pool`` is
+        accepted for API compatibility.
         """
         self._print('Apply rofl')
         if self._tilts is None:
@@ -464,7 +464,7 @@ class Simpli(_SimpliBase):
             if tuple(m.shape) != (nx, ny):
                 raise ValueError('mask shape does not match the images')
         out = torch.empty((8, nx, ny), dtype=torch.float64, device=dev)
-        _lib.rofl(x, tilt_angle, 3.0, 0.0, m, out)
+        _lib.rofl(x, tilt_angle, 3.0, 0.0, m, out, grad_mode)
         if itemsize <= 4:
             out = out.to(torch.float32)
         if not as_tensor:

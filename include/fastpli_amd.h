@@ -241,12 +241,14 @@ int fpl_epa(const double *in, int64_t n_pix, int64_t n_rho, double *transmittanc
  * (src/fastpli/simulation/__simpli.py:623-706) for every pixel.
  *   in   DEVICE float64 (5, n_pix, n_rho): the tilts (0, 0), (tau, 0),
  *        (tau, pi/2), (tau, pi), (tau, 3 pi/2), n_rho equidistant in [0, pi)
+ *   grad_mode non-zero: the fit uses the reference's analytic Jacobian
+ *        (rofl's grad_mode, MINPACK lmder); zero: finite differences (lmdif)
  *   mask DEVICE uint8 (n_pix) or NULL; masked-out pixels get NaN
  *   out  DEVICE float64 (8, n_pix): direction, inclination, t_rel, their
  *        deviations, chi^2, model evaluations */
 int fpl_rofl(const double *in, int64_t n_pix, int64_t n_rho, double tilt_angle, double gain,
-             double dir_offset, const uint8_t *mask, double *out, void *stream, char *msg,
-             size_t msg_len);
+             double dir_offset, int grad_mode, const uint8_t *mask, double *out, void *stream,
+             char *msg, size_t msg_len);
 
 /* Exact count of Mueller voxel-steps of one tilt: n_rho x rays x z-march
  * iterations (simulator.cpp:259-261).  Host only.  -1 for a tilt whose rays

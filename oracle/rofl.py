@@ -4,8 +4,8 @@ Follows src/fastpli/analysis/rofl.py:17-60 and the local-fit path of
 src/fastpli/analysis/_ROFL_with_jacobi.py (Schmitz et al., Front. Neuroanat.
 2018, doi:10.3389/fnana.2018.00075): centered intensities and weights
 (:47-57), a 6 x 6 brute-force start over inclination and t_rel (:313-370),
-scipy.optimize.leastsq (MINPACK lmdif, finite differences, maxfev 150) on the
-weighted residual (:383-400, :433-475), angle symmetrization (:60-69) and
+scipy.optimize.leastsq (MINPACK lmdif with finite differences, or lmder with
+the analytic Jacobian in grad mode; maxfev 150) on the weighted residual (:383-400, :433-475), angle symmetrization (:60-69) and
 the covariance from the Jacobian at the minimum (:489-522).  The reference
 wraps the model in numba; the arithmetic here is the same numpy arithmetic.
 
@@ -142,8 +142,17 @@ def brute_force(phi, n_incl, n_trel, n_rho, mats, dir_offset, tau, measures, wei
     return phi, grid[k][0], grid[k][1]
 
 
-def fit(stack, tilt_angle=np.deg2rad(5.5), gain=3, dir_offset=0):
-    """rofl.rofl (rofl.py:17-60) on one (5, n_rho) stack, local fit, no gradient.
+def _weighted_jacobi(p, mats, tau, n_rho, dir_offset, _, weights):
+    """_weighted_jacobi (_ROFL_with_jacobi.py:383-393): the Dfun of grad mode."""
+    d, a, t = rotated_params(p[0], p[1], p[2], mats, tau)
+    return jacobi(d, a, t, n_rho, dir_offset) * weights
+
+
+def fit(stack, tilt_angle=np.deg2rad(5.5), gain=3, dir_offset=0, grad=False):
+    """rofl.rofl (rofl.py:17-60) on one (5, n_rho) stack, local fit.
+
+    grad: leastsq with Dfun = the weighted analytic Jacobian (MINPACK lmder),
+    as _ROFL_with_jacobi.py:467-475; else finite differences (lmdif, :476-480).
 
     Returns (direction, inclination, t_rel, dir_dev, incl_dev, trel_dev, chi^2,
     function evaluations).
@@ -165,7 +174,12 @@ def fit(stack, tilt_angle=np.deg2rad(5.5), gain=3, dir_offset=0):
     weights = weights.ravel()
     args = (mats, tilt_angle, n_rho, dir_offset, centered, weights)
     start = brute_force(phi0, 6, 6, n_rho, mats, dir_offset, tilt_angle, centered, weights)
-    x, _, info, _, _ = optimize.leastsq(_residuum, start, args=args, full_output=1, maxfev=150)
+    if grad:
+        x, _, info, _, _ = optimize.leastsq(_residuum, start, args=args, full_output=1,
+                                            maxfev=150, Dfun=_weighted_jacobi, col_deriv=1)
+    else:
+        x, _, info, _, _ = optimize.leastsq(_residuum, start, args=args, full_output=1,
+                                            maxfev=150)
     fvalue = np.sum(np.square(info['fvec']))
     d, a = symmetrize(x[0], x[1])
     t = np.abs(x[2])
@@ -177,7 +191,7 @@ def fit(stack, tilt_angle=np.deg2rad(5.5), gain=3, dir_offset=0):
     return d, a, t, dev[0], dev[1], dev[2], fvalue, info['nfev']
 
 
-def rofl_map(data, tilt_angle=np.deg2rad(5.5), gain=3, dir_offset=0, mask=None):
+def rofl_map(data, tilt_angle=np.deg2rad(5.5), gain=3, dir_offset=0, mask=None, grad=False):
     """Simpli.apply_rofl (__simpli.py:623-706) without the pool: data (5, nx, ny, n_rho).
 
     Returns (direction, inclination, t_rel, (dir_dev, incl_dev, trel_dev, chi^2, nfev)).
@@ -190,5 +204,5 @@ def rofl_map(data, tilt_angle=np.deg2rad(5.5), gain=3, dir_offset=0, mask=None):
     for i in range(nx):
         for j in range(ny):
             if mask[i, j]:
-                out[:, i, j] = fit(data[:, i, j, :], tilt_angle, gain, dir_offset)
+                out[:, i, j] = fit(data[:, i, j, :], tilt_angle, gain, dir_offset, grad)
     return out[0], out[1], out[2], tuple(out[3:])

@@ -65,3 +65,54 @@ def test_rofl_noise_free_model_recovers_parameters():
         out = rofl.fit(stack, TILTS[1, 0])
         assert abs(out[0] - phi) < 1e-6 and abs(abs(out[1]) - abs(alpha)) < 1e-6
         assert abs(out[2] - tr) < 1e-6
+
+
+def test_rofl_jacobi_matches_central_differences():
+    # _calc_Jacobi's rows are the derivatives of the model intensities of each
+    # view w.r.t. that view's (phi, alpha, t_rel); inside the symmetrization's
+    # identity range a central difference of `intensities` fixes every term
+    rng = np.random.default_rng(11)
+    n_rho, h = 9, 1e-6
+    for _ in range(5):
+        phi = rng.uniform(0.1, 3.0, 5)
+        alpha = rng.uniform(-1.4, 1.4, 5)
+        t_rel = rng.uniform(0.05, 0.95, 5)
+        J = rofl.jacobi(phi, alpha, t_rel, n_rho, 0.3).reshape(3, 5, n_rho)
+        for col in range(3):
+            for j in range(5):
+                hi = [phi.copy(), alpha.copy(), t_rel.copy()]
+                lo = [phi.copy(), alpha.copy(), t_rel.copy()]
+                hi[col][j] += h
+                lo[col][j] -= h
+                fd = (rofl.intensities(*hi, n_rho, 0.3) -
+                      rofl.intensities(*lo, n_rho, 0.3)).reshape(5, n_rho)[j] / (2 * h)
+                np.testing.assert_allclose(J[col, j], fd, atol=2e-8)
+
+
+def test_rofl_grad_mode_descends_and_reports_its_residual():
+    # grad mode is leastsq with Dfun = _weighted_jacobi (MINPACK lmder, the
+    # library routine); its Dfun is pinned above, but it is not the Jacobian of
+    # the residual (it leaves out the tilt rotation's chain rule), so the fit
+    # may stop short of the minimum (the closed-form voxel: 0.5 deg off, maxfev
+    # reached).  Fixed by the method regardless: the reported chi^2 is the
+    # residual at the returned parameters, and lmder accepts no step that
+    # raises it above the brute-force start's
+    stacks = [single_voxel_stack()[:, 0, 0, :]]
+    rng = np.random.default_rng(4)
+    mats = rofl.tilt_matrices(4, TILTS[1, 0])
+    for _ in range(5):
+        phi, alpha, tr = rng.uniform(0.2, 2.9), rng.uniform(-1.2, 1.2), rng.uniform(0.1, 0.8)
+        dd, aa, tt = rofl.rotated_params(phi, alpha, tr, mats, TILTS[1, 0])
+        stacks.append(1000.0 * (1.0 + rofl.intensities(dd, aa, tt, 9, 0).reshape(5, 9)))
+    for stack in stacks:
+        out = rofl.fit(stack, TILTS[1, 0], grad=True)
+        centered, weights = rofl.centered_values(stack, 3)
+        centered, weights = centered.ravel(), weights.ravel()
+        n_rho = stack.shape[1]
+        args = (mats, TILTS[1, 0], n_rho, 0, centered, weights)
+        chi2 = np.sum(np.square(rofl._residuum(out[:3], *args)))
+        assert abs(chi2 - out[6]) <= 1e-9 * max(1.0, out[6])
+        start = rofl.brute_force(rofl.epa_direction(stack[0]), 6, 6, n_rho, mats, 0, TILTS[1, 0],
+                                 centered, weights)
+        assert out[6] <= np.sum(np.square(rofl._residuum(start, *args))) * (1 + 1e-12)
+        assert 1 <= out[7] <= 150
