@@ -6,13 +6,16 @@ layers [(0.75, 0, 5, 'b'), (1.0, -0.004, 1, 'p')], 5 tilts {(0), (5.5 deg,
 phi = 0/90/180/270)} x 9 polarizer rotations, Slerp interpolation.
 
 One step = generate_tissue (labels + optical axes) + run_simulation of all 5
-tilts, with the fiber model already resident in HBM.  The unit of work is one
+tilts, with the fiber model already resident in HBM (`value`); `api_path`
+times the same step through the Simpli API, with and without the model's
+flatten + upload, and once through host arrays.  The unit of work is one
 Mueller voxel-step = (tilt, ray, z-step, rotation); a step does
 9 * (360000 * 300 + 4 * 360000 * 301) = 4.873e9 of them.
 
 Prints one JSON line.  The CPU baseline is the FP64 OpenMP oracle (a port of
 the reference's C++ core) timed on the same workload (by default the whole
-volume, one step; --cpu-fraction bounds it to a y-slab).  The `serving`
+volume, median of 3 steps after a warm-up; --cpu-fraction bounds it to a
+y-slab).  The `serving`
 object reports p50 / p99 latency of a mixed-size request stream.
 """
 
@@ -246,31 +249,109 @@ def serving_bench(n_requests, load, seed=0, pool=4):
     }
 
 
-def cpu_baseline(fbs, fraction, threads):
+def cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def cpu_threads():
+    """Every CPU this process may run on, capped by OMP_NUM_THREADS when set
+    (the CPU share a GPU job is given)."""
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get('OMP_NUM_THREADS')
+    return min(n, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else n
+
+
+def cpu_baseline(fbs, fraction, threads, reps=3):
     """Oracle (FP64, OpenMP) on the y-slab [0, fraction * 120 um] of the workload
-    (fraction 1: the whole volume)."""
+    (fraction 1: the whole volume): one warm-up, then the median of `reps`
+    timed steps.  Like the GPU's timed step, the timed region starts with the
+    fiber model flattened and the output arrays allocated."""
     import oracle
     dim = (int(round(VOI[0] / VOXEL)), max(1, int(round(VOI[1] * fraction / VOXEL))),
            int(round(VOI[2] / VOXEL)))
     s = make_simpli(dim, fbs)
     oracle.lib()
-    t0 = time.perf_counter()
-    label, vec, props, _ = oracle.generate(fbs, dim, np.zeros(3), VOXEL, n_threads=threads)
-    for theta, phi in TILTS:
-        oracle.simulate(label, vec, props, theta, phi, voxel_size=VOXEL, filter_rotations=ROT,
-                        interpolate='Slerp', n_threads=threads)
-    dt = time.perf_counter() - t0
+    flat = fbs.to_flat()
+    label = np.zeros(dim, dtype=np.int32)
+    vec = np.zeros(tuple(dim) + (3,), dtype=np.float32)
+    images = [np.zeros((dim[0], dim[1], ROT.size)) for _ in TILTS]
+
+    def one():
+        t0 = time.perf_counter()
+        _, _, props, _ = oracle.generate(flat, dim, np.zeros(3), VOXEL, n_threads=threads,
+                                         out=(label, vec))
+        for (theta, phi), img in zip(TILTS, images):
+            oracle.simulate(label, vec, props, theta, phi, voxel_size=VOXEL,
+                            filter_rotations=ROT, interpolate='Slerp', n_threads=threads,
+                            out=img)
+        return time.perf_counter() - t0
+
+    one()
+    runs = [one() for _ in range(reps)]
+    dt = float(np.median(runs))
     w = voxel_steps(s)
     return {
         'value': w / dt,
         'unit': 'Mueller voxel-steps/s',
         'cores': threads,
         'kind': 'port',
+        'cpu_model': cpu_model(),
+        'cpus_available': len(os.sched_getaffinity(0)),
+        'runs_s': runs,
+        'build': 'g++ -O3 -fopenmp -ffp-contract=off (oracle/Makefile)',
         'sample': (('whole volume' if fraction >= 1 else 'y-slab') +
                    f' {dim[0]}x{dim[1]}x{dim[2]} voxels of the same workload '
-                   f'(generate + 5 tilts x 9 rotations, Slerp), {w:.3e} voxel-steps in '
-                   f'{dt:.2f} s, FP64 OpenMP oracle'),
+                   f'(generate + 5 tilts x 9 rotations, Slerp), {w:.3e} voxel-steps, '
+                   f'median of {reps} steps after a warm-up, FP64 OpenMP oracle, fiber model '
+                   'flattened and outputs allocated before timing'),
     }
+
+
+def api_path_bench(fbs, dim, storage, steps, work):
+    """The step through the Simpli API with device tensors:
+    generate_tissue_fields (label + optical axis + march records) +
+    run_simulation_fields (5 tilts).  'cached': the fiber model unchanged
+    between steps (flattened and uploaded once); 'model_edited': the model
+    counts as edited before every step, so every step flattens it
+    (FiberBundles.to_flat) and uploads its points.  Also one step through the
+    numpy API (generate_tissue + run_simulation_tilts on host arrays: the
+    fields cross PCIe twice)."""
+    import torch
+    from fastpli_amd.objects._sealed import edited
+    s = make_simpli(dim, fbs)
+    s.field_storage = storage
+
+    def step():
+        _, _, props, fields = s.generate_tissue_fields()
+        return s.run_simulation_fields(fields, props, TILTS)
+
+    out = {}
+    for name, edit in (('cached', False), ('model_edited', True)):
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if edit:
+                edited()
+            step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[name] = {'ms_per_step': dt * 1e3, 'value': work / dt}
+    t0 = time.perf_counter()
+    label, vector, props = s.generate_tissue()
+    s.run_simulation_tilts(label, vector, props, TILTS)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out['numpy_host_arrays'] = {'ms_per_step': dt * 1e3, 'value': work / dt, 'steps': 1}
+    out['unit'] = 'Mueller voxel-steps/s'
+    return out
 
 
 def main():
@@ -397,9 +478,12 @@ def main():
         log(f'serving stream: {args.serve_requests} requests at load {args.serve_load}')
         serving = serving_bench(args.serve_requests, args.serve_load)
 
+    log('API path: Simpli.generate_tissue_fields + run_simulation_fields')
+    api = api_path_bench(fbs, dim, args.field_storage, max(3, args.steps // 2), work)
+
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         log(f'cpu baseline: oracle, {threads} threads, fraction {args.cpu_fraction}')
         cpu = cpu_baseline(fbs, args.cpu_fraction, threads)
 
@@ -432,6 +516,14 @@ def main():
             'field_storage': args.field_storage,
         },
         'roofline': roofline,
+        # the step's algorithmic bytes (voxelizer writes + march reads and
+        # writes) over its wall time, against the HBM peak
+        'whole_step': {'algorithmic_bytes': alg_bytes['voxelize'] + alg_bytes['march'],
+                       'achieved_GBs': (alg_bytes['voxelize'] + alg_bytes['march']) /
+                       (ms_step * 1e-3) / 1e9,
+                       'frac': (alg_bytes['voxelize'] + alg_bytes['march']) /
+                       (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        'api_path': api,
         'kernels': kernels,
         'cpu_baseline': cpu,
         'serving': serving,

@@ -163,10 +163,9 @@ template <typename Idx, int ST>
 __device__ __forceinline__ Vox Field<Idx, ST>::fetch(double px, double py, double fz,
                                                      bool live) const {
    const int32_t jx = ifloor(px), jy = ifloor(py);
-   const int32_t jz = live ? (int32_t)fz : 0;
    const bool in = live & ((uint32_t)jx < (uint32_t)nx) & ((uint32_t)jy < (uint32_t)ny);
    Vox r{0, make_float3(0.0f, 0.0f, 0.0f), false};
-   if (in) r = at(jx, jy, jz);
+   if (in) r = at(jx, jy, (int32_t)fz);
    return r;
 }
 
@@ -188,21 +187,30 @@ __device__ __forceinline__ float3 lerp_orient(const float3 &v, const float3 &u, 
 }
 
 // sin(a theta) / sin(theta) = a 2F1(1 - a, 1 + a; 3/2; z), z = sin^2(theta/2),
-// theta in [0, pi/2].  Term ratio: (k^2 - a^2) z / ((k + 1/2) k).  For
-// z <= 0.1 (theta <= 37 deg) eight terms in Horner form leave < 1e-9 (one tier
-// for all small angles: lanes of a wave then take the same path); beyond that
-// the series is summed until a term drops below 1e-8 (at most 27 terms).
-__device__ __forceinline__ float slerp_weight(float a, float z) {
-   const float a2 = a * a;
-   if (z <= 0.1f) {
-      float acc = 1.0f;
+// theta in [0, pi/2].  Term ratio: (k^2 - a^2) z / ((k + 1/2) k), at most z in
+// magnitude for a in [0, 1], so K terms leave less than z^(K+1) / (1 - z).
+// Both weights of a slerp, a = 1 - t and a = t, share z: the Horner terms
+// k^2 z / ((k + 1/2) k) are computed once.  Tiers, chosen for the whole wave
+// (lanes then take the same path): z <= 0.025 (theta <= 18 deg) four terms
+// (< 1.1e-8), z <= 0.1 (theta <= 37 deg) eight terms (< 1.2e-9); beyond, the
+// series is summed until a term drops below 1e-8 (at most 27 terms).
+template <int K>
+__device__ __forceinline__ void slerp_horner(float a, float b, float z, float &wa, float &wb) {
+   const float qa = z * (a * a), qb = z * (b * b);
+   float acc_a = 1.0f, acc_b = 1.0f;
 #pragma unroll
-      for (int k = 8; k >= 1; k--) {
-         const float r = ((float)(k * k) - a2) * (1.0f / ((k + 0.5f) * (float)k));
-         acc = fmaf(z * r, acc, 1.0f);
-      }
-      return a * acc;
+   for (int k = K; k >= 1; k--) {
+      const float c = 1.0f / ((k + 0.5f) * (float)k);
+      const float pk = z * ((float)(k * k) * c);
+      acc_a = fmaf(fmaf(-qa, c, pk), acc_a, 1.0f);
+      acc_b = fmaf(fmaf(-qb, c, pk), acc_b, 1.0f);
    }
+   wa = a * acc_a;
+   wb = b * acc_b;
+}
+
+__device__ __forceinline__ float slerp_series(float a, float z) {
+   const float a2 = a * a;
    float term = 1.0f, sum = 1.0f;
    for (int k = 1; k <= 40; k++) {
       const float kf = (float)k;
@@ -211,6 +219,18 @@ __device__ __forceinline__ float slerp_weight(float a, float z) {
       if (fabsf(term) < 1e-8f) break;
    }
    return a * sum;
+}
+
+__device__ __forceinline__ void slerp_weights(float t, float z, float &wa, float &wb) {
+   const float a = 1.0f - t;
+   if (__ballot(z > 0.025f) == 0) {
+      slerp_horner<4>(a, t, z, wa, wb);
+   } else if (z <= 0.1f) {
+      slerp_horner<8>(a, t, z, wa, wb);
+   } else {
+      wa = slerp_series(a, z);
+      wb = slerp_series(t, z);
+   }
 }
 
 // sin x and 1 - cos x for the per-voxel retardance; odd / even Taylor
@@ -242,8 +262,8 @@ __device__ __forceinline__ float3 slerp_orient(const float3 &v, float3 u, float 
    if (t == 0.0f) return v;
    const float c = fminf(d * rsqrtf(dot3(v, v) * dot3(u, u)), 1.0f);
    const float z = 0.5f * (1.0f - c);
-   const float wa = slerp_weight(1.0f - t, z);
-   const float wb = slerp_weight(t, z);
+   float wa, wb;
+   slerp_weights(t, z, wa, wb);
    return make_float3(v.x * wa + u.x * wb, v.y * wa + u.y * wb, v.z * wa + u.z * wb);
 }
 
@@ -282,11 +302,36 @@ __device__ float3 get_vec_interp(const Field<Idx, ST> &F, double px, double py, 
    auto ip = [&](const float3 &a, const float3 &b, float t) {
       return slerp ? slerp_orient(a, b, t) : lerp_orient(a, b, t);
    };
-   // every corner read at once (an axis without spread repeats its corners)
-   const Vox r000 = F.at(x0, y0, z0), r100 = F.at(x1, y0, z0);
-   const Vox r010 = F.at(x0, y1, z0), r110 = F.at(x1, y1, z0);
-   const Vox r001 = F.at(x0, y0, z1), r101 = F.at(x1, y0, z1);
-   const Vox r011 = F.at(x0, y1, z1), r111 = F.at(x1, y1, z1);
+   // every corner read at once.  An axis without spread in any lane of the
+   // wave (a view tilted along x samples voxel centres in y, and vice versa)
+   // repeats its corners: they are not read again (wave-uniform branches)
+   const bool xdup = __ballot(dx) == 0, ydup = __ballot(dy) == 0;
+   Vox r000, r100, r010, r110, r001, r101, r011, r111;
+   r000 = F.at(x0, y0, z0);
+   r001 = F.at(x0, y0, z1);
+   if (xdup) {
+      r100 = r000;
+      r101 = r001;
+   } else {
+      r100 = F.at(x1, y0, z0);
+      r101 = F.at(x1, y0, z1);
+   }
+   if (ydup) {
+      r010 = r000;
+      r011 = r001;
+      r110 = r100;
+      r111 = r101;
+   } else {
+      r010 = F.at(x0, y1, z0);
+      r011 = F.at(x0, y1, z1);
+      if (xdup) {
+         r110 = r010;
+         r111 = r011;
+      } else {
+         r110 = F.at(x1, y1, z0);
+         r111 = F.at(x1, y1, z1);
+      }
+   }
    // level x
    float3 c00 = corner(r000);
    if (dx) c00 = ip(c00, corner(r100), xd);
@@ -416,17 +461,19 @@ __device__ __forceinline__ bool retarder(const AxisRot &T, float K, float3 v, bo
    return true;
 }
 
-// u <- rotation of u by -delta about n = (cos 2phi, -sin 2phi, 0) (Rodrigues),
-// the increment added in float-float
+// u <- rotation of u by -delta about n = (cos 2phi, -sin 2phi, 0), the
+// increment added in float-float.  The component along n is unchanged; the
+// components w = m . u along m = (sin 2phi, cos 2phi, 0) and u3 turn by delta:
+// (w, u3) += (sin d u3 - (1 - cos d) w, -sin d w - (1 - cos d) u3), and the
+// change of w goes back along m (Rodrigues' formula with |n| = 1)
 __device__ __forceinline__ void apply_retarder(float C, float S, float sd, float omc, float &h1,
                                                float &h2, float &h3, float &l1, float &l2,
                                                float &l3) {
-   const float nv = C * h1 - S * h2;
-   const float d1 = sd * S * h3 + omc * (C * nv - h1);
-   const float d2 = sd * C * h3 + omc * (-S * nv - h2);
-   const float d3 = -sd * (C * h2 + S * h1) - omc * h3;
-   two_sum(h1, l1, d1);
-   two_sum(h2, l2, d2);
+   const float w = S * h1 + C * h2;
+   const float g = sd * h3 - omc * w;
+   const float d3 = -(sd * w + omc * h3);
+   two_sum(h1, l1, S * g);
+   two_sum(h2, l2, C * g);
    two_sum(h3, l3, d3);
 }
 
@@ -606,18 +653,23 @@ __global__ void __launch_bounds__(WG, 6)
                QB[slot].x = __float_as_uint(mu);
             }
             wave_sync();
-            // every ray applies its steps of this batch, oldest first
-            while (npend > 0 && (((pend & 255u) - head) & 255u) < n) {
-               const uint32_t slot = pend & (QCAP - 1);
+            // every ray applies its steps of this batch, oldest first: a
+            // prefix of its queued steps (their sequence numbers increase)
+            int k = 0;
+#pragma unroll
+            for (int q = 0; q < QDEPTH; q++)
+               k += (q < npend) & (((((pend >> (8 * q)) & 255u) - head) & 255u) < n);
+            for (int q = 0; q < k; q++) {
+               const uint32_t slot = (pend >> (8 * q)) & (QCAP - 1);
                const uint4 e = QA[slot];
                const float mu = __uint_as_float(QB[slot].x);
                const float C = __uint_as_float(e.x), S = __uint_as_float(e.y);
                const float sd = __uint_as_float(e.z), omc = __uint_as_float(e.w);
                apply_retarder(C, S, sd, omc, h1, h2, h3, l1, l2, l3);
-               if (mu != 0.0f) two_sum(mh, ml, mu);
-               pend >>= 8;
-               npend--;
+               two_sum(mh, ml, mu); // exact no-op for mu = 0
             }
+            pend = (uint32_t)((uint64_t)pend >> (8 * k));
+            npend -= k;
             head += n;
          }
       }

@@ -89,20 +89,30 @@ def _ptr(a):
 
 
 def generate(fiber_bundles, dim, origin, voxel_size, only_label=False,
-             n_threads=1):
+             n_threads=1, out=None):
     """Voxelize ``FiberBundles`` -> (label int32, vector f32 or (0,), props f64, warn).
 
-    ``props[0, 1]`` is left 0 (the Simpli layer sets the background absorption).
+    ``fiber_bundles`` may also be the tuple ``FiberBundles.to_flat()`` returns;
+    ``out`` = (label, vector) C-contiguous arrays to write into (the oracle
+    clears them itself).  ``props[0, 1]`` is left 0 (the Simpli layer sets the
+    background absorption).
     """
-    points, fiber_ptr, bundle_ptr, layer_ptr, table = fiber_bundles.to_flat()
+    flat = fiber_bundles if isinstance(fiber_bundles, tuple) else fiber_bundles.to_flat()
+    points, fiber_ptr, bundle_ptr, layer_ptr, table = flat
     scale, dn, mu, model = table
     n_bundles = len(bundle_ptr) - 1
     max_layer = int(np.max(np.diff(layer_ptr))) if n_bundles else 0
     n_props = n_bundles * max_layer + 1
     nx, ny, nz = (int(d) for d in dim)
-    label = np.zeros((nx, ny, nz), dtype=np.int32)
-    vector = (np.zeros((0,), dtype=np.float32) if only_label else np.zeros(
-        (nx, ny, nz, 3), dtype=np.float32))
+    if out is not None:
+        label, vector = out
+        assert label.shape == (nx, ny, nz) and label.dtype == np.int32
+        assert label.flags.c_contiguous and (only_label or vector.flags.c_contiguous)
+        assert only_label or (vector.shape == (nx, ny, nz, 3) and vector.dtype == np.float32)
+    else:
+        label = np.zeros((nx, ny, nz), dtype=np.int32)
+        vector = (np.zeros((0,), dtype=np.float32) if only_label else np.zeros(
+            (nx, ny, nz, 3), dtype=np.float32))
     props = np.zeros((n_props, 2), dtype=np.float64)
     warn = np.zeros(1, dtype=np.uint32)
     origin = np.ascontiguousarray(origin, dtype=np.float64)
@@ -123,14 +133,16 @@ def simulate(label, vector, props, theta, phi, *, step_size=1.0,
              light_intensity=26000.0, voxel_size=1.0, wavelength=525.0,
              tissue_refrection=1.0, interpolate='Slerp',
              untilt_sensor_view=True, flip_z=False, filter_rotations=None,
-             n_threads=1):
-    """March one tilt -> (images f64 (nx, ny, n_rho), warn)."""
+             n_threads=1, out=None):
+    """March one tilt -> (images f64 (nx, ny, n_rho), warn); ``out``: the images array."""
     label = np.ascontiguousarray(label, dtype=np.int32)
     vector = np.ascontiguousarray(vector, dtype=np.float32)
     props = np.ascontiguousarray(props, dtype=np.float64)
     rho = np.ascontiguousarray(filter_rotations, dtype=np.float64)
     nx, ny, nz = label.shape
-    out = np.empty((nx, ny, rho.size), dtype=np.float64)
+    if out is None:
+        out = np.empty((nx, ny, rho.size), dtype=np.float64)
+    assert out.shape == (nx, ny, rho.size) and out.dtype == np.float64 and out.flags.c_contiguous
     warn = np.zeros(1, dtype=np.uint32)
     rc = lib().oracle_simulate(nx, ny, nz, _ptr(label), _ptr(vector),
                                _ptr(props), props.shape[0], float(step_size),
