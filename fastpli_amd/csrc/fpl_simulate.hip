@@ -779,6 +779,191 @@ __device__ __forceinline__ StepEval eval_step(const Params &P, const TiltDev &T,
    return E;
 }
 
+// W(a, z) = sin(a theta) / sin(theta) (z = sin^2(theta/2)) and dW/dz, from
+// the series of slerp_weights and its term-wise derivative
+__device__ void slerp_weight_dz(float a, float z, float &W, float &dW) {
+   const float a2 = a * a;
+   float A = 1.0f, zk1 = 1.0f, P = 1.0f, dP = 0.0f; // A_k, z^(k-1)
+   for (int k = 1; k <= 40; k++) {
+      A *= ((float)(k * k) - a2) / ((k + 0.5f) * (float)k);
+      const float termd = (float)k * A * zk1;
+      zk1 *= z;
+      const float term = A * zk1;
+      P += term;
+      dP += termd;
+      if (fabsf(term) < 1e-9f && fabsf(termd) < 1e-9f) break;
+   }
+   W = a * P;
+   dW = a * dP;
+}
+
+// Reverse of one orientation interpolation s = interp(v, u, t) (lerp_orient /
+// slerp_orient, with the same branches): sb = dL/ds -> vb, ub.
+__device__ void interp_back(const float3 &v, const float3 &u, float t, int slerp,
+                            const float3 &sb, float3 &vb, float3 &ub) {
+   vb = make_float3(0.0f, 0.0f, 0.0f);
+   ub = vb;
+   if (is_zero(v)) {
+      ub = sb;
+      return;
+   }
+   if (is_zero(u)) {
+      vb = sb;
+      return;
+   }
+   if (!slerp) {
+      const float3 a = make_float3(v.x * (1.0f - t), v.y * (1.0f - t), v.z * (1.0f - t));
+      const float3 b = make_float3(u.x * t, u.y * t, u.z * t);
+      const float sg = dot3(a, b) >= 0.0f ? t : -t;
+      vb = make_float3(sb.x * (1.0f - t), sb.y * (1.0f - t), sb.z * (1.0f - t));
+      ub = make_float3(sb.x * sg, sb.y * sg, sb.z * sg);
+      return;
+   }
+   float d = dot3(v, u), sg = 1.0f;
+   float3 w = u;
+   if (d < 0.0f) {
+      w = make_float3(-u.x, -u.y, -u.z);
+      d = -d;
+      sg = -1.0f;
+   }
+   const float3 df = make_float3(v.x - w.x, v.y - w.y, v.z - w.z);
+   if (dot3(df, df) < 1e-12f || t == 0.0f) {
+      vb = sb;
+      return;
+   }
+   const float nv2 = dot3(v, v), nw2 = dot3(w, w);
+   const float inv = rsqrtf(nv2 * nw2);
+   const float craw = d * inv, c = fminf(craw, 1.0f);
+   const float z = 0.5f * (1.0f - c);
+   float wa, dwa, wb, dwb;
+   slerp_weight_dz(1.0f - t, z, wa, dwa);
+   slerp_weight_dz(t, z, wb, dwb);
+   // s = wa v + wb w, wa and wb functions of z = (1 - c) / 2, c = v.w / (|v||w|)
+   vb = make_float3(wa * sb.x, wa * sb.y, wa * sb.z);
+   float3 wbar = make_float3(wb * sb.x, wb * sb.y, wb * sb.z);
+   if (craw < 1.0f) {
+      const float gc = -0.5f * (dot3(sb, v) * dwa + dot3(sb, w) * dwb);
+      const float cv = c / nv2, cw = c / nw2;
+      vb.x += gc * (w.x * inv - v.x * cv);
+      vb.y += gc * (w.y * inv - v.y * cv);
+      vb.z += gc * (w.z * inv - v.z * cv);
+      wbar.x += gc * (v.x * inv - w.x * cw);
+      wbar.y += gc * (v.y * inv - w.y * cw);
+      wbar.z += gc * (v.z * inv - w.z * cw);
+   }
+   ub = make_float3(sg * wbar.x, sg * wbar.y, sg * wbar.z);
+}
+
+// Reverse of get_vec_interp (OWN = true): vb = dL/d(the interpolated axis) is
+// distributed over the corners, each adding to its voxel's axis gradient
+// (API layout [x][y][z][3], z mirrored under flip_z).  A corner of another
+// label entered the interpolation as 0 and gets no gradient.
+template <typename Idx>
+__device__ void get_vec_interp_back(const Field<Idx, ST_F32> &F, double px, double py,
+                                    double pz, int32_t lab, int slerp, const float3 &vb,
+                                    float *__restrict__ g_vec) {
+   const double fx = px - 0.5, fy = py - 0.5, fz = pz - 0.5;
+   const double flx = floor(fx), fly = floor(fy), flz = floor(fz);
+   const int32_t x0 = clampi((int32_t)flx, F.nx), x1 = clampi((int32_t)flx + (fx != flx), F.nx);
+   const int32_t y0 = clampi((int32_t)fly, F.ny), y1 = clampi((int32_t)fly + (fy != fly), F.ny);
+   const int32_t z0 = clampi((int32_t)flz, F.nz), z1 = clampi((int32_t)flz + (fz != flz), F.nz);
+   auto add = [&](int32_t x, int32_t y, int32_t z, int32_t l, const float3 &g) {
+      if (l != lab) return;
+      const int32_t zz = F.flip_z ? F.nz - 1 - z : z;
+      float *o = g_vec + 3 * (((int64_t)x * F.ny + y) * F.nz + zz);
+      unsafeAtomicAdd(o + 0, g.x);
+      unsafeAtomicAdd(o + 1, g.y);
+      unsafeAtomicAdd(o + 2, g.z);
+   };
+   const Vox r000 = F.at(x0, y0, z0);
+   if (x0 == x1 && y0 == y1 && z0 == z1) { // the voxel's own axis
+      add(x0, y0, z0, r000.lab, vb);
+      return;
+   }
+   const bool dx = x0 != x1, dy = y0 != y1, dz = z0 != z1;
+   const float xd = dx ? (float)((px - (double)x0) - 0.5) : 0.0f;
+   const float yd = dy ? (float)((py - (double)y0) - 0.5) : 0.0f;
+   const float zd = dz ? (float)((pz - (double)z0) - 0.5) : 0.0f;
+   const Vox r100 = F.at(x1, y0, z0), r010 = F.at(x0, y1, z0), r110 = F.at(x1, y1, z0);
+   const Vox r001 = F.at(x0, y0, z1), r101 = F.at(x1, y0, z1);
+   const Vox r011 = F.at(x0, y1, z1), r111 = F.at(x1, y1, z1);
+   auto corner = [&](const Vox &r) {
+      return r.lab != lab ? make_float3(0.0f, 0.0f, 0.0f) : r.v;
+   };
+   auto ip = [&](const float3 &a, const float3 &b, float t) {
+      return slerp ? slerp_orient(a, b, t) : lerp_orient(a, b, t);
+   };
+   const float3 m000 = corner(r000), m100 = corner(r100), m010 = corner(r010),
+                m110 = corner(r110), m001 = corner(r001), m101 = corner(r101),
+                m011 = corner(r011), m111 = corner(r111);
+   // forward, keeping the intermediate values (get_vec_interp's tree)
+   float3 c00 = m000;
+   if (dx) c00 = ip(m000, m100, xd);
+   float3 c01 = c00, c10 = c00, c11 = c00;
+   if (dz) {
+      c01 = m001;
+      if (dx) c01 = ip(m001, m101, xd);
+   }
+   if (dy) {
+      c10 = m010;
+      if (dx) c10 = ip(m010, m110, xd);
+      if (dz) {
+         c11 = m011;
+         if (dx) c11 = ip(m011, m111, xd);
+      }
+   }
+   float3 c0 = c00, c1 = c01;
+   if (dy) {
+      c0 = ip(c00, c10, yd);
+      if (dz) c1 = ip(c01, c11, yd);
+   }
+   // backward
+   const float3 zero = make_float3(0.0f, 0.0f, 0.0f);
+   float3 c0b = vb, c1b = zero;
+   if (dz) interp_back(c0, c1, zd, slerp, vb, c0b, c1b);
+   float3 c00b = c0b, c01b = c1b, c10b = zero, c11b = zero;
+   if (dy) {
+      interp_back(c00, c10, yd, slerp, c0b, c00b, c10b);
+      c01b = zero;
+      if (dz) interp_back(c01, c11, yd, slerp, c1b, c01b, c11b);
+   }
+   float3 g0, g1;
+   if (dx) {
+      interp_back(m000, m100, xd, slerp, c00b, g0, g1);
+      add(x0, y0, z0, r000.lab, g0);
+      add(x1, y0, z0, r100.lab, g1);
+   } else {
+      add(x0, y0, z0, r000.lab, c00b);
+   }
+   if (dz) {
+      if (dx) {
+         interp_back(m001, m101, xd, slerp, c01b, g0, g1);
+         add(x0, y0, z1, r001.lab, g0);
+         add(x1, y0, z1, r101.lab, g1);
+      } else {
+         add(x0, y0, z1, r001.lab, c01b);
+      }
+   }
+   if (dy) {
+      if (dx) {
+         interp_back(m010, m110, xd, slerp, c10b, g0, g1);
+         add(x0, y1, z0, r010.lab, g0);
+         add(x1, y1, z0, r110.lab, g1);
+      } else {
+         add(x0, y1, z0, r010.lab, c10b);
+      }
+      if (dz) {
+         if (dx) {
+            interp_back(m011, m111, xd, slerp, c11b, g0, g1);
+            add(x0, y1, z1, r011.lab, g0);
+            add(x1, y1, z1, r111.lab, g1);
+         } else {
+            add(x0, y1, z1, r011.lab, c11b);
+         }
+      }
+   }
+}
+
 // PROPS_LDS: the property table fits the LDS copy (every lookup a plain LDS read)
 template <typename Idx, bool PROPS_LDS>
 __global__ void __launch_bounds__(WG)
@@ -972,10 +1157,15 @@ __global__ void __launch_bounds__(WG)
                      w1 = b;
                      w2 = cc;
                   }
-                  float *o = g_vec + 3 * (((int64_t)E.vx * P.ny + E.vy) * P.nz + E.vz);
-                  unsafeAtomicAdd(o + 0, w0);
-                  unsafeAtomicAdd(o + 1, w1);
-                  unsafeAtomicAdd(o + 2, w2);
+                  if (P.interp == 0 || T.centers) { // the step's axis is the voxel's own
+                     float *o = g_vec + 3 * (((int64_t)E.vx * P.ny + E.vy) * P.nz + E.vz);
+                     unsafeAtomicAdd(o + 0, w0);
+                     unsafeAtomicAdd(o + 1, w1);
+                     unsafeAtomicAdd(o + 2, w2);
+                  } else { // through the interpolation to its corners
+                     get_vec_interp_back(F, rx, ry, rz, E.lab, P.interp == 2,
+                                         make_float3(w0, w1, w2), g_vec);
+                  }
                }
             }
             // ubar <- (I + sin d K + (1 - cos d) K^2) ubar
@@ -1529,10 +1719,6 @@ Status adjoint(const fpl_setup *setup, const int64_t dim[3], const int32_t *labe
       return make_status(FPL_E_INVALID, "null array");
    if (B.storage != ST_F32)
       return make_status(FPL_E_INVALID, "the adjoint reads the f32 field storage");
-   if (grad_vector && setup->interpolate != 0)
-      return make_status(FPL_E_INVALID,
-                         "optical_axis gradients need interpolate='NN' (the axis of a step is "
-                         "then the voxel's own)");
    const TiltDev *tdev = (const TiltDev *)(B.bytes.data() + B.off_t);
    int64_t max_steps = 0;
    for (int64_t t = 0; t < n_tilts; t++) max_steps = std::max(max_steps, tilt_steps(tdev[t], dim[2]));

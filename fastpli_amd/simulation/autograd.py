@@ -4,11 +4,15 @@
 returns the image stack of ``Simpli.run_simulation_tilts`` (n_tilts, nx, ny,
 n_rho), float64 on the GPU, as a node of the autograd graph.  Its backward is
 the hand-written adjoint kernel (``fpl_simulate_adjoint``): gradients with
-respect to ``tissue_properties`` (the (dn, mu) table, any float tensor) and,
-for ``interpolate='NN'``, ``optical_axis`` (float32 on the GPU).  The reference
-has no derivatives; the function differentiated is its forward
-(src/simulation/simulator.cpp:182-411).  Labels whose dn is 0 take the
-reference's attenuation-only branch (:280-288) and have a zero dn-gradient.
+respect to ``tissue_properties`` (the (dn, mu) table, any float tensor) and
+``optical_axis`` (float32 on the GPU).  The reference has no derivatives; the
+function differentiated is its forward (src/simulation/simulator.cpp:182-411).
+Labels whose dn is 0 take the reference's attenuation-only branch (:280-288)
+and have a zero dn-gradient.  With NN a step's axis is its voxel's own; with
+Lerp / Slerp (GetVec, :449-555) the gradient of a step's interpolated axis is
+taken back through the interpolation tree to the corners of the ray's label
+(the branches of the interpolation, such as the hemisphere flip, are held
+fixed: the derivative is that of the branch the forward took).
 """
 
 import numpy as np
@@ -74,7 +78,7 @@ def simulate_stack(simpli, tissue, optical_axis, tissue_properties, tilts):
     """Differentiable image stack (n_tilts, nx, ny, n_rho) of ``simpli``'s setup.
 
     tissue: int32 tensor (nx, ny, nz) on the GPU; optical_axis: float32 tensor
-    (nx, ny, nz, 3) on the GPU (may require grad, NN interpolation only);
+    (nx, ny, nz, 3) on the GPU (may require grad);
     tissue_properties: float tensor (n, 2) = (dn, mu) (may require grad);
     tilts: (n_tilts, 2) = (theta, phi) in radian.
     """

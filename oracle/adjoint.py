@@ -1,15 +1,19 @@
 """FP64 reference gradient of the ray march -- TEST INFRASTRUCTURE.
 
 A torch float64 (CPU) restatement of ``PliSimulator::RunSimulation``
-(src/simulation/simulator.cpp:182-411) for nearest-neighbour views with the
-untilted-sensor start, written as the reference writes it and vectorized over
-the rays of a tilt:
+(src/simulation/simulator.cpp:182-411) for views with the untilted-sensor
+start, written as the reference writes it and vectorized over the rays of a
+tilt:
 
   * starts (i + 1/2, j + 1/2) - shift / 2 with shift = nz dir / cos(theta), then
     + step / 2, and `pos += step` while floor(z) lies in [0, nz)
     (:227-231, :259-261, :721-773);
   * GetLabel / GetVec NN: floor of the position, label 0 outside (:420-443,
-    :557-579);
+    :557-579); GetVec Lerp / Slerp (:449-555): the eight corners around
+    position - 1/2 (clamped), those of another label zeroed, seven
+    interpolations x, y, z with LinearOrientationInterpolation /
+    SphericalInterpolation (zero inputs, hemisphere flip, the 1e-12 identity
+    test, acos and sin);
   * the attenuation-only branch for dn == 0 or label 0 (:280-288), the skipped
     zero axis (:292-297), and otherwise the per-rho Stokes update with asin,
     atan2, sin and cos (:309-343) and exp(-mu 1e3 t) attenuation (:276-278);
@@ -66,8 +70,76 @@ def _rot_zyz(phi, theta):
     return rz(phi) @ (ry(theta) @ rz(-phi))
 
 
+def _dot(a, b):
+    return (a * b).sum(dim=1)
+
+
+def _lerp(v, u, t):
+    """LinearOrientationInterpolation (simulator.cpp:449-471) over rays."""
+    vz, uz = (v == 0).all(dim=1), (u == 0).all(dim=1)
+    a, b = v * (1 - t)[:, None], u * t[:, None]
+    s = torch.where((_dot(a, b) >= 0)[:, None], a + b, a - b)
+    s = torch.where(uz[:, None], v, s)
+    return torch.where(vz[:, None], u, s)
+
+
+def _slerp(v, u, t):
+    """SphericalInterpolation (simulator.cpp:473-494) over rays.  The acos
+    branch gets fixed orthogonal inputs where it is not taken, so that its
+    unused gradients stay finite."""
+    vz, uz = (v == 0).all(dim=1), (u == 0).all(dim=1)
+    u = torch.where((_dot(v, u) < 0)[:, None], -u, u)
+    same = ((v - u) ** 2).sum(dim=1) < 1e-12
+    take = (~vz & ~uz & ~same)[:, None]
+    vs = torch.where(take, v, torch.tensor([1.0, 0.0, 0.0], dtype=v.dtype))
+    us = torch.where(take, u, torch.tensor([0.0, 1.0, 0.0], dtype=v.dtype))
+    r = _dot(vs, us) / (torch.sqrt(_dot(vs, vs)) * torch.sqrt(_dot(us, us)))
+    th = torch.acos(torch.clamp(r, -1.0, 1.0))
+    st = torch.sin(th)
+    s = vs * (torch.sin((1 - t) * th) / st)[:, None] + us * (torch.sin(t * th) / st)[:, None]
+    s = torch.where(take, s, v)
+    s = torch.where(uz[:, None], v, s)
+    return torch.where(vz[:, None], u, s)
+
+
+def _get_vec(lab, axis, l, px, py, pz, interpolate):
+    """GetVec (simulator.cpp:496-579) for every ray at (px, py, pz); l = the
+    rays' own labels."""
+    nx, ny, nz = lab.shape
+    ix, iy = np.floor(px).astype(np.int64), np.floor(py).astype(np.int64)
+    iz = int(math.floor(pz))
+    cx, cy = np.clip(ix, 0, nx - 1), np.clip(iy, 0, ny - 1)
+    if interpolate == 'NN':
+        return axis[cx, cy, iz]
+
+    def corners(p, n):
+        lo = np.clip(np.floor(p - 0.5), 0, n - 1).astype(np.int64)
+        hi = np.clip(np.ceil(p - 0.5), 0, n - 1).astype(np.int64)
+        d = np.where(lo == hi, 0.0, (p - lo - 0.5) / np.where(lo == hi, 1, hi - lo))
+        return lo, hi, d
+
+    x0, x1, xd = corners(px, nx)
+    y0, y1, yd = corners(py, ny)
+    z0, z1, zd = corners(np.full(px.shape, pz), nz)
+    xd, yd, zd = (torch.as_tensor(a) for a in (xd, yd, zd))
+
+    def c(x, y, z):
+        return axis[x, y, z] * (lab[x, y, z] == l)[:, None].to(axis.dtype)
+
+    ip = _slerp if interpolate == 'Slerp' else _lerp
+    c00 = ip(c(x0, y0, z0), c(x1, y0, z0), xd)
+    c01 = ip(c(x0, y0, z1), c(x1, y0, z1), xd)
+    c10 = ip(c(x0, y1, z0), c(x1, y1, z0), xd)
+    c11 = ip(c(x0, y1, z1), c(x1, y1, z1), xd)
+    v = ip(ip(c00, c10, yd), ip(c01, c11, yd), zd)
+    same = torch.as_tensor((x0 == x1) & (y0 == y1) & (z0 == z1))
+    v = torch.where(same[:, None], axis[x0, y0, z0], v)
+    return torch.where((l == 0)[:, None], torch.zeros_like(v), v)
+
+
 def simulate(tissue, optical_axis, props, tilts, *, voxel_size, wavelength=525.0,
-             light_intensity=26000.0, filter_rotations=None, step_size=1.0):
+             light_intensity=26000.0, filter_rotations=None, step_size=1.0,
+             interpolate='NN'):
     """Images (n_tilts, nx, ny, n_rho) float64 as a differentiable torch tensor.
 
     tissue: integer array (nx, ny, nz); optical_axis: torch float64
@@ -110,7 +182,7 @@ def simulate(tissue, optical_axis, props, tilts, *, voxel_size, wavelength=525.0
             mu = props[l, 1] * 1e3
             att = torch.exp(-mu * thick)
             bg = (dn.detach() == 0) | (l == 0)
-            v = optical_axis[cx, cy, iz]
+            v = _get_vec(lab, optical_axis, l, px, py, pz[0], interpolate)
             zero = (v.detach() == 0).all(dim=1)
             act = ~bg & ~zero
             # the retarder branch on a safe axis where it is not taken, so that

@@ -1,8 +1,9 @@
 """GPU adjoint of the ray march (fpl_simulate_adjoint, simulate_stack).
 
 Checked against the FP64 reference gradient (torch autograd of the FP64
-restatement in oracle/adjoint.py), against central differences of the GPU
-forward itself (h = 1e-3 relative), and through torch.autograd.
+restatement in oracle/adjoint.py) for NN, Lerp and Slerp views, against
+central differences of the GPU forward itself (h = 1e-3 relative), and
+through torch.autograd.
 """
 
 import sys
@@ -27,7 +28,7 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def _small_tissue(seed=0, dim=(24, 20, 16), voxel=0.5):
+def _small_tissue(seed=0, dim=(24, 20, 16), voxel=0.5, interp='NN'):
     """A vMF bundle (kappa = 10) in a small volume, generated on the GPU."""
     s = Simpli()
     s.voxel_size = voxel
@@ -37,7 +38,7 @@ def _small_tissue(seed=0, dim=(24, 20, 16), voxel=0.5):
     s.filter_rotations = ROT9
     s.light_intensity = 26000
     s.wavelength = 525
-    s.interpolate = 'NN'
+    s.interpolate = interp
     with warnings.catch_warnings():
         warnings.simplefilter('ignore')
         label, vector, props = s.generate_tissue()
@@ -55,16 +56,19 @@ def _adjoint(s, label, vector, props, w, with_axis=True):
     return gp, (gv.cpu().numpy() if with_axis else None)
 
 
-def test_adjoint_matches_fp64_reference_gradient():
-    _say('adjoint: vs FP64 reference gradient, vMF bundle 24 x 20 x 16, NN, 5 tilts x 9 rot')
-    s, label, vector, props = _small_tissue()
+@pytest.mark.parametrize('interp', ['NN', 'Lerp', 'Slerp'])
+def test_adjoint_matches_fp64_reference_gradient(interp):
+    _say(f'adjoint: vs FP64 reference gradient, vMF bundle 24 x 20 x 16, {interp}, '
+         '5 tilts x 9 rot')
+    s, label, vector, props = _small_tissue(interp=interp)
     props = props.copy()
     props[0, 1] = 0.3  # background absorption, so that label 0 has a mu-gradient too
     rng = np.random.default_rng(1)
     w = rng.normal(size=(5,) + label.shape[:2] + (9,))
     gp, gv = _adjoint(s, label, vector, props, w)
     rp, rv, _ = ref.gradient(label, vector, props, TILTS5, w, voxel_size=0.5,
-                             wavelength=525.0, light_intensity=26000.0, filter_rotations=ROT9)
+                             wavelength=525.0, light_intensity=26000.0, filter_rotations=ROT9,
+                             interpolate=interp)
     scale_p = np.abs(rp).max()
     for l in range(props.shape[0]):
         for c in range(2):
@@ -80,9 +84,10 @@ def test_adjoint_matches_fp64_reference_gradient():
     assert np.abs(gv - rv).max() < 1e-2 * np.abs(rv).max()
 
 
-def test_adjoint_matches_central_differences():
-    _say('adjoint: vs central differences (h = 1e-3) of the GPU forward')
-    s, label, vector, props = _small_tissue(seed=2)
+@pytest.mark.parametrize('interp', ['NN', 'Slerp'])
+def test_adjoint_matches_central_differences(interp):
+    _say(f'adjoint: vs central differences (h = 1e-3) of the GPU forward, {interp}')
+    s, label, vector, props = _small_tissue(seed=2, interp=interp)
     props = props.copy()
     props[0, 1] = 0.3
     # positive weights that vary over pixels, tilts and rotations (a sum over
@@ -144,12 +149,70 @@ def test_simulate_stack_autograd():
     # the same kernel; only the order of its atomic sums differs between calls
     assert np.allclose(pr.grad.numpy(), gp, rtol=1e-9, atol=0)
     assert np.allclose(axis.grad.cpu().numpy(), gv, rtol=1e-5, atol=1e-6 * np.abs(gv).max())
-    # props-only gradients also work with Slerp views
+    # Slerp views: axis gradients through the interpolation, as the C entry gives them
     s.interpolate = 'Slerp'
     pr2 = torch.tensor(props, dtype=torch.float64, requires_grad=True)
-    img2 = simulate_stack(s, lab, torch.from_numpy(vector).to(dev), pr2, TILTS5)
-    img2.sum().backward()
-    assert torch.isfinite(pr2.grad).all()
-    with pytest.raises(ValueError, match='NN'):
-        axis2 = torch.from_numpy(vector).to(dev).requires_grad_(True)
-        simulate_stack(s, lab, axis2, pr2.detach(), TILTS5).sum().backward()
+    axis2 = torch.from_numpy(vector).to(dev).requires_grad_(True)
+    img2 = simulate_stack(s, lab, axis2, pr2, TILTS5)
+    (img2 * w).sum().backward()
+    gp2, gv2 = _adjoint(s, label, vector, props, w.cpu().numpy())
+    assert torch.isfinite(pr2.grad).all() and torch.isfinite(axis2.grad).all()
+    assert np.allclose(pr2.grad.numpy(), gp2, rtol=1e-9, atol=0)
+    assert np.allclose(axis2.grad.cpu().numpy(), gv2, rtol=1e-5, atol=1e-6 * np.abs(gv2).max())
+
+
+@pytest.mark.parametrize('interp', ['NN', 'Slerp'])
+def test_adjoint_central_differences_at_c4_size(interp):
+    # C4: one vMF bundle of 400 fibers, 300 x 300 x 150 voxels @ 0.2 um,
+    # 5 tilts x 9 rotations: dL/d dn and dL/d mu of the myelin label and the
+    # derivative along a rotation of the whole axis field, against central
+    # differences of the GPU forward
+    _say(f'adjoint: C4 300 x 300 x 150 central differences, {interp}')
+    dim = (300, 300, 150)
+    s = Simpli()
+    s.voxel_size = 0.2
+    s.dim = np.array(dim)
+    s.fiber_bundles = synthetic.single_bundle(seed=0, size=tuple(np.array(dim) * 0.2))
+    s.filter_rotations = ROT9
+    s.light_intensity = 26000
+    s.wavelength = 525
+    s.interpolate = interp
+    with warnings.catch_warnings():
+        warnings.simplefilter('ignore')
+        label, vector, props = s.generate_tissue()
+    ii, jj = np.meshgrid(np.arange(dim[0]), np.arange(dim[1]), indexing='ij')
+    w = ((1.0 + 0.5 * np.cos(0.05 * ii + 0.03 * jj))[None, :, :, None] *
+         (1.0 + 0.5 * np.cos(2 * ROT9 + 0.7))[None, None, None, :] *
+         (1.0 + 0.2 * np.arange(5))[:, None, None, None])
+    gp, gv = _adjoint(s, label, vector, props, w)
+    dev = torch.device('cuda', 0)
+    lab_d = torch.from_numpy(label).to(dev)
+    w_d = torch.from_numpy(w).to(dev)
+
+    def loss(p, vec):
+        with warnings.catch_warnings():
+            warnings.simplefilter('ignore')
+            img = s.run_simulation_tilts(lab_d, torch.from_numpy(vec).to(dev), p, TILTS5)
+        return float((img * w_d).sum())
+
+    p_lab = 2  # the bundle's myelin ('p') layer: label 1 + layer 1
+    assert props[p_lab, 0] != 0
+    for (l, c) in [(p_lab, 0), (p_lab, 1), (1, 1)]:
+        h = 1e-3 * abs(props[l, c])
+        pp, pm = props.copy(), props.copy()
+        pp[l, c] += h
+        pm[l, c] -= h
+        fd = (loss(pp, vector) - loss(pm, vector)) / (2 * h)
+        assert np.isclose(gp[l, c], fd, rtol=5e-3), (l, c, gp[l, c], fd)
+    om = np.array([0.3, -0.5, 0.8])
+    om /= np.linalg.norm(om)
+    k = np.array([[0, -om[2], om[1]], [om[2], 0, -om[0]], [-om[1], om[0], 0]])
+
+    def rotated(e):
+        R = np.eye(3) + np.sin(e) * k + (1 - np.cos(e)) * (k @ k)
+        return np.ascontiguousarray((vector.astype(np.float64) @ R.T).astype(np.float32))
+
+    eps = 1e-3
+    fd = (loss(props, rotated(eps)) - loss(props, rotated(-eps))) / (2 * eps)
+    an = float((gv.astype(np.float64) * np.cross(om, vector.astype(np.float64))).sum())
+    assert np.isclose(an, fd, rtol=5e-3), (an, fd)

@@ -1,9 +1,11 @@
 """GPU parity of run_simulation (FP32 path) vs the FP64 oracle.
 
-Tolerance (FP32 mode): |I_gpu - I_oracle| <= 1e-4 * |I_oracle| for every
-pixel and rotation, except where the intensity itself is within 1e-4 of zero
-relative to the pixel's transmittance (I < 1e-4 * I0/4 * A): there the bound
-is 1e-8 * I0 absolute.
+Tolerance (FP32 mode), for every pixel and rotation:
+|I_gpu - I_oracle| <= max(1e-4 * |I_oracle|, 1e-8 * I0).  The relative bound
+governs wherever I_oracle >= 1e-4 * I0 (2.6 for I0 = 26000); below that the
+absolute floor 1e-8 * I0 does (intensities that small come from near-total
+extinction, where a relative bound would ask for more digits than FP32
+attenuation sums carry).
 """
 
 import sys

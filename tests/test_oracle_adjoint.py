@@ -26,26 +26,29 @@ def _fields(seed, dim=(7, 6, 9)):
     return label, v.astype(np.float32), props
 
 
+@pytest.mark.parametrize('interp', ['NN', 'Lerp', 'Slerp'])
 @pytest.mark.parametrize('seed', [0, 1])
-def test_forward_matches_cpp_oracle(seed):
+def test_forward_matches_cpp_oracle(seed, interp):
     label, vec, props = _fields(seed)
     got = adjoint.simulate(label, torch.tensor(vec, dtype=torch.float64),
-                           torch.tensor(props), TILTS, **SETUP).numpy()
+                           torch.tensor(props), TILTS, interpolate=interp, **SETUP).numpy()
     for t, (theta, phi) in enumerate(TILTS):
         ref, _ = oracle.simulate(label, vec, props, theta, phi, voxel_size=0.5,
-                                 interpolate='NN', filter_rotations=ROT)
+                                 interpolate=interp, filter_rotations=ROT)
         assert np.allclose(got[t], ref, rtol=1e-12, atol=1e-9), (t, np.abs(got[t] - ref).max())
 
 
-def test_gradient_matches_central_differences():
+@pytest.mark.parametrize('interp', ['NN', 'Lerp', 'Slerp'])
+def test_gradient_matches_central_differences(interp):
     label, vec, props = _fields(3)
     rng = np.random.default_rng(4)
     w = rng.normal(size=(len(TILTS),) + label.shape[:2] + (ROT.size,))
-    gp, gv, _ = adjoint.gradient(label, vec, props, TILTS, w, **SETUP)
+    gp, gv, _ = adjoint.gradient(label, vec, props, TILTS, w, interpolate=interp, **SETUP)
 
     def loss(p, v):
         with torch.no_grad():
-            img = adjoint.simulate(label, torch.tensor(v), torch.tensor(p), TILTS, **SETUP)
+            img = adjoint.simulate(label, torch.tensor(v), torch.tensor(p), TILTS,
+                                   interpolate=interp, **SETUP)
         return float((img.numpy() * w).sum())
 
     vec64 = vec.astype(np.float64)
