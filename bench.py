@@ -82,40 +82,89 @@ def measured_traffic(kernel, storage):
 
 
 def adjoint_bench(reps=5):
-    """C4 (one vMF bundle, 300 x 300 x 150 @ 0.2 um), NN, 5 tilts x 9 rotations:
-    the forward march and the adjoint kernel (gradients w.r.t. the (dn, mu) table
-    and the optical-axis field of L = sum(g * images)), HIP-event times."""
+    """C4 (one vMF bundle, 300 x 300 x 150 @ 0.2 um), 5 tilts x 9 rotations,
+    NN and Slerp: the forward march and the adjoint kernel (gradients w.r.t.
+    the (dn, mu) table and the optical-axis field of L = sum(g * images)),
+    HIP-event times.  The adjoint's algorithmic bytes per ray-step: the 16-byte
+    record read three times (forward, replay, backward) and a 40-byte
+    checkpoint written and read once per 8 steps; per pixel the float64
+    gradient images read."""
     import torch
     from fastpli_amd import _lib, synthetic
     from fastpli_amd.simulation import simulate_stack
     dim = (300, 300, 150)
     s = make_simpli(dim, synthetic.single_bundle(seed=0))
-    s.interpolate = 'NN'
     dev = torch.device('cuda', 0)
     with warnings.catch_warnings():
         warnings.simplefilter('ignore')
         label, vector, props = s.generate_tissue()
     lab = torch.from_numpy(label).to(dev)
-    axis = torch.from_numpy(vector).to(dev).requires_grad_(True)
-    pr = torch.tensor(props, requires_grad=True)
-    g = None
-    for k in range(reps + 2):
-        if k == 2:
+    out = {'config': 'one vMF(kappa 10) bundle of 400 fibers, 300x300x150 voxels @0.2um, '
+                     '5 tilts x 9 rotations',
+           'gradients': 'dn, mu per label; optical axis per voxel'}
+    for interp in ('NN', 'Slerp'):
+        s.interpolate = interp
+        axis = torch.from_numpy(vector).to(dev).requires_grad_(True)
+        pr = torch.tensor(props, requires_grad=True)
+        g = None
+        for k in range(reps + 2):
+            if k == 2:
+                _lib.profile_enable(True)
+            img = simulate_stack(s, lab, axis, pr, TILTS)
+            if g is None:
+                g = torch.randn_like(img)
+            (img * g).sum().backward()
+            axis.grad = None
+            pr.grad = None
+        torch.cuda.synchronize()
+        fwd, nf = _lib.profile_read('march')
+        adj, na = _lib.profile_read('adjoint')
+        _lib.profile_enable(False)
+        ray_steps = sum(s.count_voxel_steps(t, p) for t, p in TILTS) / ROT.size
+        nbytes = ray_steps * (3 * 16.0 + 2 * 40.0 / 8) + len(TILTS) * dim[0] * dim[1] * 8.0 * ROT.size
+        adj_ms = adj / max(na, 1)
+        out[interp] = {'forward_march_ms': fwd / max(nf, 1), 'adjoint_ms': adj_ms,
+                       'adjoint_bytes': nbytes, 'adjoint_GBs': nbytes / (adj_ms * 1e-3) / 1e9}
+    return out
+
+
+def precision_bench(fbs, dim, reps=10):
+    """The march in the three field storages on the C2 volume: the untilted
+    view with NN (the precision form of the plain workload, a pure streaming
+    march) and the 5-tilt Slerp step; HIP-event times, the storage's bytes
+    (records read once per tilt + float64 images written) and GB/s."""
+    import torch
+    from fastpli_amd import _lib
+    nx, ny, nz = dim
+    n_vox, n_rho = nx * ny * nz, ROT.size
+    out = {}
+    for storage in ('f32', 'f16', 'angle16'):
+        s = make_simpli(dim, fbs)
+        s.field_storage = storage
+        with warnings.catch_warnings():
+            warnings.simplefilter('ignore')
+            _, _, props, fields = s.generate_tissue_fields(with_arrays=False)
+        rec = {'f32': 16.0, 'f16': 8.0, 'angle16': 6.0}[storage]
+        row = {}
+        for name, interp, tilts in (('untilted_nn', 'NN', TILTS[:1]),
+                                    ('tilts5_slerp', 'Slerp', TILTS)):
+            s.interpolate = interp
+            for _ in range(2):
+                s.run_simulation_fields(fields, props, tilts)
+            torch.cuda.synchronize()
             _lib.profile_enable(True)
-        img = simulate_stack(s, lab, axis, pr, TILTS)
-        if g is None:
-            g = torch.randn_like(img)
-        (img * g).sum().backward()
-        axis.grad = None
-        pr.grad = None
-    torch.cuda.synchronize()
-    fwd, nf = _lib.profile_read('march')
-    adj, na = _lib.profile_read('adjoint')
-    _lib.profile_enable(False)
-    return {'config': 'one vMF(kappa 10) bundle of 400 fibers, 300x300x150 voxels @0.2um, NN, '
-                      '5 tilts x 9 rotations',
-            'forward_march_ms': fwd / max(nf, 1), 'adjoint_ms': adj / max(na, 1),
-            'gradients': 'dn, mu per label; optical axis per voxel'}
+            for _ in range(reps):
+                s.run_simulation_fields(fields, props, tilts)
+            torch.cuda.synchronize()
+            ms, n = _lib.profile_read('march')
+            _lib.profile_enable(False)
+            ms /= max(n, 1)
+            nbytes = len(tilts) * (rec * n_vox + 8.0 * n_rho * nx * ny)
+            row[name] = {'march_ms': ms, 'bytes': nbytes, 'GBs': nbytes / (ms * 1e-3) / 1e9}
+        row['bytes_per_voxel'] = rec
+        out[storage] = row
+        del fields
+    return out
 
 
 def serving_bench(n_requests, load, seed=0, pool=4):
@@ -368,6 +417,8 @@ def main():
                     help='skip the adjoint timing (C4 forward + reverse march)')
     ap.add_argument('--no-serving', action='store_true',
                     help='skip the mixed-size serving stream (p50 / p99 latency)')
+    ap.add_argument('--no-precision', action='store_true',
+                    help='skip the march timings of the three field storages')
     ap.add_argument('--serve-requests', type=int, default=1500)
     ap.add_argument('--serve-load', type=float, default=0.7,
                     help='arrival rate as a fraction of the one-at-a-time service rate')
@@ -473,6 +524,11 @@ def main():
         log('adjoint: C4 forward + reverse march')
         adjoint = adjoint_bench()
 
+    precision = None
+    if not args.no_precision:
+        log('precision storages: f32 / f16 / angle16 march')
+        precision = precision_bench(fbs, dim)
+
     serving = None
     if not args.no_serving:
         log(f'serving stream: {args.serve_requests} requests at load {args.serve_load}')
@@ -528,6 +584,7 @@ def main():
         'cpu_baseline': cpu,
         'serving': serving,
         'adjoint': adjoint,
+        'precision': precision,
     }
     if cpu:
         out['speedup_vs_cpu_baseline'] = out['value'] / cpu['value']
