@@ -419,6 +419,8 @@ def main():
                     help='skip the mixed-size serving stream (p50 / p99 latency)')
     ap.add_argument('--no-precision', action='store_true',
                     help='skip the march timings of the three field storages')
+    ap.add_argument('--no-api-path', action='store_true',
+                    help='skip the step timed through the Simpli API')
     ap.add_argument('--serve-requests', type=int, default=1500)
     ap.add_argument('--serve-load', type=float, default=0.7,
                     help='arrival rate as a fraction of the one-at-a-time service rate')
@@ -534,8 +536,10 @@ def main():
         log(f'serving stream: {args.serve_requests} requests at load {args.serve_load}')
         serving = serving_bench(args.serve_requests, args.serve_load)
 
-    log('API path: Simpli.generate_tissue_fields + run_simulation_fields')
-    api = api_path_bench(fbs, dim, args.field_storage, max(3, args.steps // 2), work)
+    api = None
+    if not args.no_api_path:
+        log('API path: Simpli.generate_tissue_fields + run_simulation_fields')
+        api = api_path_bench(fbs, dim, args.field_storage, max(3, args.steps // 2), work)
 
     cpu = None
     if not args.no_cpu_baseline:
