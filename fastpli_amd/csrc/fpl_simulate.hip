@@ -302,50 +302,43 @@ __device__ float3 get_vec_interp(const Field<Idx, ST> &F, double px, double py, 
    auto ip = [&](const float3 &a, const float3 &b, float t) {
       return slerp ? slerp_orient(a, b, t) : lerp_orient(a, b, t);
    };
-   // every corner read at once.  An axis without spread in any lane of the
-   // wave (a view tilted along x samples voxel centres in y, and vice versa)
-   // repeats its corners: they are not read again (wave-uniform branches)
+   // the corners of the plane y0 first, those of y1 only when some lane of the
+   // wave has spread along y (a view tilted along x samples voxel centres in y);
+   // an axis without spread in any lane repeats its corners: they are not read
+   // again (wave-uniform branches)
    const bool xdup = __ballot(dx) == 0, ydup = __ballot(dy) == 0;
-   Vox r000, r100, r010, r110, r001, r101, r011, r111;
-   r000 = F.at(x0, y0, z0);
-   r001 = F.at(x0, y0, z1);
-   if (xdup) {
-      r100 = r000;
-      r101 = r001;
-   } else {
-      r100 = F.at(x1, y0, z0);
-      r101 = F.at(x1, y0, z1);
+   float3 c00, c01, c10, c11;
+   {
+      const Vox r000 = F.at(x0, y0, z0), r001 = F.at(x0, y0, z1);
+      Vox r100 = r000, r101 = r001;
+      if (!xdup) {
+         r100 = F.at(x1, y0, z0);
+         r101 = F.at(x1, y0, z1);
+      }
+      c00 = corner(r000);
+      if (dx) c00 = ip(c00, corner(r100), xd);
+      c01 = c00;
+      if (dz) {
+         c01 = corner(r001);
+         if (dx) c01 = ip(c01, corner(r101), xd);
+      }
    }
-   if (ydup) {
-      r010 = r000;
-      r011 = r001;
-      r110 = r100;
-      r111 = r101;
-   } else {
-      r010 = F.at(x0, y1, z0);
-      r011 = F.at(x0, y1, z1);
-      if (xdup) {
-         r110 = r010;
-         r111 = r011;
-      } else {
+   c10 = c00;
+   c11 = c00;
+   if (!ydup) {
+      const Vox r010 = F.at(x0, y1, z0), r011 = F.at(x0, y1, z1);
+      Vox r110 = r010, r111 = r011;
+      if (!xdup) {
          r110 = F.at(x1, y1, z0);
          r111 = F.at(x1, y1, z1);
       }
-   }
-   // level x
-   float3 c00 = corner(r000);
-   if (dx) c00 = ip(c00, corner(r100), xd);
-   float3 c01 = c00, c10 = c00, c11 = c00;
-   if (dz) {
-      c01 = corner(r001);
-      if (dx) c01 = ip(c01, corner(r101), xd);
-   }
-   if (dy) {
-      c10 = corner(r010);
-      if (dx) c10 = ip(c10, corner(r110), xd);
-      if (dz) {
-         c11 = corner(r011);
-         if (dx) c11 = ip(c11, corner(r111), xd);
+      if (dy) {
+         c10 = corner(r010);
+         if (dx) c10 = ip(c10, corner(r110), xd);
+         if (dz) {
+            c11 = corner(r011);
+            if (dx) c11 = ip(c11, corner(r111), xd);
+         }
       }
    }
    // level y
