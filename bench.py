@@ -82,7 +82,7 @@ def measured_traffic(kernel, storage):
 
 
 def adjoint_bench(reps=5):
-    """C4 (one vMF bundle, 300 x 300 x 150 @ 0.2 um), 5 tilts x 9 rotations,
+    """One vMF bundle (400 fibers, 300 x 300 x 150 @ 0.2 um), 5 tilts x 9 rotations,
     NN and Slerp: the forward march and the adjoint kernel (gradients w.r.t.
     the (dn, mu) table and the optical-axis field of L = sum(g * images)),
     HIP-event times.  The adjoint's algorithmic bytes per ray-step: the 16-byte
@@ -129,10 +129,11 @@ def adjoint_bench(reps=5):
 
 
 def precision_bench(fbs, dim, reps=10):
-    """The march in the three field storages on the C2 volume: the untilted
-    view with NN (the precision form of the plain workload, a pure streaming
-    march) and the 5-tilt Slerp step; HIP-event times, the storage's bytes
-    (records read once per tilt + float64 images written) and GB/s."""
+    """The march in the three field storages on the crossing-bundle volume
+    (600 x 600 x 300): the untilted view with NN (the precision form of the
+    plain workload, a pure streaming march) and the 5-tilt Slerp step;
+    HIP-event times, the storage's bytes (records read once per tilt + float64
+    images written) and GB/s."""
     import torch
     from fastpli_amd import _lib
     nx, ny, nz = dim
@@ -414,7 +415,7 @@ def main():
     ap.add_argument('--field-storage', default='f32', choices=['f32', 'f16', 'angle16'],
                     help='storage of the fields inside the march (precision modes)')
     ap.add_argument('--no-adjoint', action='store_true',
-                    help='skip the adjoint timing (C4 forward + reverse march)')
+                    help='skip the adjoint timing (vMF bundle forward + reverse march)')
     ap.add_argument('--no-serving', action='store_true',
                     help='skip the mixed-size serving stream (p50 / p99 latency)')
     ap.add_argument('--no-precision', action='store_true',
@@ -523,7 +524,7 @@ def main():
 
     adjoint = None
     if not args.no_adjoint:
-        log('adjoint: C4 forward + reverse march')
+        log('adjoint: vMF bundle forward + reverse march')
         adjoint = adjoint_bench()
 
     precision = None

@@ -32,7 +32,7 @@ MYELIN = [(0.75, 0.0, 5.0, 'b'), (1.0, -0.004, 1.0, 'p')]
 
 def parallel_fibers(seed=0, n_side=8, spacing=2.2, radius=1.0, size=30.0, jitter=0.05,
                     layers=MYELIN, size_z=None, n_rows=None):
-    """Straight fibers along x on a jittered hexagonal grid (config 1: 8 x 8 in a
+    """Straight fibers along x on a jittered hexagonal grid (default: 8 x 8 in a
     30 um cube).  ``size_z`` / ``n_rows`` give a taller stack of rows."""
     rng = np.random.default_rng(seed)
     pitch = spacing * np.sqrt(3.0) / 2.0
@@ -79,7 +79,8 @@ def _undulating_bundle(rng, axis, n_u, n_v, extent_u, extent_v, length, seg, amp
 
 def crossing_bundles(seed=0, n_fibers=2000, size_xy=120.0, size_z=60.0, seg=0.5, amp=0.5,
                      period=20.0, sigma=0.2, jitter=0.2, layers=MYELIN):
-    """Two interpenetrating bundles along x and y (config 2)."""
+    """Two interpenetrating bundles along x and y (default: 2 x 2000 undulating
+    fibers in 120 x 120 x 60 um)."""
     rng = np.random.default_rng(seed)
     n_u = int(round(np.sqrt(n_fibers * size_xy / size_z)))
     n_v = int(np.ceil(n_fibers / n_u))

@@ -162,12 +162,12 @@ def test_simulate_stack_autograd():
 
 
 @pytest.mark.parametrize('interp', ['NN', 'Slerp'])
-def test_adjoint_central_differences_at_c4_size(interp):
-    # C4: one vMF bundle of 400 fibers, 300 x 300 x 150 voxels @ 0.2 um,
+def test_adjoint_central_differences_full_size(interp):
+    # one vMF bundle of 400 fibers, 300 x 300 x 150 voxels @ 0.2 um,
     # 5 tilts x 9 rotations: dL/d dn and dL/d mu of the myelin label and the
     # derivative along a rotation of the whole axis field, against central
     # differences of the GPU forward
-    _say(f'adjoint: C4 300 x 300 x 150 central differences, {interp}')
+    _say(f'adjoint: vMF bundle 300 x 300 x 150 central differences, {interp}')
     dim = (300, 300, 150)
     s = Simpli()
     s.voxel_size = 0.2

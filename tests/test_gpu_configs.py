@@ -1,7 +1,7 @@
-"""The benchmark configurations at full size against the FP64 oracle.
+"""The benchmark workloads at full size against the FP64 oracle.
 
-C2: crossing bundles 2 x 2000 fibers, 600 x 600 x 300 voxels @ 0.2 um, Slerp.
-C4: one bundle of 400 fibers with vMF(kappa = 10) directions, 300 x 300 x 150
+crossing: bundles 2 x 2000 fibers, 600 x 600 x 300 voxels @ 0.2 um, Slerp.
+vmf: one bundle of 400 fibers with vMF(kappa = 10) directions, 300 x 300 x 150
     voxels @ 0.2 um, Slerp.
 Both with the 5-tilt set and 9 rotations.  Labels and optical axes
 bit-exact; intensities to 1e-4 relative (floor 1e-8 I0); the size-independent
@@ -34,15 +34,15 @@ def _say(msg):
 
 
 CONFIGS = {
-    'C2': (lambda: synthetic.crossing_bundles(seed=0), (600, 600, 300)),
-    'C4': (lambda: synthetic.single_bundle(seed=0), (300, 300, 150)),
+    'crossing_600x600x300': (lambda: synthetic.crossing_bundles(seed=0), (600, 600, 300)),
+    'vmf_300x300x150': (lambda: synthetic.single_bundle(seed=0), (300, 300, 150)),
 }
 
 
-@pytest.mark.parametrize('name', ['C4', 'C2'])
+@pytest.mark.parametrize('name', ['vmf_300x300x150', 'crossing_600x600x300'])
 def test_config_full_size(name):
     make, dim = CONFIGS[name]
-    _say(f'config {name} {dim}: generation vs oracle ({THREADS} threads)')
+    _say(f'workload {name}: generation vs oracle ({THREADS} threads)')
     fbs = make()
     s = Simpli()
     s.voxel_size = 0.2
@@ -53,7 +53,7 @@ def test_config_full_size(name):
     s.wavelength = 525
     s.interpolate = 'Slerp'
     with warnings.catch_warnings():
-        warnings.simplefilter('ignore')  # C2's bundles interpenetrate: "objects overlap"
+        warnings.simplefilter('ignore')  # the crossing bundles interpenetrate: "objects overlap"
         label, vector, props = s.generate_tissue()
     ol, ov, op, _ = oracle.generate(fbs, dim, np.zeros(3), 0.2, n_threads=THREADS)
     assert np.array_equal(label, ol), f'{np.count_nonzero(label != ol)} labels differ'

@@ -1,6 +1,6 @@
-"""Fit of dn and the mean optical-axis orientation through the adjoint (C4).
+"""Fit of dn and the mean optical-axis orientation through the adjoint.
 
-C4: one bundle of 400 fibers with vMF (kappa = 10) directions, 300 x 300 x 150
+One bundle of 400 fibers with vMF (kappa = 10) directions, 300 x 300 x 150
 voxels @ 0.2 um, NN, 5 tilts x 9 rotations.  The target stack comes from
 another seed of the fiber model (same mean fiber axis, other fibers); the
 model is the seed-0 tissue whose axis field is rotated so that its mean axis
@@ -72,7 +72,8 @@ def _rotation(a0, a):
 
 
 def test_fit_dn_and_mean_orientation():
-    _say('fit: C4 300 x 300 x 150, dn and mean axis from a stack of another seed (L-BFGS)')
+    _say('fit: vMF bundle 300 x 300 x 150, dn and mean axis from a stack of another seed '
+         '(L-BFGS)')
     dev = torch.device('cuda', 0)
     c2 = torch.tensor(np.cos(2 * ROT9), device=dev)
     s2 = torch.tensor(np.sin(2 * ROT9), device=dev)
