@@ -710,6 +710,18 @@ struct Ckpt { // state before the first step of a segment
    float pad;
 };
 
+// the record of the voxel holding (px, py, pz) (background outside) and its index
+template <typename Idx>
+__device__ __forceinline__ Vox fetch_own(const Field<Idx, ST_F32> &F, double px, double py,
+                                         double pz, int32_t &ix, int32_t &iy, int32_t &iz) {
+   ix = ifloor(px);
+   iy = ifloor(py);
+   iz = ifloor(floor(pz));
+   Vox r{0, make_float3(0.0f, 0.0f, 0.0f), false};
+   if (F.inside(ix, iy, iz)) r = F.at(ix, iy, iz);
+   return r;
+}
+
 // What one z-step of the march does at a position (k_march's arithmetic).
 struct StepEval {
    int kind;   // 0: zero axis, skipped; 1: attenuation only; 2: retarder
@@ -963,7 +975,8 @@ __global__ void __launch_bounds__(WG)
     k_adjoint(const Params *__restrict__ prm, const TiltDev *__restrict__ tilts,
               const PropDev *__restrict__ props, const void *__restrict__ rec_a,
               const double *__restrict__ grad_images, Ckpt *__restrict__ ck, int64_t ck_stride,
-              double kfac, double mfac, double *__restrict__ g_props, float *__restrict__ g_vec) {
+              double kfac, double mfac, double *__restrict__ g_props, float *__restrict__ g_vec,
+              int tilt0, int n_sub) {
    __shared__ PropDev sp[MAX_ADJ_PROPS];
    __shared__ double acc[MAX_ADJ_PROPS][2];
    __shared__ float ub[CK * 3][WG]; // states before the steps of the current segment
@@ -976,8 +989,9 @@ __global__ void __launch_bounds__(WG)
    }
    __syncthreads();
 
-   const int tilt = (int)(blockIdx.x % (unsigned)P.n_tilts);
-   const int64_t tile = blockIdx.x / (unsigned)P.n_tilts;
+   // the tilts [tilt0, tilt0 + n_sub) of the launch
+   const int tilt = tilt0 + (int)(blockIdx.x % (unsigned)n_sub);
+   const int64_t tile = blockIdx.x / (unsigned)n_sub;
    const int64_t ray = tile * WG + threadIdx.x;
    const int64_t i = ray / P.ny, j = ray - i * P.ny;
    const int tid = threadIdx.x;
@@ -1172,6 +1186,453 @@ __global__ void __launch_bounds__(WG)
       }
       flush();
    }
+   __syncthreads();
+   for (int k = threadIdx.x; k < np_lds; k += WG) {
+      if (acc[k][0] != 0.0) atomicAdd(&g_props[2 * k], acc[k][0]);
+      if (acc[k][1] != 0.0) atomicAdd(&g_props[2 * k + 1], acc[k][1]);
+   }
+}
+
+// ------------------------------------------------------- adjoint, queued
+// The adjoint of the interpolating views (Lerp / Slerp), built on the march's
+// wave-cooperative queue.  One thread per ray again, but the costly parts of
+// a birefringent step run one step per lane:
+//   forward   the queued march of k_march; the evaluation of each queued step
+//             also stores its result (cos 2phi, sin 2phi, sin delta,
+//             1 - cos delta, g = |v| cos^2(alpha), the tilted axis v) in the
+//             ray's row of `res` (index = the ray's count of birefringent
+//             steps), and a checkpoint (position, Poincare state, that count)
+//             is kept before every CKQ-th step: the position when the ray
+//             passes it, the state when the ray applies its first step at or
+//             after it;
+//   reverse   segment by segment from the last (the rays of a wave take the
+//             same number of steps, so they walk the segments together): the
+//             labels of the segment's steps are read, the states before each
+//             step rebuilt from the checkpoint with the stored results (no
+//             interpolation), and the steps walked backwards as in k_adjoint;
+//             the gradient of a step's interpolated axis goes to a second
+//             per-wave queue, and 64 queued steps at a time take it back
+//             through the interpolation to the corners (get_vec_interp_back).
+constexpr int CKQ = 4; // steps per checkpoint segment of the queued adjoint
+
+struct StepRes {      // a birefringent step's evaluation, stored by the forward
+   float C, S, sd, omc;
+   float g;           // |v| cos^2(alpha); -1: zero axis (skipped, no attenuation)
+   float vx, vy, vz;  // the axis after the tilt rotation
+};
+
+// the reverse's axis-gradient queue: ng queued steps, one per lane, back
+// through the interpolation to the corners (out of line: called from the
+// unrolled walk back)
+template <typename Idx>
+__device__ __noinline__ void grad_drain(Field<Idx, ST_F32> F, const uint4 *QA, const uint4 *QB,
+                                        const uint4 *QC, uint32_t ghead, uint32_t ng, int lane,
+                                        int slerp, float *g_vec) {
+   if ((uint32_t)lane < ng) {
+      const uint32_t slot = (ghead + (uint32_t)lane) & (QCAP - 1);
+      const uint4 a = QA[slot], b = QB[slot], w = QC[slot];
+      get_vec_interp_back(F, pack_double(a.x, a.y), pack_double(a.z, a.w), pack_double(b.x, b.y),
+                          (int32_t)b.z, slerp,
+                          make_float3(__uint_as_float(w.x), __uint_as_float(w.y),
+                                      __uint_as_float(w.z)),
+                          g_vec);
+   }
+}
+
+template <typename Idx, bool PROPS_LDS>
+__global__ void __launch_bounds__(WG)
+    k_adjoint_queue(const Params *__restrict__ prm, const TiltDev *__restrict__ tilts,
+                    const PropDev *__restrict__ props, const void *__restrict__ rec_a,
+                    const double *__restrict__ grad_images, Ckpt *__restrict__ ck,
+                    int64_t ck_stride, StepRes *__restrict__ res, int64_t res_stride,
+                    double kfac, double mfac, double *__restrict__ g_props,
+                    float *__restrict__ g_vec, int tilt0, int n_sub) {
+   __shared__ PropDev sp[MAX_ADJ_PROPS];
+   __shared__ double acc[MAX_ADJ_PROPS][2];
+   // forward: the march queue (qc: count index, step, owner lane); reverse:
+   // the axis-gradient queue (qa, qb: position, label; qc: the gradient)
+   __shared__ uint4 qa[WG / 64][QCAP];
+   __shared__ uint4 qb[WG / 64][QCAP];
+   __shared__ uint4 qc[WG / 64][QCAP];
+   const Params &P = *prm;
+   const int np_lds = P.n_props < MAX_ADJ_PROPS ? (int)P.n_props : MAX_ADJ_PROPS;
+   for (int k = threadIdx.x; k < np_lds; k += WG) {
+      sp[k] = props[k];
+      acc[k][0] = 0.0;
+      acc[k][1] = 0.0;
+   }
+   __syncthreads();
+   auto prop = [&](int32_t l) -> PropDev { return PROPS_LDS ? sp[l] : props[l]; };
+
+   const int tilt = tilt0 + (int)(blockIdx.x % (unsigned)n_sub);
+   const int64_t tile = blockIdx.x / (unsigned)n_sub;
+   const int lane = (int)(threadIdx.x & 63u);
+   uint4 *const QA = qa[threadIdx.x >> 6];
+   uint4 *const QB = qb[threadIdx.x >> 6];
+   uint4 *const QC = qc[threadIdx.x >> 6];
+   const int64_t ray = tile * WG + threadIdx.x;
+   const int64_t wave_ray0 = ray - lane;
+   const int64_t i = ray / P.ny, j = ray - i * P.ny;
+   const bool valid = i < P.nx;
+   const TiltDev &T = tilts[tilt];
+   const int64_t nxy = P.nx * P.ny;
+   const Field<Idx, ST_F32> F{rec_a, nullptr, (int32_t)P.nx, (int32_t)P.ny, (int32_t)P.nz,
+                              P.flip_z};
+   const int slerp = P.interp == 2;
+   AxisRot rot;
+   for (int k = 0; k < 9; k++) rot.R[k] = T.R[k];
+   rot.on = T.rotate != 0;
+   Ckpt *const my = ck + ((int64_t)tilt * nxy + (valid ? ray : 0)) * ck_stride;
+   StepRes *const res_t = res + (int64_t)tilt * nxy * res_stride;
+
+   double px = 0.0, py = 0.0, pz = 0.0;
+   if (P.untilt) {
+      px = ((double)i + 0.5) - T.half_shift[0];
+      py = ((double)j + 0.5) - T.half_shift[1];
+   } else if (valid) {
+      tilted_start(T, i, j, px, py);
+   }
+   px = px + T.half_step[0];
+   py = py + T.half_step[1];
+   pz = pz + T.half_step[2];
+   const double sx = T.step[0], sy = T.step[1], sz = T.step[2];
+   const double nzd = (double)P.nz;
+
+   // ---------------------------------------------------------------- forward
+   float h1 = 0.0f, h2 = 0.0f, h3 = -1.0f, l1 = 0.0f, l2 = 0.0f, l3 = 0.0f;
+   float mh = 0.0f, ml = 0.0f;
+   int64_t n = 0;        // steps taken
+   uint32_t nb = 0;      // birefringent steps queued
+   int64_t ck_next = 0;  // first checkpoint whose state is not written yet
+   {
+      uint32_t pend = 0;
+      int npend = 0;
+      uint32_t tail = 0, head = 0;
+      const double fz0 = floor(pz);
+      bool live = valid & (fz0 >= 0.0) & (fz0 < nzd);
+      Vox r = F.fetch(px, py, fz0, live);
+      for (;;) {
+         int32_t lab = r.lab;
+         if (lab < 0 || lab >= P.n_props) lab = 0; // ranges are checked before the launch
+         const PropDev pd = prop(lab);
+         const bool bire = live && !pd.bg;
+         const bool go = live && (!bire || npend < QDEPTH);
+         const uint64_t gomask = __ballot(go);
+         if (gomask == 0 && tail == head) break;
+         if (gomask != 0) {
+            const uint64_t qmask = __ballot(go && bire);
+            if (go) {
+               if (n % CKQ == 0) {
+                  Ckpt &c = my[n / CKQ];
+                  c.p[0] = px;
+                  c.p[1] = py;
+                  c.p[2] = pz;
+                  c.pad = __uint_as_float(nb);
+               }
+               if (bire) {
+                  const uint32_t seq =
+                      tail + __builtin_amdgcn_mbcnt_hi(
+                                 (uint32_t)(qmask >> 32),
+                                 __builtin_amdgcn_mbcnt_lo((uint32_t)qmask, 0u));
+                  const uint32_t slot = seq & (QCAP - 1);
+                  QA[slot] = make_uint4(__double2loint(px), __double2hiint(px),
+                                        __double2loint(py), __double2hiint(py));
+                  QB[slot] = make_uint4(__double2loint(pz), __double2hiint(pz), (uint32_t)lab,
+                                        0u);
+                  QC[slot] = make_uint4(nb, (uint32_t)n, (uint32_t)lane, 0u);
+                  pend |= (seq & 255u) << (8 * npend);
+                  npend++;
+                  nb++;
+               } else if (pd.mu_t != 0.0f) {
+                  two_sum(mh, ml, pd.mu_t);
+               }
+               n++;
+               px = px + sx;
+               py = py + sy;
+               pz = pz + sz;
+               const double fz = floor(pz);
+               live = (fz >= 0.0) & (fz < nzd);
+               r = F.fetch(px, py, fz, live);
+            }
+            tail += (uint32_t)__popcll(qmask);
+         }
+         const uint32_t queued = tail - head;
+         if (queued >= 64 || gomask == 0) {
+            const uint32_t nq = queued < 64 ? queued : 64;
+            wave_sync();
+            if ((uint32_t)lane < nq) {
+               const uint32_t slot = (head + (uint32_t)lane) & (QCAP - 1);
+               const uint4 a = QA[slot], b = QB[slot], cq = QC[slot];
+               const int32_t elab = (int32_t)b.z;
+               const PropDev epd = prop(elab);
+               const double ex = pack_double(a.x, a.y), ey = pack_double(a.z, a.w);
+               const double ez = pack_double(b.x, b.y);
+               float3 v = get_vec_interp<Idx, ST_F32, false>(
+                   F, ex, ey, ez, elab, make_float3(0.0f, 0.0f, 0.0f), slerp);
+               StepRes o;
+               o.C = 1.0f;
+               o.S = 0.0f;
+               o.sd = 0.0f;
+               o.omc = 0.0f;
+               o.g = -1.0f;
+               o.vx = o.vy = o.vz = 0.0f;
+               float mu = 0.0f;
+               if (!is_zero(v)) {
+                  const float nrm = sqrtf(dot3(v, v));
+                  if (rot.on) {
+                     const float3 w = v;
+                     v.x = rot.R[0] * w.x + rot.R[1] * w.y + rot.R[2] * w.z;
+                     v.y = rot.R[3] * w.x + rot.R[4] * w.y + rot.R[5] * w.z;
+                     v.z = rot.R[6] * w.x + rot.R[7] * w.y + rot.R[8] * w.z;
+                  }
+                  const float xx = v.x * v.x, yy = v.y * v.y;
+                  const float rxy = xx + yy;
+                  const float rr = rxy + v.z * v.z;
+                  o.g = nrm * (rxy / rr);
+                  if (rxy > 0.0f) {
+                     const float inv = 1.0f / rxy;
+                     o.C = (xx - yy) * inv;
+                     o.S = 2.0f * v.x * v.y * inv;
+                  }
+                  sin_omc((epd.K * nrm) * (rxy / rr), o.sd, o.omc);
+                  o.vx = v.x;
+                  o.vy = v.y;
+                  o.vz = v.z;
+                  mu = epd.mu_t;
+               }
+               res_t[(wave_ray0 + (int64_t)cq.z) * res_stride + cq.x] = o;
+               QA[slot] = make_uint4(__float_as_uint(o.C), __float_as_uint(o.S),
+                                     __float_as_uint(o.sd), __float_as_uint(o.omc));
+               QB[slot].x = __float_as_uint(mu);
+            }
+            wave_sync();
+            int k = 0;
+#pragma unroll
+            for (int q = 0; q < QDEPTH; q++)
+               k += (q < npend) & (((((pend >> (8 * q)) & 255u) - head) & 255u) < nq);
+            for (int q = 0; q < k; q++) {
+               const uint32_t slot = (pend >> (8 * q)) & (QCAP - 1);
+               // the state before this step closes the checkpoints up to it
+               const int64_t step = (int64_t)QC[slot].y;
+               while (ck_next * CKQ <= step) {
+                  Ckpt &c = my[ck_next];
+                  c.u[0] = h1 + l1;
+                  c.u[1] = h2 + l2;
+                  c.u[2] = h3 + l3;
+                  ck_next++;
+               }
+               const uint4 e = QA[slot];
+               const float mu = __uint_as_float(QB[slot].x);
+               apply_retarder(__uint_as_float(e.x), __uint_as_float(e.y), __uint_as_float(e.z),
+                              __uint_as_float(e.w), h1, h2, h3, l1, l2, l3);
+               two_sum(mh, ml, mu);
+            }
+            pend = (uint32_t)((uint64_t)pend >> (8 * k));
+            npend -= k;
+            head += nq;
+         }
+      }
+      while (ck_next * CKQ < n) { // no step after these checkpoints changes the state
+         Ckpt &c = my[ck_next];
+         c.u[0] = h1 + l1;
+         c.u[1] = h2 + l2;
+         c.u[2] = h3 + l3;
+         ck_next++;
+      }
+   }
+
+   // ------------------------------------------------------------------ seeds
+   float b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
+   double mbar = 0.0;
+   if (valid) {
+      const double A = exp(-((double)mh + (double)ml));
+      const double u1 = (double)h1 + (double)l1, u2 = (double)h2 + (double)l2;
+      const double base = P.quarter_I0 * A;
+      const double *g = grad_images + ((int64_t)T.out * nxy + ray) * P.n_rho;
+      double gc = 0.0, gs = 0.0;
+      for (int k = 0; k < P.n_rho; k++) {
+         const double I = base * (1.0 - (P.c2r[k] * u1 - P.s2r[k] * u2));
+         mbar -= g[k] * I;
+         gc += g[k] * P.c2r[k];
+         gs += g[k] * P.s2r[k];
+      }
+      b1 = (float)(-base * gc);
+      b2 = (float)(base * gs);
+   }
+
+   // ---------------------------------------------------------------- reverse
+   int32_t run_lab = -1;
+   double run_k = 0.0;
+   int64_t run_n = 0;
+   auto flush = [&]() {
+      if (run_lab < 0) return;
+      const double dk = run_k * kfac, dm = mbar * (double)run_n * mfac;
+      if (run_lab < MAX_ADJ_PROPS) {
+         atomicAdd(&acc[run_lab][0], dk);
+         atomicAdd(&acc[run_lab][1], dm);
+      } else {
+         atomicAdd(&g_props[2 * run_lab], dk);
+         atomicAdd(&g_props[2 * run_lab + 1], dm);
+      }
+   };
+   StepRes *const mr = res_t + (valid ? ray : 0) * res_stride;
+   // every valid ray of the wave took the same number of steps
+   int64_t nw = n;
+   {
+      const uint64_t vm = __ballot(valid);
+      const int first = vm ? __builtin_ctzll(vm) : 0;
+      nw = (int64_t)__builtin_amdgcn_readlane((int)n, first);
+   }
+   uint32_t gtail = 0, ghead = 0; // the axis-gradient queue
+   auto gdrain = [&](uint32_t ng) {
+      wave_sync();
+      grad_drain<Idx>(F, QA, QB, QC, ghead, ng, lane, slerp, g_vec);
+      wave_sync();
+      ghead += ng;
+   };
+   for (int64_t sgm = (nw - 1) / CKQ; sgm >= 0 && nw > 0; sgm--) {
+      // the segment: labels of its steps, and the states before them
+      const int len = (int)min((int64_t)CKQ, nw - sgm * CKQ);
+      double q0x = 0.0, q0y = 0.0, q0z = 0.0;
+      int32_t lab[CKQ];
+      bool bi[CKQ];
+      float ux[CKQ], uy[CKQ], uz[CKQ];
+      uint32_t ri = 0; // the count index of the segment's first birefringent step
+      if (valid) {
+         const Ckpt c = my[sgm];
+         q0x = c.p[0];
+         q0y = c.p[1];
+         q0z = c.p[2];
+         ri = __float_as_uint(c.pad);
+         float a1 = c.u[0], a2 = c.u[1], a3 = c.u[2], e1 = 0.0f, e2 = 0.0f, e3 = 0.0f;
+         double qx = q0x, qy = q0y, qz = q0z;
+         uint32_t rk = ri;
+#pragma unroll
+         for (int t = 0; t < CKQ; t++) {
+            ux[t] = a1 + e1;
+            uy[t] = a2 + e2;
+            uz[t] = a3 + e3;
+            lab[t] = 0;
+            bi[t] = false;
+            if (t < len) {
+               int32_t ix, iy, iz;
+               const Vox rr = fetch_own(F, qx, qy, qz, ix, iy, iz);
+               const int32_t l = (rr.lab < 0 || rr.lab >= P.n_props) ? 0 : rr.lab;
+               lab[t] = l;
+               bi[t] = !prop(l).bg;
+               if (bi[t]) {
+                  const StepRes o = mr[rk++];
+                  if (o.g >= 0.0f) {
+                     const float w = o.S * a1 + o.C * a2;
+                     const float g = o.sd * a3 - o.omc * w;
+                     const float d3 = -(o.sd * w + o.omc * a3);
+                     two_sum(a1, e1, o.S * g);
+                     two_sum(a2, e2, o.C * g);
+                     two_sum(a3, e3, d3);
+                  }
+               }
+               qx = qx + sx;
+               qy = qy + sy;
+               qz = qz + sz;
+            }
+         }
+      }
+      // backwards through the segment (uniform over the wave)
+      uint32_t rk = ri;
+#pragma unroll
+      for (int t = 0; t < CKQ; t++)
+         rk += (valid && t < len && bi[t]) ? 1u : 0u;
+#pragma unroll
+      for (int t = CKQ - 1; t >= 0; t--) {
+         bool qg = false;
+         float w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
+         double rx = q0x, ry = q0y, rz = q0z;
+         if (valid && t < len) {
+            for (int q = 0; q < t; q++) {
+               rx = rx + sx;
+               ry = ry + sy;
+               rz = rz + sz;
+            }
+            StepRes o;
+            o.g = 0.0f;
+            if (bi[t]) o = mr[--rk];
+            const bool skipped = bi[t] && o.g < 0.0f; // zero axis: no attenuation either
+            if (!skipped) {
+               if (lab[t] != run_lab) {
+                  flush();
+                  run_lab = lab[t];
+                  run_k = 0.0;
+                  run_n = 0;
+               }
+               run_n++;
+            }
+            if (bi[t] && !skipped) {
+               const float x1 = ux[t], x2 = uy[t], x3 = uz[t];
+               const float sd = o.sd, omc = o.omc, co = 1.0f - omc, C = o.C, S = o.S;
+               const float nv = C * x1 - S * x2;
+               const float dd1 = co * S * x3 + sd * (C * nv - x1);
+               const float dd2 = co * C * x3 + sd * (-S * nv - x2);
+               const float dd3 = -co * (C * x2 + S * x1) - sd * x3;
+               const float dbar = b1 * dd1 + b2 * dd2 + b3 * dd3;
+               const float cbar = b1 * (omc * (nv + C * x1)) + b2 * (sd * x3 - omc * S * x1) +
+                                  b3 * (-sd * x2);
+               const float sbar = b1 * (sd * x3 - omc * C * x2) + b2 * (omc * (S * x2 - nv)) +
+                                  b3 * (-sd * x1);
+               run_k += (double)(dbar * o.g);
+               if (g_vec) {
+                  const float vx = o.vx, vy = o.vy, vz = o.vz;
+                  const float rxy = vx * vx + vy * vy;
+                  if (rxy > 0.0f) {
+                     const float K = prop(lab[t]).K;
+                     const float n2 = rxy + vz * vz, nr = sqrtf(n2);
+                     const float in3 = 1.0f / (n2 * nr), ir2 = 1.0f / (rxy * rxy);
+                     const float ddx = K * (2.0f * vx / nr - rxy * vx * in3);
+                     const float ddy = K * (2.0f * vy / nr - rxy * vy * in3);
+                     const float ddz = -K * rxy * vz * in3;
+                     const float dcx = 4.0f * vx * vy * vy * ir2, dcy = -4.0f * vy * vx * vx * ir2;
+                     const float dsx = 2.0f * vy * (vy * vy - vx * vx) * ir2;
+                     const float dsy = 2.0f * vx * (vx * vx - vy * vy) * ir2;
+                     w0 = dbar * ddx + cbar * dcx + sbar * dsx;
+                     w1 = dbar * ddy + cbar * dcy + sbar * dsy;
+                     w2 = dbar * ddz;
+                     if (rot.on) {
+                        const float a = rot.R[0] * w0 + rot.R[3] * w1 + rot.R[6] * w2;
+                        const float b = rot.R[1] * w0 + rot.R[4] * w1 + rot.R[7] * w2;
+                        const float cc = rot.R[2] * w0 + rot.R[5] * w1 + rot.R[8] * w2;
+                        w0 = a;
+                        w1 = b;
+                        w2 = cc;
+                     }
+                     qg = true;
+                  }
+               }
+               const float nu = C * b1 - S * b2;
+               const float k1 = -S * b3, k2 = -C * b3, k3 = S * b1 + C * b2;
+               const float q1 = C * nu - b1, q2 = -S * nu - b2, q3 = -b3;
+               b1 = b1 + sd * k1 + omc * q1;
+               b2 = b2 + sd * k2 + omc * q2;
+               b3 = b3 + sd * k3 + omc * q3;
+            }
+         }
+         // the step's axis gradient to the corners, 64 steps at a time
+         const uint64_t gm = __ballot(qg);
+         if (qg) {
+            const uint32_t gslot =
+                (gtail + __builtin_amdgcn_mbcnt_hi((uint32_t)(gm >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)gm, 0u))) &
+                (QCAP - 1);
+            QA[gslot] = make_uint4(__double2loint(rx), __double2hiint(rx), __double2loint(ry),
+                                   __double2hiint(ry));
+            QB[gslot] = make_uint4(__double2loint(rz), __double2hiint(rz), (uint32_t)lab[t], 0u);
+            QC[gslot] = make_uint4(__float_as_uint(w0), __float_as_uint(w1),
+                                   __float_as_uint(w2), 0u);
+         }
+         gtail += (uint32_t)__popcll(gm);
+         if (gtail - ghead >= 64) gdrain(64);
+      }
+   }
+   while (gtail != ghead) gdrain(gtail - ghead < 64 ? gtail - ghead : 64);
+   flush();
    __syncthreads();
    for (int k = threadIdx.x; k < np_lds; k += WG) {
       if (acc[k][0] != 0.0) atomicAdd(&g_props[2 * k], acc[k][0]);
@@ -1715,7 +2176,12 @@ Status adjoint(const fpl_setup *setup, const int64_t dim[3], const int32_t *labe
    const TiltDev *tdev = (const TiltDev *)(B.bytes.data() + B.off_t);
    int64_t max_steps = 0;
    for (int64_t t = 0; t < n_tilts; t++) max_steps = std::max(max_steps, tilt_steps(tdev[t], dim[2]));
-   const int64_t ck_stride = std::max<int64_t>(1, (max_steps + CK - 1) / CK);
+   // the interpolating tilts (first in the block's order) run the queued
+   // adjoint, the others k_adjoint; both index checkpoints by the tilt
+   int n_iq = 0;
+   for (int64_t t = 0; t < n_tilts; t++) n_iq += (setup->interpolate != 0 && !tdev[t].centers);
+   const int64_t ck_stride = std::max<int64_t>(1, (max_steps + CKQ - 1) / CKQ);
+   const int64_t res_stride = std::max<int64_t>(1, max_steps);
    const double lambda = setup->wavelength * 1e-9;
    const double thickness = setup->voxel_size * 1e-6 * setup->step_size;
    const double kfac = M_PI_2 * (4.0 * thickness / lambda); // dK / d dn
@@ -1729,9 +2195,13 @@ Status adjoint(const fpl_setup *setup, const int64_t dim[3], const int32_t *labe
    const int64_t n_vox = dim[0] * dim[1] * dim[2];
    const size_t ck_bytes = sizeof(Ckpt) * (size_t)n_tilts * (size_t)(dim[0] * dim[1]) *
                            (size_t)ck_stride;
-   FPL_TRY(ws_get(st, WS_ADJOINT, ck_bytes + gp_bytes, &scratch));
+   // the queued adjoint's stored step results: a row of max_steps per ray
+   const size_t res_bytes = sizeof(StepRes) * (size_t)n_iq * (size_t)(dim[0] * dim[1]) *
+                            (size_t)res_stride;
+   FPL_TRY(ws_get(st, WS_ADJOINT, ck_bytes + res_bytes + gp_bytes, &scratch));
    Ckpt *d_ck = (Ckpt *)scratch;
-   double *d_gp = (double *)((char *)scratch + ck_bytes);
+   StepRes *d_res = (StepRes *)((char *)scratch + ck_bytes);
+   double *d_gp = (double *)((char *)scratch + ck_bytes + res_bytes);
    FPL_HIP(hipStreamSynchronize(st));
    memcpy(stage, B.bytes.data(), B.bytes.size());
    FPL_HIP(hipMemcpyAsync(small, stage, B.bytes.size(), hipMemcpyHostToDevice, st));
@@ -1753,14 +2223,24 @@ Status adjoint(const fpl_setup *setup, const int64_t dim[3], const int32_t *labe
    if (grad_vector) FPL_HIP(hipMemsetAsync(grad_vector, 0, sizeof(float) * 3 * n_vox, st));
 
    const int64_t n_tiles = ray_tiles(dim[0], dim[1]);
-   const dim3 grid((unsigned)(n_tiles * n_tilts));
    const Params *d_p = (const Params *)small;
    const TiltDev *d_t = (const TiltDev *)((const char *)small + B.off_t);
    const PropDev *d_pr = (const PropDev *)((const char *)small + B.off_pr);
    KernelTimer kt("adjoint", st);
+   const int n_nq = (int)n_tilts - n_iq;
+   const dim3 grid_q((unsigned)(n_tiles * (n_iq > 0 ? n_iq : 1)));
+   const dim3 grid((unsigned)(n_tiles * (n_nq > 0 ? n_nq : 1)));
 #define FPL_ADJ(IDX, PL)                                                                          \
-   hipLaunchKernelGGL((k_adjoint<IDX, PL>), grid, dim3(WG), 0, st, d_p, d_t, d_pr, packed,          \
-                      grad_images, d_ck, ck_stride, kfac, mfac, d_gp, grad_vector)
+   do {                                                                                           \
+      if (n_iq > 0)                                                                               \
+         hipLaunchKernelGGL((k_adjoint_queue<IDX, PL>), grid_q, dim3(WG), 0, st, d_p, d_t, d_pr,  \
+                            packed, grad_images, d_ck, ck_stride, d_res, res_stride, kfac, mfac,  \
+                            d_gp, grad_vector, 0, n_iq);                                          \
+      if (n_nq > 0)                                                                               \
+         hipLaunchKernelGGL((k_adjoint<IDX, PL>), grid, dim3(WG), 0, st, d_p, d_t, d_pr, packed,  \
+                            grad_images, d_ck, ck_stride, kfac, mfac, d_gp, grad_vector, n_iq,    \
+                            n_nq);                                                                \
+   } while (0)
    const bool props_lds = n_props <= MAX_ADJ_PROPS;
    if (n_vox < ((int64_t)1 << 32)) {
       if (props_lds) FPL_ADJ(uint32_t, true); else FPL_ADJ(uint32_t, false);
