@@ -85,10 +85,14 @@ def adjoint_bench(reps=5):
     """One vMF bundle (400 fibers, 300 x 300 x 150 @ 0.2 um), 5 tilts x 9 rotations,
     NN and Slerp: the forward march and the adjoint kernel (gradients w.r.t.
     the (dn, mu) table and the optical-axis field of L = sum(g * images)),
-    HIP-event times.  The adjoint's algorithmic bytes per ray-step: the 16-byte
-    record read three times (forward, replay, backward) and a 40-byte
-    checkpoint written and read once per 8 steps; per pixel the float64
-    gradient images read."""
+    HIP-event times.  The adjoint's algorithmic bytes, NN (one thread per ray):
+    per ray-step the 16-byte record read three times (forward, replay,
+    backward) and a 40-byte checkpoint written and read once per 8 steps.
+    Slerp (the queued adjoint): per ray-step the record read twice (forward,
+    replay) and a 40-byte checkpoint written and read once per 4 steps; per
+    birefringent step a 32-byte step result written once and read twice
+    (birefringent steps taken as the ray-steps times the birefringent share
+    of the voxels).  Both: per pixel the float64 gradient images read."""
     import torch
     from fastpli_amd import _lib, synthetic
     from fastpli_amd.simulation import simulate_stack
@@ -99,9 +103,11 @@ def adjoint_bench(reps=5):
         warnings.simplefilter('ignore')
         label, vector, props = s.generate_tissue()
     lab = torch.from_numpy(label).to(dev)
+    bire = float(((props[label, 0] != 0) & (label != 0)).mean())
     out = {'config': 'one vMF(kappa 10) bundle of 400 fibers, 300x300x150 voxels @0.2um, '
                      '5 tilts x 9 rotations',
-           'gradients': 'dn, mu per label; optical axis per voxel'}
+           'gradients': 'dn, mu per label; optical axis per voxel',
+           'birefringent_share': bire}
     for interp in ('NN', 'Slerp'):
         s.interpolate = interp
         axis = torch.from_numpy(vector).to(dev).requires_grad_(True)
@@ -121,7 +127,11 @@ def adjoint_bench(reps=5):
         adj, na = _lib.profile_read('adjoint')
         _lib.profile_enable(False)
         ray_steps = sum(s.count_voxel_steps(t, p) for t, p in TILTS) / ROT.size
-        nbytes = ray_steps * (3 * 16.0 + 2 * 40.0 / 8) + len(TILTS) * dim[0] * dim[1] * 8.0 * ROT.size
+        nbytes = len(TILTS) * dim[0] * dim[1] * 8.0 * ROT.size
+        if interp == 'NN':
+            nbytes += ray_steps * (3 * 16.0 + 2 * 40.0 / 8)
+        else:
+            nbytes += ray_steps * (2 * 16.0 + 2 * 40.0 / 4) + ray_steps * bire * 3 * 32.0
         adj_ms = adj / max(na, 1)
         out[interp] = {'forward_march_ms': fwd / max(nf, 1), 'adjoint_ms': adj_ms,
                        'adjoint_bytes': nbytes, 'adjoint_GBs': nbytes / (adj_ms * 1e-3) / 1e9}
