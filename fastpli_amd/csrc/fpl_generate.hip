@@ -952,13 +952,16 @@ __device__ __forceinline__ void write_voxel(const Outputs &o, const Params &pr, 
    // the past-the-end layer index of the last bundle gives label n_props
    // (beyond the property table, which CheckInput rejects): bit 0 of flags[3]
    if (lab >= pr.n_props) atomicOr(&flags[3], 1u);
+   // streaming (nontemporal) stores: nothing here reads the outputs again, and
+   // L2 keeps the segment records the tiles stage instead (the arrays and the
+   // records of 600 x 600 x 300 voxels, 3.5 GB, would otherwise pass through it)
    if (o.label || o.vector) {
       const int64_t ind = ((int64_t)vx * pr.ny + vy) * pr.nz + vz;
-      if (o.label) o.label[ind] = lab;
+      if (o.label) __builtin_nontemporal_store(lab, o.label + ind);
       if (o.vector) {
-         o.vector[3 * ind + 0] = vo[0];
-         o.vector[3 * ind + 1] = vo[1];
-         o.vector[3 * ind + 2] = vo[2];
+         typedef float v3f __attribute__((ext_vector_type(3), aligned(4)));
+         const v3f e = {vo[0], vo[1], vo[2]};
+         __builtin_nontemporal_store(e, (v3f *)(o.vector + 3 * ind));
       }
    }
    if (o.rec_a) {
@@ -966,8 +969,9 @@ __device__ __forceinline__ void write_voxel(const Outputs &o, const Params &pr, 
       if constexpr (REC16)
          store_record_short(o, r, lab, vo[0], vo[1], vo[2]);
       else
-         ((int4 *)o.rec_a)[r] =
-             make_int4(lab, __float_as_int(vo[0]), __float_as_int(vo[1]), __float_as_int(vo[2]));
+         rec::store(rec::ST_F32, o.rec_a, nullptr, r,
+                    make_int4(lab, __float_as_int(vo[0]), __float_as_int(vo[1]),
+                              __float_as_int(vo[2])));
    }
 }
 
@@ -1046,8 +1050,12 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
       W.key[s] = KEY_EMPTY;
       if constexpr (!short_seg) W.seg[s] = (SegT)-1;
    }
-   bool ov = (__atomic_load_n(&flags[0], __ATOMIC_RELAXED) & FPL_WARN_OVERLAP) != 0;
-   ov = uniform((int)ov) != 0;
+   // one read of the warning for the workgroup: the mode below must be the
+   // same in all its waves, whose keys the final pass reads across waves
+   __shared__ uint32_t wg_warn;
+   if (threadIdx.x == 0) wg_warn = __atomic_load_n(&flags[0], __ATOMIC_RELAXED);
+   __syncthreads();
+   bool ov = uniform((int)((wg_warn & FPL_WARN_OVERLAP) != 0)) != 0;
    // free mode (see the key layout above): the warning is raised already, or
    // the list (global segment order, bundle-major) starts and ends in one
    // bundle, so no update here can replace another bundle's label (but for
@@ -1245,7 +1253,14 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
                             0)) != 0;
    }
 
-   wave_sync();
+   // the results: wave w writes the layers z = zt + 8 w + (0..7) of the tile
+   // (lane: y = lane >> 3, z offset lane & 7), each voxel's key read from the
+   // wave that owns its layer, so that a column's eight labels and axes are
+   // contiguous; the keys are final once every wave is here
+   __syncthreads();
+   const int zo = 8 * wave + (lane & 7);
+   const WaveState<SegT> &WS = wstate[zo & 3];
+   const int lbase = (zo >> 2) * 64 + (lane >> 3);
    const bool need_vec = out.vector != nullptr || out.rec_a != nullptr;
    uint64_t nonzero = 0; // wave-uniform: some voxel of the wave got a label != 0
    if (short_seg && free_mode) {
@@ -1256,21 +1271,25 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
       uint32_t los[8];
 #pragma unroll
       for (int j = 0; j < 8; j++) {
-         const unsigned long long key = W.key[(lane & 7) * 64 + j * 8 + (lane >> 3)];
+         const unsigned long long key = WS.key[lbase + j * 8];
          const bool hit = (uint32_t)(key >> 32) != 0x7F800000u;
          los[j] = hit ? (uint32_t)key : 0u;
          sgs[j] = -1;
          if (need_vec && ((los[j] >> 14) & 3u) != 0u)
             sgs[j] = (int32_t)list[beg + (0x10000u - (los[j] >> 16)) - 1u];
       }
+      // every load of the pass before its first store: on this part a load
+      // issued after a store also waits for the store's completion
+      int32_t labs[8];
+      float vos[8][3];
 #pragma unroll
       for (int j = 0; j < 8; j++) {
-         const int col = j * 8 + (lane >> 3), zz = lane & 7;
-         const int vx = xt + (col >> 3), vy = yt + (col & 7), vz = zw + 4 * zz;
+         const int vx = xt + j, vy = yt + (lane >> 3), vz = zt + zo;
+         const uint32_t lo = los[j];
+         labs[j] = (int32_t)(lo & 0x3FFFu);
+         float *vo = vos[j];
+         vo[0] = vo[1] = vo[2] = 0.0f;
          if (vx < pr.nx && vy < pr.ny && vz < pr.nz) {
-            const uint32_t lo = los[j];
-            const int32_t lab = (int32_t)(lo & 0x3FFFu);
-            float vo[3] = {0.0f, 0.0f, 0.0f};
             if (need_vec) {
                const uint32_t m2 = (lo >> 14) & 3u;
                if (m2 == 1u) {
@@ -1288,8 +1307,14 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
                   unit_to_float(pt[0] - cp.c[0], pt[1] - cp.c[1], pt[2] - cp.c[2], vo);
                }
             }
-            write_voxel<REC16>(out, pr, vx, vy, vz, lab, vo, flags);
-            nonzero |= __ballot(lab != 0);
+         }
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+         const int vx = xt + j, vy = yt + (lane >> 3), vz = zt + zo;
+         if (vx < pr.nx && vy < pr.ny && vz < pr.nz) {
+            write_voxel<REC16>(out, pr, vx, vy, vz, labs[j], vos[j], flags);
+            nonzero |= __ballot(labs[j] != 0);
          }
       }
       note_nonzero(nonzero, flags);
@@ -1300,26 +1325,25 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
    // the tile list for 16-bit positions) two passes ahead, the record one pass
    // ahead.
    auto winner = [&](int j) -> int32_t {
-      const int slot = (lane & 7) * 64 + j * 8 + (lane >> 3);
+      const int slot = lbase + j * 8;
       if constexpr (short_seg) {
-         const unsigned long long key = W.key[slot];
+         const unsigned long long key = WS.key[slot];
          const uint32_t w = (uint32_t)key & 0xFFFFu; // list position + 1 (0: none)
          return w ? (int32_t)list[beg + w - 1u] : -1;
       } else {
-         return (int32_t)W.seg[slot];
+         return (int32_t)WS.seg[slot];
       }
    };
    int32_t sg = winner(0), sg1 = winner(1);
    WinRec cur, nxt;
    if (sg >= 0) cur = load_winner(segs[sg], seg_dir[sg]);
    for (int j = 0; j < 8; j++) {
-      const int col = j * 8 + (lane >> 3), zz = lane & 7;
-      const int slot = zz * 64 + col;
+      const int slot = lbase + j * 8;
       const int32_t sg2 = j < 6 ? winner(j + 2) : -1;
       if (j < 7 && sg1 >= 0) nxt = load_winner(segs[sg1], seg_dir[sg1]);
-      const int vx = xt + (col >> 3), vy = yt + (col & 7), vz = zw + 4 * zz;
+      const int vx = xt + j, vy = yt + (lane >> 3), vz = zt + zo;
       if (vx < pr.nx && vy < pr.ny && vz < pr.nz) {
-         const unsigned long long key = W.key[slot];
+         const unsigned long long key = WS.key[slot];
          int32_t lab = (int32_t)(uint32_t)(key & 0x7FFFFFFFull);
          if (short_seg) {
             const uint32_t l15 = (uint32_t)(key >> 16) & 0x7FFFu;

@@ -80,14 +80,20 @@ __device__ __forceinline__ void encode(int ST, int32_t l, float x, float y, floa
 }
 
 // Stores an encoded voxel at record index o.
+// (streaming stores: the writers, the voxelizer and the pack, never read the
+// records back)
 __device__ __forceinline__ void store(int ST, void *a, uint32_t *b, int64_t o, const int4 &e) {
+   typedef int v4i __attribute__((ext_vector_type(4)));
+   typedef uint32_t v2u __attribute__((ext_vector_type(2)));
    if (ST == ST_F32) {
-      ((int4 *)a)[o] = e;
+      const v4i v = {e.x, e.y, e.z, e.w};
+      __builtin_nontemporal_store(v, (v4i *)a + o);
    } else if (ST == ST_F16) {
-      ((uint2 *)a)[o] = make_uint2((uint32_t)e.x, (uint32_t)e.y);
+      const v2u v = {(uint32_t)e.x, (uint32_t)e.y};
+      __builtin_nontemporal_store(v, (v2u *)a + o);
    } else {
-      ((uint16_t *)a)[o] = (uint16_t)e.x;
-      b[o] = (uint32_t)e.y;
+      __builtin_nontemporal_store((uint16_t)e.x, (uint16_t *)a + o);
+      __builtin_nontemporal_store((uint32_t)e.y, b + o);
    }
 }
 
