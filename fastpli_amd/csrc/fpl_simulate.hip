@@ -384,6 +384,26 @@ __host__ __device__ inline int64_t ray_tiles(int64_t nx, int64_t ny) {
    return (nx * ny + WG - 1) / WG;
 }
 
+// Workgroup b of a launch over n_tiles ray tiles x nh tilts -> (tile, tilt).
+// Workgroups go to the 8 XCDs round-robin (b mod 8), each XCD with its own
+// L2: the nh tilts of a tile, which read overlapping columns at the same
+// time, get workgroup numbers equal mod 8 (one XCD), consecutive on that XCD;
+// tiles stay interleaved over the XCDs, which keeps their loads even.  The
+// last n_tiles mod 8 tiles take the plain order.
+__device__ __forceinline__ void tile_tilt(unsigned b, unsigned nh, unsigned n_tiles, int &tilt,
+                                          int64_t &tile) {
+   const unsigned full = n_tiles / 8u * 8u * nh;
+   if (b < full) {
+      const unsigned g = b / (8u * nh), r = b - g * (8u * nh);
+      tilt = (int)(r >> 3);
+      tile = (int64_t)g * 8 + (r & 7u);
+   } else {
+      const unsigned c = b - full;
+      tilt = (int)(c % nh);
+      tile = (int64_t)(full / nh + c / nh);
+   }
+}
+
 // Wave-cooperative march of interpolating views.  A quarter of the voxels of
 // a fiber tissue are birefringent, scattered over the 64 rays of a wave; one
 // thread per ray would run the costly part of a step (corner gather +
@@ -494,19 +514,19 @@ __global__ void __launch_bounds__(WG, 6)
    }
    auto prop = [&](int32_t l) -> PropDev { return PROPS_LDS ? sp[l] : props[l]; };
 
-   // the interpolating tilts of one ray tile are consecutive workgroups: they
-   // march through overlapping columns at the same time, so re-reads hit L2 /
-   // MALL.  Tilts without interpolation (a third of the work per ray) come
-   // last and fill the end of the launch, when the long rays are finishing.
+   // the interpolating tilts of one ray tile run on one XCD (tile_tilt): they
+   // march through overlapping columns at the same time, so re-reads hit its
+   // L2.  Tilts without interpolation (a third of the work per ray) come last
+   // and fill the end of the launch, when the long rays are finishing.
    int tilt;
    int64_t tile;
    {
       const unsigned nh = (unsigned)P.n_heavy, nt = (unsigned)P.n_tilts;
-      const unsigned split = nh * (unsigned)ray_tiles(P.nx, P.ny);
+      const unsigned n_tiles = (unsigned)ray_tiles(P.nx, P.ny);
+      const unsigned split = nh * n_tiles;
       const unsigned b = blockIdx.x;
       if (b < split) {
-         tilt = (int)(b % nh);
-         tile = b / nh;
+         tile_tilt(b, nh, n_tiles, tilt, tile);
       } else {
          const unsigned c = b - split;
          tilt = (int)(nh + c % (nt - nh));
@@ -990,8 +1010,10 @@ __global__ void __launch_bounds__(WG)
    __syncthreads();
 
    // the tilts [tilt0, tilt0 + n_sub) of the launch
-   const int tilt = tilt0 + (int)(blockIdx.x % (unsigned)n_sub);
-   const int64_t tile = blockIdx.x / (unsigned)n_sub;
+   int tilt;
+   int64_t tile;
+   tile_tilt(blockIdx.x, (unsigned)n_sub, (unsigned)ray_tiles(P.nx, P.ny), tilt, tile);
+   tilt += tilt0;
    const int64_t ray = tile * WG + threadIdx.x;
    const int64_t i = ray / P.ny, j = ray - i * P.ny;
    const int tid = threadIdx.x;
@@ -1264,8 +1286,10 @@ __global__ void __launch_bounds__(WG)
    __syncthreads();
    auto prop = [&](int32_t l) -> PropDev { return PROPS_LDS ? sp[l] : props[l]; };
 
-   const int tilt = tilt0 + (int)(blockIdx.x % (unsigned)n_sub);
-   const int64_t tile = blockIdx.x / (unsigned)n_sub;
+   int tilt;
+   int64_t tile;
+   tile_tilt(blockIdx.x, (unsigned)n_sub, (unsigned)ray_tiles(P.nx, P.ny), tilt, tile);
+   tilt += tilt0;
    const int lane = (int)(threadIdx.x & 63u);
    uint4 *const QA = qa[threadIdx.x >> 6];
    uint4 *const QB = qb[threadIdx.x >> 6];
