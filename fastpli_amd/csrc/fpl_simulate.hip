@@ -59,8 +59,11 @@ struct TiltDev {
    int32_t rotate;       // theta != 0
    int32_t centers;      // every sample point is a voxel centre: no interpolation
    int32_t out;          // index of the tilt in the caller's order (image slot)
-   int32_t pad_;
+   int32_t trips;        // z-steps of every ray (tilt_steps)
 };
+
+// z-march trip count of one tilt (host)
+int64_t tilt_steps(const TiltDev &T, int64_t nz);
 
 struct Params {
    int64_t nx, ny, nz;
@@ -112,44 +115,70 @@ template <typename Idx, int ST> struct Field {
    // `fz` is inside [0, nz) when `live`; outside x / y it is background
    // (simulator.cpp:430-432).  One branch: all tests are evaluated.
    __device__ __forceinline__ Vox fetch(double px, double py, double fz, bool live) const;
-   // record of an in-range voxel; flip_z mirrors z as GetLabel / GetVec do
-   // (simulator.cpp:427-428, 559-560)
-   __device__ __forceinline__ Vox at(int32_t x, int32_t y, int32_t z) const {
+   // The storage words of an in-range voxel, undecoded: ST_F32 (label, x, y, z),
+   // ST_F16 (label | x << 16, y | z << 16), ST_ANGLE16 (label word, angle
+   // codes).  flip_z mirrors z as GetLabel / GetVec do (simulator.cpp:427-428,
+   // 559-560).  The march decodes the axis only where a step needs it.
+   __device__ __forceinline__ uint4 raw(int32_t x, int32_t y, int32_t z) const {
       if (flip_z) z = nz - 1 - z;
-      const Idx i = ((Idx)x * (Idx)nz + (Idx)z) * (Idx)ny + (Idx)y;
-      Vox o;
+      return load(((Idx)x * (Idx)nz + (Idx)z) * (Idx)ny + (Idx)y);
+   }
+   // the words of record i ([x][z][y] order)
+   __device__ __forceinline__ uint4 load(Idx i) const {
       if (ST == ST_F32) {
          const int4 r = ((const int4 *)a)[i];
-         o.lab = r.x;
-         o.v = make_float3(__int_as_float(r.y), __int_as_float(r.z), __int_as_float(r.w));
-         o.nn = false;
+         return make_uint4((uint32_t)r.x, (uint32_t)r.y, (uint32_t)r.z, (uint32_t)r.w);
       } else if (ST == ST_F16) {
          const uint2 r = ((const uint2 *)a)[i];
-         o.lab = (int32_t)(r.x & 0xffffu);
-         const uint32_t hz = r.y >> 16;
-         o.nn = (hz & 0x8000u) != 0;
-         o.v = make_float3(__half2float(__ushort_as_half((unsigned short)(r.x >> 16))),
-                           __half2float(__ushort_as_half((unsigned short)(r.y & 0xffffu))),
-                           __half2float(__ushort_as_half((unsigned short)(hz & 0x7fffu))));
-         if (!o.nn && !(o.v.x == 0.0f && o.v.y == 0.0f && o.v.z == 0.0f)) {
-            const float inv = rsqrtf(o.v.x * o.v.x + o.v.y * o.v.y + o.v.z * o.v.z);
-            o.v = make_float3(o.v.x * inv, o.v.y * inv, o.v.z * inv);
-         }
-      } else {
-         const uint32_t l = ((const uint16_t *)a)[i];
-         o.lab = (int32_t)(l & 0x7fffu);
-         o.nn = (l & 0x8000u) != 0;
-         const uint32_t w = b[i];
-         const uint32_t tc = w & 0xffffu, pc = w >> 16;
-         if (tc == 0xffffu) {
-            o.v = make_float3(0.0f, 0.0f, 0.0f);
-         } else {
-            const float th = (float)(tc * ANG_THETA), ph = (float)(pc * ANG_PHI) - 3.14159265f;
-            const float st = __sinf(th);
-            o.v = make_float3(st * __cosf(ph), st * __sinf(ph), __cosf(th));
-         }
+         return make_uint4(r.x, r.y, 0u, 0u);
       }
-      return o;
+      return make_uint4((uint32_t)((const uint16_t *)a)[i], b[i], 0u, 0u);
+   }
+   // Loads raw() of the voxel holding the sample point (px, py, pz) whose z
+   // floor `fz` is inside [0, nz) when `live`, and returns whether the point is
+   // inside the volume; outside x / y it is background (simulator.cpp:430-432)
+   // and `r` holds voxel (0, 0, 0): the caller takes label 0 for it.  The load
+   // is issued in every case and nothing is selected on its result here: a
+   // load under a branch, or a value selected from it before the loop's next
+   // turn, makes the compiler wait for every load in flight, which serialises
+   // the loads the march keeps in flight.
+   __device__ __forceinline__ bool fetch_raw(double px, double py, double fz, bool live,
+                                             uint4 &r) const;
+   __device__ static __forceinline__ int32_t label_of(const uint4 &r) {
+      return ST == ST_F32 ? (int32_t)r.x : (int32_t)(r.x & (ST == ST_F16 ? 0xffffu : 0x7fffu));
+   }
+   // the precision storages' "not normalized" bit of the source axis
+   __device__ static __forceinline__ bool nn_of(const uint4 &r) {
+      return ST == ST_F32 ? false
+                          : (ST == ST_F16 ? (r.y & 0x80000000u) != 0 : (r.x & 0x8000u) != 0);
+   }
+   // The optical axis.  ST_F16 with `renorm`: an axis that passed the
+   // reference's normalisation test is scaled back to unit length after the
+   // fp16 rounding.
+   __device__ static __forceinline__ float3 axis_of(const uint4 &r, bool renorm) {
+      if (ST == ST_F32) return make_float3(__uint_as_float(r.y), __uint_as_float(r.z),
+                                           __uint_as_float(r.w));
+      if (ST == ST_F16) {
+         // z is stored >= 0; its sign bit is the "not normalized" bit
+         float3 v = make_float3(__half2float(__ushort_as_half((unsigned short)(r.x >> 16))),
+                                __half2float(__ushort_as_half((unsigned short)(r.y & 0xffffu))),
+                                fabsf(__half2float(__ushort_as_half((unsigned short)(r.y >> 16)))));
+         if (renorm && !nn_of(r) && !(v.x == 0.0f && v.y == 0.0f && v.z == 0.0f)) {
+            const float inv = rsqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+            v = make_float3(v.x * inv, v.y * inv, v.z * inv);
+         }
+         return v;
+      }
+      const uint32_t tc = r.y & 0xffffu, pc = r.y >> 16;
+      if (tc == 0xffffu) return make_float3(0.0f, 0.0f, 0.0f);
+      const float th = (float)(tc * ANG_THETA), ph = (float)(pc * ANG_PHI) - 3.14159265f;
+      const float st = __sinf(th);
+      return make_float3(st * __cosf(ph), st * __sinf(ph), __cosf(th));
+   }
+   // decoded record of an in-range voxel
+   __device__ __forceinline__ Vox at(int32_t x, int32_t y, int32_t z) const {
+      const uint4 r = raw(x, y, z);
+      return Vox{label_of(r), axis_of(r, true), nn_of(r)};
    }
 };
 
@@ -167,6 +196,15 @@ __device__ __forceinline__ Vox Field<Idx, ST>::fetch(double px, double py, doubl
    Vox r{0, make_float3(0.0f, 0.0f, 0.0f), false};
    if (in) r = at(jx, jy, (int32_t)fz);
    return r;
+}
+
+template <typename Idx, int ST>
+__device__ __forceinline__ bool Field<Idx, ST>::fetch_raw(double px, double py, double fz,
+                                                         bool live, uint4 &r) const {
+   const int32_t jx = ifloor(px), jy = ifloor(py);
+   const bool in = live & ((uint32_t)jx < (uint32_t)nx) & ((uint32_t)jy < (uint32_t)ny);
+   r = raw(in ? jx : 0, in ? jy : 0, in ? (int32_t)fz : (flip_z ? nz - 1 : 0));
+   return in;
 }
 
 __device__ __forceinline__ bool is_zero(const float3 &v) {
@@ -234,9 +272,15 @@ __device__ __forceinline__ void slerp_weights(float t, float z, float &wa, float
 }
 
 // sin x and 1 - cos x for the per-voxel retardance; odd / even Taylor
-// polynomials for |x| <= 0.25 (truncation < 1e-11), sincosf beyond
+// polynomials: two terms when every lane of the wave has |x| <= 2^-6
+// (truncation below 5e-10 relative; per-voxel retardances of thin voxels are
+// ~1e-2), four for |x| <= 0.25 (truncation < 1e-11), sincosf beyond
 __device__ __forceinline__ void sin_omc(float x, float &s, float &omc) {
-   if (fabsf(x) <= 0.25f) {
+   if (__ballot(fabsf(x) > 0.015625f) == 0) {
+      const float x2 = x * x;
+      s = x * fmaf(x2, -1.0f / 6.0f, 1.0f);
+      omc = x2 * fmaf(x2, -1.0f / 24.0f, 0.5f);
+   } else if (fabsf(x) <= 0.25f) {
       const float x2 = x * x;
       s = x * fmaf(x2, fmaf(x2, fmaf(x2, -1.0f / 5040.0f, 1.0f / 120.0f), -1.0f / 6.0f), 1.0f);
       omc = x2 * fmaf(x2, fmaf(x2, fmaf(x2, -1.0f / 40320.0f, 1.0f / 720.0f), -1.0f / 24.0f), 0.5f);
@@ -280,9 +324,13 @@ __device__ __forceinline__ int32_t clampi(int32_t v, int32_t n) {
 // interpolations return their first argument exactly.
 // With OWN false the record of the ray's voxel is not at hand: when all corners
 // coincide it is read like a corner (it carries the label `lab`).
+// `unit` (optional) is set when the result is a unit axis by construction: a
+// slerp of unit axes in a 16-bit storage (the retarder then needs no length).
 template <typename Idx, int ST, bool OWN = true>
 __device__ float3 get_vec_interp(const Field<Idx, ST> &F, double px, double py, double pz,
-                                 int32_t lab, const float3 &own, int slerp) {
+                                 int32_t lab, const float3 &own, int slerp,
+                                 bool *unit = nullptr) {
+   if (unit) *unit = false;
    const double fx = px - 0.5, fy = py - 0.5, fz = pz - 0.5;
    const double flx = floor(fx), fly = floor(fy), flz = floor(fz);
    // ceil(f) = floor(f) + (f != floor(f)); both clamped to the volume
@@ -294,10 +342,19 @@ __device__ float3 get_vec_interp(const Field<Idx, ST> &F, double px, double py, 
    const float xd = dx ? (float)((px - (double)x0) - 0.5) : 0.0f;
    const float yd = dy ? (float)((py - (double)y0) - 0.5) : 0.0f;
    const float zd = dz ? (float)((pz - (double)z0) - 0.5) : 0.0f;
-   // corners with another label are zeroed (simulator.cpp:533-540)
-   auto corner = [&](const Vox &r) {
-      if (r.lab != lab) return make_float3(0.0f, 0.0f, 0.0f);
-      return r.v;
+   // corners with another label are zeroed (simulator.cpp:533-540).  ST_F16
+   // under Slerp: the corners are used as decoded, and unless a corner used
+   // carries the "not normalized" bit the result counts as a unit axis (a
+   // slerp of unit axes is one; only its direction is used); under Lerp each
+   // corner is scaled to unit length (the chord's length enters the
+   // retardance).  ST_ANGLE16 decodes unit axes.
+   using Fd = Field<Idx, ST>;
+   const bool late = ST == ST_F16 && slerp;
+   bool keep = false;
+   auto corner = [&](const uint4 &r) {
+      if (Fd::label_of(r) != lab) return make_float3(0.0f, 0.0f, 0.0f);
+      if (late) keep |= Fd::nn_of(r);
+      return Fd::axis_of(r, !late);
    };
    auto ip = [&](const float3 &a, const float3 &b, float t) {
       return slerp ? slerp_orient(a, b, t) : lerp_orient(a, b, t);
@@ -309,11 +366,11 @@ __device__ float3 get_vec_interp(const Field<Idx, ST> &F, double px, double py, 
    const bool xdup = __ballot(dx) == 0, ydup = __ballot(dy) == 0;
    float3 c00, c01, c10, c11;
    {
-      const Vox r000 = F.at(x0, y0, z0), r001 = F.at(x0, y0, z1);
-      Vox r100 = r000, r101 = r001;
+      const uint4 r000 = F.raw(x0, y0, z0), r001 = F.raw(x0, y0, z1);
+      uint4 r100 = r000, r101 = r001;
       if (!xdup) {
-         r100 = F.at(x1, y0, z0);
-         r101 = F.at(x1, y0, z1);
+         r100 = F.raw(x1, y0, z0);
+         r101 = F.raw(x1, y0, z1);
       }
       c00 = corner(r000);
       if (dx) c00 = ip(c00, corner(r100), xd);
@@ -326,11 +383,11 @@ __device__ float3 get_vec_interp(const Field<Idx, ST> &F, double px, double py, 
    c10 = c00;
    c11 = c00;
    if (!ydup) {
-      const Vox r010 = F.at(x0, y1, z0), r011 = F.at(x0, y1, z1);
-      Vox r110 = r010, r111 = r011;
+      const uint4 r010 = F.raw(x0, y1, z0), r011 = F.raw(x0, y1, z1);
+      uint4 r110 = r010, r111 = r011;
       if (!xdup) {
-         r110 = F.at(x1, y1, z0);
-         r111 = F.at(x1, y1, z1);
+         r110 = F.raw(x1, y1, z0);
+         r111 = F.raw(x1, y1, z1);
       }
       if (dy) {
          c10 = corner(r010);
@@ -348,6 +405,7 @@ __device__ float3 get_vec_interp(const Field<Idx, ST> &F, double px, double py, 
       if (dz) c1 = ip(c01, c11, yd);
    }
    // level z
+   if (unit) *unit = (late && !keep) || (ST == ST_ANGLE16 && slerp);
    if (dz) return ip(c0, c1, zd);
    return c0;
 }
@@ -440,8 +498,10 @@ struct AxisRot {
    bool on; // theta != 0
 };
 
+// `unit`: v is a unit axis by construction (16-bit storages), so |v| = 1 in
+// d_rel (simulator.cpp:309); every other use of v is invariant to its length.
 template <int ST>
-__device__ __forceinline__ bool retarder(const AxisRot &T, float K, float3 v, bool nn,
+__device__ __forceinline__ bool retarder(const AxisRot &T, float K, float3 v, bool nn, bool unit,
                                          uint32_t &warn, float &C, float &S, float &sd,
                                          float &omc) {
    C = 1.0f;
@@ -452,9 +512,8 @@ __device__ __forceinline__ bool retarder(const AxisRot &T, float K, float3 v, bo
       warn |= FPL_WARN_ZERO_VEC;
       return false;
    }
-   const float n2 = dot3(v, v);
-   if (ST == ST_F32 ? (n2 - 1.0f > 1e-5f) : nn) warn |= FPL_WARN_NOT_NORM;
-   const float dr = K * sqrtf(n2);
+   if (ST == ST_F32 ? (dot3(v, v) - 1.0f > 1e-5f) : nn) warn |= FPL_WARN_NOT_NORM;
+   const float dr = (ST != ST_F32 && unit) ? K : K * sqrtf(dot3(v, v));
    if (T.on) {
       const float3 w = v;
       v.x = T.R[0] * w.x + T.R[1] * w.y + T.R[2] * w.z;
@@ -508,6 +567,8 @@ __global__ void __launch_bounds__(WG, 6)
    __shared__ uint4 qb[WG / 64][QCAP];
    const Params &P = *prm;
    const int64_t n_props = P.n_props;
+   // a label outside [0, n_props) as one unsigned compare
+   const uint32_t n_labels = (uint32_t)(n_props < 0x7fffffff ? n_props : 0x7fffffff);
    if (PROPS_LDS) {
       for (int k = threadIdx.x; k < (int)n_props; k += WG) sp[k] = props[k];
       __syncthreads();
@@ -568,21 +629,18 @@ __global__ void __launch_bounds__(WG, 6)
    uint32_t warn = 0, err = 0;
    if (!interp) {
       // voxel axes used as they are (nearest neighbour, or every sample point
-      // a voxel centre): a step is cheap, so every ray runs its own steps and
-      // the record of step k + 1 is requested before step k is processed
-      double fz = floor(pz);
-      bool live = valid & (fz >= 0.0) & (fz < nzd);
-      Vox r = F.fetch(px, py, fz, live);
-      while (live) {
-         px = px + sx;
-         py = py + sy;
-         pz = pz + sz;
-         fz = floor(pz);
-         const bool qlive = (fz >= 0.0) & (fz < nzd);
-         const Vox rn = F.fetch(px, py, fz, qlive);
-
-         int32_t lab = r.lab;
-         if (lab < 0 || lab >= n_props) {
+      // a voxel centre): a step is cheap, so every ray runs its own steps, with
+      // the records of its next three steps in flight while it processes one (one
+      // load in flight leaves the loop waiting on memory latency).  Every ray
+      // takes the same number of steps, T.trips: the reference's loop test
+      // (simulator.cpp:259-261) looks at z alone, and every ray starts at
+      // z = step_z / 2.  Four slots hold steps k .. k + 3; a slot is processed,
+      // then refilled with the step four ahead in the same registers (refilling
+      // first would need a register copy, which waits for the load).
+      const int64_t trips = T.trips;
+      auto process = [&](const uint4 &r, bool in) {
+         int32_t lab = in ? F.label_of(r) : 0;
+         if ((uint32_t)lab >= n_labels) {
             err = 1;
             lab = 0;
          }
@@ -591,13 +649,64 @@ __global__ void __launch_bounds__(WG, 6)
             if (pd.mu_t != 0.0f) two_sum(mh, ml, pd.mu_t);
          } else {
             float C, S, sd, omc;
-            if (retarder<ST>(rot, pd.K, r.v, r.nn, warn, C, S, sd, omc)) {
+            // 16-bit storages: an axis that passed the normalisation test counts
+            // as unit length (no rescaling after decoding)
+            const bool nn = F.nn_of(r);
+            if (retarder<ST>(rot, pd.K, F.axis_of(r, false), nn, ST == ST_ANGLE16 || !nn, warn,
+                             C, S, sd, omc)) {
                apply_retarder(C, S, sd, omc, h1, h2, h3, l1, l2, l3);
                if (pd.mu_t != 0.0f) two_sum(mh, ml, pd.mu_t);
             }
          }
-         live = qlive;
-         r = rn;
+      };
+      auto run = [&](auto refill) {
+         uint4 r0, r1, r2, r3;
+         bool n0, n1, n2, n3;
+         refill(r0, n0);
+         refill(r1, n1);
+         refill(r2, n2);
+         refill(r3, n3);
+         // rays past the grid's end run along (label 0, nothing written)
+         for (int64_t k = 0; k < trips; k += 4) {
+            process(r0, n0);
+            refill(r0, n0);
+            if (k + 1 >= trips) break;
+            process(r1, n1);
+            refill(r1, n1);
+            if (k + 2 >= trips) break;
+            process(r2, n2);
+            refill(r2, n2);
+            if (k + 3 >= trips) break;
+            process(r3, n3);
+            refill(r3, n3);
+         }
+      };
+      int64_t kf = 0; // the next step to fetch
+      if (sx == 0.0 && sy == 0.0) {
+         // the light runs along z (untilted views): the ray stays in one voxel
+         // column, x and y never change, and step k's record is the column's
+         // base plus floor(z_k) rows of ny records
+         const int32_t jx = ifloor(px), jy = ifloor(py);
+         const bool in = valid & ((uint32_t)jx < (uint32_t)P.nx) & ((uint32_t)jy < (uint32_t)P.ny);
+         const Idx nyi = (Idx)P.ny;
+         const Idx base = in ? ((Idx)jx * (Idx)P.nz) * nyi + (Idx)jy : (Idx)0;
+         run([&](uint4 &r, bool &rin) {
+            int32_t z = (int32_t)floor(pz);
+            rin = in & (kf < trips);
+            z = rin ? z : 0;
+            if (P.flip_z) z = (int32_t)P.nz - 1 - z;
+            r = F.load(base + (Idx)z * nyi);
+            pz = pz + sz;
+            kf++;
+         });
+      } else {
+         run([&](uint4 &r, bool &rin) {
+            rin = F.fetch_raw(px, py, floor(pz), valid & (kf < trips), r);
+            px = px + sx;
+            py = py + sy;
+            pz = pz + sz;
+            kf++;
+         });
       }
    } else {
       // interpolating views: the wave-cooperative queue described above
@@ -609,10 +718,11 @@ __global__ void __launch_bounds__(WG, 6)
       // condition is the reference's (simulator.cpp:259-261)
       const double fz0 = floor(pz);
       bool live = valid & (fz0 >= 0.0) & (fz0 < nzd);
-      Vox r = F.fetch(px, py, fz0, live);
+      uint4 r;
+      bool rin = F.fetch_raw(px, py, fz0, live, r);
       for (;;) {
-         int32_t lab = r.lab;
-         if (lab < 0 || lab >= n_props) {
+         int32_t lab = rin ? F.label_of(r) : 0;
+         if ((uint32_t)lab >= n_labels) {
             err = 1;
             lab = 0;
          }
@@ -632,7 +742,7 @@ __global__ void __launch_bounds__(WG, 6)
                   QA[slot] = make_uint4(__double2loint(px), __double2hiint(px), __double2loint(py),
                                         __double2hiint(py));
                   QB[slot] = make_uint4(__double2loint(pz), __double2hiint(pz), (uint32_t)lab,
-                                        r.nn ? 1u : 0u);
+                                        F.nn_of(r) ? 1u : 0u);
                   pend |= (seq & 255u) << (8 * npend);
                   npend++;
                } else if (pd.mu_t != 0.0f) {
@@ -643,7 +753,7 @@ __global__ void __launch_bounds__(WG, 6)
                pz = pz + sz;
                const double fz = floor(pz);
                live = (fz >= 0.0) & (fz < nzd);
-               r = F.fetch(px, py, fz, live);
+               rin = F.fetch_raw(px, py, fz, live, r);
             }
             tail += (uint32_t)__popcll(qmask);
          }
@@ -656,11 +766,13 @@ __global__ void __launch_bounds__(WG, 6)
                const uint4 a = QA[slot], b = QB[slot];
                const int32_t elab = (int32_t)b.z;
                const PropDev epd = prop(elab);
+               bool unit;
                const float3 v = get_vec_interp<Idx, ST, false>(
                    F, pack_double(a.x, a.y), pack_double(a.z, a.w), pack_double(b.x, b.y), elab,
-                   make_float3(0.0f, 0.0f, 0.0f), slerp);
+                   make_float3(0.0f, 0.0f, 0.0f), slerp, &unit);
                float C, S, sd, omc, mu = 0.0f;
-               if (retarder<ST>(rot, epd.K, v, b.w != 0, warn, C, S, sd, omc)) mu = epd.mu_t;
+               if (retarder<ST>(rot, epd.K, v, b.w != 0, unit, warn, C, S, sd, omc))
+                  mu = epd.mu_t;
                QA[slot] = make_uint4(__float_as_uint(C), __float_as_uint(S), __float_as_uint(sd),
                                      __float_as_uint(omc));
                QB[slot].x = __float_as_uint(mu);
@@ -1907,6 +2019,7 @@ Status prepare(const fpl_setup *setup, const int64_t dim[3], const double *props
          return make_status(FPL_E_INVALID,
                             "tilt theta = %f needs %.3g z-steps per ray, more than %lld",
                             tilts[2 * t], trips, (long long)MAX_RAY_STEPS);
+      tdev[t].trips = (int32_t)tilt_steps(tdev[t], dim[2]);
    }
    // tilts that interpolate (long rays of work) first, so that the short
    // nearest-neighbour workgroups fill the end of the launch
