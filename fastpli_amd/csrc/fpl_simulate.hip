@@ -714,12 +714,44 @@ __global__ void __launch_bounds__(WG, 6)
       int npend = 0;
       uint32_t tail = 0, head = 0; // wave: steps queued / evaluated so far
 
-      // the record of the current step is loaded one iteration ahead; the loop
-      // condition is the reference's (simulator.cpp:259-261)
-      const double fz0 = floor(pz);
-      bool live = valid & (fz0 >= 0.0) & (fz0 < nzd);
+      // The record of the current step is loaded one iteration ahead.  A ray
+      // takes T.trips steps (the reference's loop test, simulator.cpp:259-261,
+      // looks at z alone).  A coordinate the light does not move in any lane
+      // (a view tilted along the other axis: adding the step leaves it as it
+      // is, so it stays so) is neither stepped nor floored again.
+      const int32_t trips = T.trips;
+      const bool xfix = __ballot(px + sx != px) == 0, yfix = __ballot(py + sy != py) == 0;
+      int32_t jx = ifloor(px), jy = ifloor(py); // the current step's voxel column
+      int32_t ks = 0;                           // steps taken
+      bool live = valid & (trips > 0);
+      auto fetch = [&](uint4 &rec) -> bool {
+         const bool in = live & ((uint32_t)jx < (uint32_t)F.nx) & ((uint32_t)jy < (uint32_t)F.ny);
+         int32_t jz = (int32_t)floor(pz);
+         if (F.flip_z) jz = F.nz - 1 - jz;
+         // the load is issued in every case (record 0 for a point outside)
+         const Idx i = ((Idx)jx * (Idx)F.nz + (Idx)jz) * (Idx)F.ny + (Idx)jy;
+         rec = F.load(in ? i : (Idx)0);
+         return in;
+      };
+      // the next step's position; the empty asm statements keep the
+      // wave-uniform branches (the compiler would otherwise compute both sides)
+      auto advance = [&]() {
+         if (!xfix) {
+            __asm__ volatile("");
+            px = px + sx;
+            jx = ifloor(px);
+         }
+         if (!yfix) {
+            __asm__ volatile("");
+            py = py + sy;
+            jy = ifloor(py);
+         }
+         pz = pz + sz;
+         ks++;
+         live = ks < trips;
+      };
       uint4 r;
-      bool rin = F.fetch_raw(px, py, fz0, live, r);
+      bool rin = fetch(r);
       for (;;) {
          int32_t lab = rin ? F.label_of(r) : 0;
          if ((uint32_t)lab >= n_labels) {
@@ -748,12 +780,8 @@ __global__ void __launch_bounds__(WG, 6)
                } else if (pd.mu_t != 0.0f) {
                   two_sum(mh, ml, pd.mu_t);
                }
-               px = px + sx;
-               py = py + sy;
-               pz = pz + sz;
-               const double fz = floor(pz);
-               live = (fz >= 0.0) & (fz < nzd);
-               rin = F.fetch_raw(px, py, fz, live, r);
+               advance();
+               rin = fetch(r);
             }
             tail += (uint32_t)__popcll(qmask);
          }
