@@ -385,15 +385,71 @@ __device__ __forceinline__ TileBox tile_box(const SegRec &R, int xt, int yt, int
    return b;
 }
 
+// Per-tile counting of a workgroup's (tile, segment) pairs in LDS: the
+// segments of a workgroup are consecutive pieces of a few fibers and share
+// most of their tiles, so one global atomic per distinct tile replaces one
+// per pair (same-address global atomics serialise in L2).  Open addressing
+// with a bounded probe: a tile that finds no slot takes the global atomic
+// itself.  A tile is either in the table or finds no slot for every pair of
+// the workgroup (slots are never freed).
+constexpr int HASH_N = 2048;           // slots of a workgroup's table
+constexpr uint32_t HASH_EMPTY = 0xffffffffu;
+constexpr int HASH_PROBES = 32;
+
+struct TileHash {
+   uint32_t key[HASH_N];
+   uint32_t cnt[HASH_N];
+};
+
+__device__ __forceinline__ uint32_t hash_start(uint32_t t) {
+   return (t * 2654435761u) >> (32 - 11); // 11 bits: HASH_N slots
+}
+
+// slot of tile t (inserted if absent), or -1 when its probe run is full
+__device__ __forceinline__ int hash_insert(TileHash &H, uint32_t t) {
+   uint32_t h = hash_start(t);
+   for (int p = 0; p < HASH_PROBES; p++) {
+      const uint32_t prev = atomicCAS(&H.key[h], HASH_EMPTY, t);
+      if (prev == HASH_EMPTY || prev == t) return (int)h;
+      h = (h + 1) & (HASH_N - 1);
+   }
+   return -1;
+}
+
+// slot of tile t after every insertion of the workgroup, or -1
+__device__ __forceinline__ int hash_find(const TileHash &H, uint32_t t) {
+   uint32_t h = hash_start(t);
+   for (int p = 0; p < HASH_PROBES; p++) {
+      const uint32_t k = H.key[h];
+      if (k == t) return (int)h;
+      if (k == HASH_EMPTY) return -1;
+      h = (h + 1) & (HASH_N - 1);
+   }
+   return -1;
+}
+
+__device__ __forceinline__ void hash_clear(TileHash &H) {
+   for (int k = threadIdx.x; k < HASH_N; k += blockDim.x) {
+      H.key[k] = HASH_EMPTY;
+      H.cnt[k] = 0;
+   }
+}
+
 __global__ void k_segments(Params pr, const double *__restrict__ P,
                            const int64_t *__restrict__ fiber_ptr,
                            const BundleDev *__restrict__ bd, const uint8_t *__restrict__ fiber_ok,
-                           SegRec *segs, float4 *seg_dir, float4 *seg_clip, int64_t *seg_count) {
+                           SegRec *segs, float4 *seg_dir, float4 *seg_clip, int64_t *seg_count,
+                           uint32_t *tile_count) {
+   __shared__ TileHash H;
+   if (tile_count) {
+      hash_clear(H);
+      __syncthreads();
+   }
    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (i >= pr.n_points) return;
-   const int64_t f = find_fiber(fiber_ptr, pr.n_fibers, i);
+   const bool seg = i < pr.n_points;
+   const int64_t f = seg ? find_fiber(fiber_ptr, pr.n_fibers, i) : 0;
    int64_t count = 0;
-   if (fiber_ok[f] && i + 1 < fiber_ptr[f + 1]) {
+   if (seg && fiber_ok[f] && i + 1 < fiber_ptr[f + 1]) {
       const int b = find_bundle(bd, pr.n_bundles, f);
       SegRec R;
       double rp, rq;
@@ -456,13 +512,27 @@ __global__ void k_segments(Params pr, const double *__restrict__ P,
                   if (b.za < b.zb) {
                      count++;
                      if (k < 31) mask |= 1u << k;
+                     // the binning by atomics counts the tile's segments
+                     if (tile_count) {
+                        const uint32_t t = (uint32_t)((tx * pr.nty + ty) * pr.ntz + tz);
+                        const int h = hash_insert(H, t);
+                        if (h >= 0)
+                           atomicAdd(&H.cnt[h], 1u);
+                        else
+                           atomicAdd(&tile_count[t], 1u);
+                     }
                   }
                }
          if (k > 31) mask = 0x80000000u;
          seg_dir[i].w = __uint_as_float(mask);
       }
    }
-   seg_count[i] = count;
+   if (seg) seg_count[i] = count;
+   if (tile_count) {
+      __syncthreads();
+      for (int k = threadIdx.x; k < HASH_N; k += blockDim.x)
+         if (H.key[k] != HASH_EMPTY) atomicAdd(&tile_count[H.key[k]], H.cnt[k]);
+   }
 }
 
 __global__ void k_emit(Params pr, const SegRec *__restrict__ segs,
@@ -538,6 +608,167 @@ __global__ void k_ranges(const uint32_t *__restrict__ keys, const uint32_t *__re
       range[2 * (int64_t)t + 1] = (uint32_t)(p + 1);
       tile_bundle[t].y = segs[vals[p]].bundle;
    }
+}
+
+// ---- binning by atomics: tile lists of at most SORT_MAX segments.
+// k_segments counts each tile's segments, an exclusive scan gives every tile
+// its list offset, k_emit_atomic places the (tile, segment) pairs by atomic
+// cursors (in no particular order) and k_tile_sort puts every list in global
+// segment order, one wave per tile.  A longer list makes the host bin the
+// volume again with the radix sort above.
+constexpr uint32_t SORT_MAX = 256;
+
+// Pairs of segment i into their tiles' lists: position tile_off[t] + (the
+// count left in tile_count[t] after taking them; the pairs of a workgroup
+// reserve their places per tile through the LDS table); tile_count ends at
+// zero.
+// The pair count into flags[4..5] for the host's next call; flags[2] bit 0
+// when it exceeds the capacity the host assumed (the pairs beyond it are
+// dropped and the host bins again with the exact count).
+__global__ void k_emit_atomic(Params pr, const SegRec *__restrict__ segs,
+                              const float4 *__restrict__ seg_dir,
+                              const int64_t *__restrict__ seg_count, uint32_t *tile_count,
+                              const uint32_t *__restrict__ tile_off, uint32_t n_tiles, int64_t cap,
+                              uint32_t *list, uint32_t *flags) {
+   __shared__ TileHash H;
+   __shared__ uint32_t base[HASH_N];
+   hash_clear(H);
+   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i == 0) {
+      const uint32_t n = tile_off[n_tiles];
+      flags[4] = n;
+      flags[5] = 0;
+      if ((int64_t)n > cap) atomicOr(&flags[2], 1u);
+   }
+   __syncthreads();
+   const bool seg = i < pr.n_points && seg_count[i] != 0;
+   // the kept tiles of segment i in k_segments' order, f(tile index)
+   auto for_tiles = [&](auto fn) {
+      const SegRec &R = segs[i];
+      const int tx0 = R.lo[0] / TX, tx1 = (R.hi[0] - 1) / TX;
+      const int ty0 = R.lo[1] / TY, ty1 = (R.hi[1] - 1) / TY;
+      const int tz0 = R.lo[2] / TZ, tz1 = (R.hi[2] - 1) / TZ;
+      const uint32_t mask = __float_as_uint(seg_dir[i].w); // k_segments' tile mask
+      int k = 0;
+      for (int tx = tx0; tx <= tx1; tx++)
+         for (int ty = ty0; ty <= ty1; ty++)
+            for (int tz = tz0; tz <= tz1; tz++, k++) {
+               bool keep = (mask >> k) & 1u;
+               if (mask & 0x80000000u) { // as k_segments counts
+                  const TileBox b = tile_box(R, tx * TX, ty * TY, tz * TZ);
+                  keep = b.za < b.zb;
+               }
+               if (keep) fn((uint32_t)((tx * pr.nty + ty) * pr.ntz + tz));
+            }
+   };
+   // the workgroup's pairs per tile; then one reservation per tile (the count
+   // left in tile_count after taking them); then every pair its place
+   if (seg)
+      for_tiles([&](uint32_t t) {
+         const int h = hash_insert(H, t);
+         if (h >= 0) atomicAdd(&H.cnt[h], 1u);
+      });
+   __syncthreads();
+   for (int k = threadIdx.x; k < HASH_N; k += blockDim.x) {
+      if (H.key[k] != HASH_EMPTY) {
+         base[k] = atomicSub(&tile_count[H.key[k]], H.cnt[k]) - H.cnt[k];
+         H.cnt[k] = 0;
+      }
+   }
+   __syncthreads();
+   if (seg)
+      for_tiles([&](uint32_t t) {
+         const int h = hash_find(H, t);
+         const uint32_t r = h >= 0 ? base[h] + atomicAdd(&H.cnt[h], 1u)
+                                   : atomicSub(&tile_count[t], 1u) - 1u;
+         const int64_t o = (int64_t)tile_off[t] + r;
+         if (o < cap) list[o] = (uint32_t)i;
+      });
+}
+
+// Bitonic sort of the n <= 64 E values at p (ascending), element e * 64 + lane
+// in register e of the lane; the padding sorts last.
+template <int E>
+__device__ __forceinline__ void wave_sort(uint32_t *p, uint32_t n, int lane, uint32_t &first,
+                                          uint32_t &last) {
+   uint32_t v[E];
+#pragma unroll
+   for (int e = 0; e < E; e++) {
+      const uint32_t idx = (uint32_t)(e * 64 + lane);
+      v[e] = idx < n ? p[idx] : 0xffffffffu;
+   }
+#pragma unroll
+   for (int k = 2; k <= 64 * E; k <<= 1) {
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+         if (j >= 64) { // the partner is another register of this lane
+            const int je = j >> 6;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+               if (e & je) continue;
+               const bool up = (((e * 64 + lane) & k) == 0);
+               const uint32_t lo = min(v[e], v[e | je]), hi = max(v[e], v[e | je]);
+               v[e] = up ? lo : hi;
+               v[e | je] = up ? hi : lo;
+            }
+         } else {
+            const bool lower = (lane & j) == 0;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+               const uint32_t o = (uint32_t)__shfl_xor((int)v[e], j);
+               const bool up = (((e * 64 + lane) & k) == 0);
+               v[e] = (up == lower) ? min(v[e], o) : max(v[e], o);
+            }
+         }
+      }
+   }
+#pragma unroll
+   for (int e = 0; e < E; e++) {
+      const uint32_t idx = (uint32_t)(e * 64 + lane);
+      if (idx < n) p[idx] = v[e];
+   }
+   const uint32_t el = (n - 1) >> 6;
+   uint32_t w = v[0];
+#pragma unroll
+   for (int e = 1; e < E; e++)
+      if (el == (uint32_t)e) w = v[e];
+   first = (uint32_t)__shfl((int)v[0], 0);
+   last = (uint32_t)__shfl((int)w, (int)((n - 1) & 63));
+}
+
+// One wave per tile: its list in global segment order, [begin, end) (clamped
+// to the capacity: after an overflow the voxelizer still reads in bounds) and
+// the bundles of its first and last segment.  A list longer than SORT_MAX
+// sets flags[2] bit 1.
+__global__ void __launch_bounds__(256) k_tile_sort(uint32_t n_tiles,
+                                                   const uint32_t *__restrict__ tile_off,
+                                                   int64_t cap, uint32_t *list,
+                                                   const SegRec *__restrict__ segs,
+                                                   uint32_t *range, int2 *tile_bundle,
+                                                   uint32_t *flags) {
+   const uint32_t t = blockIdx.x * 4u + (threadIdx.x >> 6);
+   const int lane = threadIdx.x & 63;
+   if (t >= n_tiles) return;
+   const int64_t b = tile_off[t], e = tile_off[t + 1];
+   const uint32_t cb = (uint32_t)min(b, cap), ce = (uint32_t)min(e, cap);
+   if (lane == 0) {
+      range[2 * (int64_t)t] = cb;
+      range[2 * (int64_t)t + 1] = ce;
+   }
+   const uint32_t n = ce - cb;
+   if (n == 0 || e > cap) return; // empty, or the host bins again
+   if (n > SORT_MAX) {
+      if (lane == 0) atomicOr(&flags[2], 2u);
+      return;
+   }
+   uint32_t first, last;
+   if (n <= 64)
+      wave_sort<1>(list + cb, n, lane, first, last);
+   else if (n <= 128)
+      wave_sort<2>(list + cb, n, lane, first, last);
+   else
+      wave_sort<4>(list + cb, n, lane, first, last);
+   if (lane == 0) tile_bundle[t] = make_int2(segs[first].bundle, segs[last].bundle);
 }
 
 struct SegExact { // what the FP64 distance test reads of a segment
@@ -1394,10 +1625,18 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
 // in the middle of the call (a host round trip that leaves the GPU idle);
 // `overflow` reports a count above it (the outputs are then incomplete and
 // the caller runs again without the hint).
+// A scene hint with this bit: its tile lists are too long for the per-tile sort
+constexpr int64_t LONG_LISTS = (int64_t)1 << 62;
+
+// One generation.  `overflow`: the pairs exceeded the capacity this call
+// assumed (the previous call's count); `long_lists`: a tile list exceeded the
+// per-tile sort.  The caller runs again in either case (the outputs are
+// incomplete), with the hint updated.
 Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outputs &out,
                 double *props, uint32_t *warn, int32_t *label_max, hipStream_t st, bool use_hint,
-                bool &overflow) {
+                bool force_radix, bool &overflow, bool &long_lists) {
    overflow = false;
+   long_lists = false;
    if (!vol || !fbs) return make_status(FPL_E_INVALID, "null argument");
    const int64_t nx = vol->dim[0], ny = vol->dim[1], nz = vol->dim[2];
    if (nx <= 0 || ny <= 0 || nz <= 0)
@@ -1540,67 +1779,109 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
    }
    int64_t total = 0;
    // a hint above 64 tiles per point (a much larger previous scene) would make
-   // this call pad and sort mostly padding: it reads its count back instead
+   // this call pad and sort mostly padding: it reads its count back instead.
+   // Bit 62 of a hint: the scene's tile lists are too long for the per-tile
+   // sort (the radix sort bins it).
    const std::array<int64_t, 5> scene{fbs->n_points, fbs->n_fibers, nx, ny, nz};
-   int64_t hint = use_hint ? pair_hint(st, scene) : 0;
+   const int64_t raw_hint = pair_hint(st, scene);
+   const bool radix = force_radix || (raw_hint & LONG_LISTS) != 0;
+   int64_t hint = use_hint ? (raw_hint & ~LONG_LISTS) : 0;
    if (hint > 64 * fbs->n_points) hint = 0;
+   void *ka, *va, *kb, *vb, *rng;
+   // the per-tile counts / offsets of the binning by atomics (n_tiles + 1 each)
+   uint32_t *tile_count = nullptr, *tile_off = nullptr;
+   if (!radix) {
+      FPL_TRY(ws_get(st, WS_KEYS_A, sizeof(uint32_t) * (n_tiles + 1), &ka));
+      FPL_TRY(ws_get(st, WS_KEYS_B, sizeof(uint32_t) * (n_tiles + 1), &kb));
+      tile_count = (uint32_t *)ka;
+      tile_off = (uint32_t *)kb;
+      hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n_tiles + 1, B)), dim3(B), 0, st, tile_count,
+                         n_tiles + 1, 0u);
+   }
    if (np > 0) {
       hipLaunchKernelGGL(k_segments, dim3(grid_for(np, B)), dim3(B), 0, st, pr, fbs->points,
                          fbs->fiber_ptr, d_bd, (const uint8_t *)fok, (SegRec *)segs,
-                         seg_dir, seg_clip, (int64_t *)cnt);
-      // exclusive scan over np + 1 entries (the last one is 0) gives the total
-      FPL_HIP(hipMemsetAsync((int64_t *)cnt + np, 0, sizeof(int64_t), st));
+                         seg_dir, seg_clip, (int64_t *)cnt, tile_count);
+      // exclusive scan: per tile (its list offsets; the last entry, 0, gives
+      // the total) or per segment (its pair positions)
       size_t tmp_bytes = 0;
-      FPL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, (int64_t *)cnt, (int64_t *)off,
-                                               (int)(np + 1), st));
       void *tmp;
-      FPL_TRY(ws_get(st, WS_CUB_TEMP, tmp_bytes, &tmp));
-      FPL_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, (int64_t *)cnt, (int64_t *)off,
-                                               (int)(np + 1), st));
-      if (hint > 0) {
-         total = hint; // k_pad pads to it or reports more
+      if (!radix) {
+         FPL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, tile_count, tile_off,
+                                                  (int)(n_tiles + 1), st));
+         FPL_TRY(ws_get(st, WS_CUB_TEMP, tmp_bytes, &tmp));
+         FPL_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, tile_count, tile_off,
+                                                  (int)(n_tiles + 1), st));
       } else {
-         FPL_HIP(hipMemcpyAsync(stage, (int64_t *)off + np, sizeof(int64_t),
-                                hipMemcpyDeviceToHost, st));
+         FPL_HIP(hipMemsetAsync((int64_t *)cnt + np, 0, sizeof(int64_t), st));
+         FPL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, (int64_t *)cnt,
+                                                  (int64_t *)off, (int)(np + 1), st));
+         FPL_TRY(ws_get(st, WS_CUB_TEMP, tmp_bytes, &tmp));
+         FPL_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, (int64_t *)cnt, (int64_t *)off,
+                                                  (int)(np + 1), st));
+      }
+      if (hint > 0) {
+         total = hint; // k_pad / k_emit_atomic report a larger count
+      } else {
+         if (!radix)
+            FPL_HIP(hipMemcpyAsync(stage, tile_off + n_tiles, sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, st));
+         else
+            FPL_HIP(hipMemcpyAsync(stage, (int64_t *)off + np, sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, st));
          FPL_HIP(hipStreamSynchronize(st));
-         total = *(int64_t *)stage;
+         total = !radix ? (int64_t) * (uint32_t *)stage : *(int64_t *)stage;
       }
    }
    if (total >= ((int64_t)1 << 31))
       return make_status(FPL_E_INVALID, "too many segment/tile pairs: %lld", (long long)total);
 
-   void *ka, *va, *kb, *vb, *rng;
-   FPL_TRY(ws_get(st, WS_KEYS_A, sizeof(uint32_t) * (total + 1), &ka));
-   FPL_TRY(ws_get(st, WS_VALS_A, sizeof(uint32_t) * (total + 1), &va));
-   FPL_TRY(ws_get(st, WS_KEYS_B, sizeof(uint32_t) * (total + 1), &kb));
    FPL_TRY(ws_get(st, WS_VALS_B, sizeof(uint32_t) * (total + 1), &vb));
    FPL_TRY(ws_get(st, WS_TILE_RANGE, sizeof(uint32_t) * 4 * n_tiles, &rng));
    int2 *tile_bundle = (int2 *)((uint32_t *)rng + 2 * n_tiles); // set with the ranges
-   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(2 * n_tiles, B)), dim3(B), 0, st,
-                      (uint32_t *)rng, 2 * n_tiles, 0u);
-   if (np > 0)
-      hipLaunchKernelGGL(k_pad, dim3(grid_for(total > 0 ? total : 1, B)), dim3(B), 0, st,
-                         (const int64_t *)off + np, total, (uint32_t)n_tiles, (uint32_t *)ka,
-                         (uint32_t *)va, d_flags);
-   if (total > 0) {
-      hipLaunchKernelGGL(k_emit, dim3(grid_for(np, B)), dim3(B), 0, st, pr, (const SegRec *)segs,
-                         (const float4 *)seg_dir, (const int64_t *)cnt, (const int64_t *)off,
-                         total, (uint32_t *)ka, (uint32_t *)va);
-      // keys up to n_tiles (the padding key) take part in the sort
-      int nbits = 1;
-      while (((int64_t)1 << nbits) <= n_tiles) nbits++;
-      size_t tmp_bytes = 0;
-      FPL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (uint32_t *)ka,
-                                                 (uint32_t *)kb, (uint32_t *)va, (uint32_t *)vb,
-                                                 (int)total, 0, nbits, st));
-      void *tmp;
-      FPL_TRY(ws_get(st, WS_CUB_TEMP, tmp_bytes, &tmp));
-      FPL_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, (uint32_t *)ka, (uint32_t *)kb,
-                                                 (uint32_t *)va, (uint32_t *)vb, (int)total, 0,
-                                                 nbits, st));
-      hipLaunchKernelGGL(k_ranges, dim3(grid_for(total, B)), dim3(B), 0, st, (uint32_t *)kb,
-                         (const uint32_t *)vb, total, (uint32_t)n_tiles, (const SegRec *)segs,
-                         (uint32_t *)rng, tile_bundle);
+   if (!radix) {
+      if (np > 0) {
+         hipLaunchKernelGGL(k_emit_atomic, dim3(grid_for(np, B)), dim3(B), 0, st, pr,
+                            (const SegRec *)segs, (const float4 *)seg_dir, (const int64_t *)cnt,
+                            tile_count, (const uint32_t *)tile_off, (uint32_t)n_tiles, total,
+                            (uint32_t *)vb, d_flags);
+         hipLaunchKernelGGL(k_tile_sort, dim3((unsigned)((n_tiles + 3) / 4)), dim3(B), 0, st,
+                            (uint32_t)n_tiles, (const uint32_t *)tile_off, total, (uint32_t *)vb,
+                            (const SegRec *)segs, (uint32_t *)rng, tile_bundle, d_flags);
+      } else {
+         hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(2 * n_tiles, B)), dim3(B), 0, st,
+                            (uint32_t *)rng, 2 * n_tiles, 0u);
+      }
+   } else {
+      FPL_TRY(ws_get(st, WS_KEYS_A, sizeof(uint32_t) * (total + 1), &ka));
+      FPL_TRY(ws_get(st, WS_VALS_A, sizeof(uint32_t) * (total + 1), &va));
+      FPL_TRY(ws_get(st, WS_KEYS_B, sizeof(uint32_t) * (total + 1), &kb));
+      hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(2 * n_tiles, B)), dim3(B), 0, st,
+                         (uint32_t *)rng, 2 * n_tiles, 0u);
+      if (np > 0)
+         hipLaunchKernelGGL(k_pad, dim3(grid_for(total > 0 ? total : 1, B)), dim3(B), 0, st,
+                            (const int64_t *)off + np, total, (uint32_t)n_tiles, (uint32_t *)ka,
+                            (uint32_t *)va, d_flags);
+      if (total > 0) {
+         hipLaunchKernelGGL(k_emit, dim3(grid_for(np, B)), dim3(B), 0, st, pr,
+                            (const SegRec *)segs, (const float4 *)seg_dir, (const int64_t *)cnt,
+                            (const int64_t *)off, total, (uint32_t *)ka, (uint32_t *)va);
+         // keys up to n_tiles (the padding key) take part in the sort
+         int nbits = 1;
+         while (((int64_t)1 << nbits) <= n_tiles) nbits++;
+         size_t tmp_bytes = 0;
+         FPL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (uint32_t *)ka,
+                                                    (uint32_t *)kb, (uint32_t *)va,
+                                                    (uint32_t *)vb, (int)total, 0, nbits, st));
+         void *tmp;
+         FPL_TRY(ws_get(st, WS_CUB_TEMP, tmp_bytes, &tmp));
+         FPL_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, (uint32_t *)ka,
+                                                    (uint32_t *)kb, (uint32_t *)va,
+                                                    (uint32_t *)vb, (int)total, 0, nbits, st));
+         hipLaunchKernelGGL(k_ranges, dim3(grid_for(total, B)), dim3(B), 0, st, (uint32_t *)kb,
+                            (const uint32_t *)vb, total, (uint32_t)n_tiles,
+                            (const SegRec *)segs, (uint32_t *)rng, tile_bundle);
+      }
    }
    const bool lsq_lds = pr.n_lsq <= LSQ_LDS;
    const bool rec16 = out.rec_a && out.rec_storage != rec::ST_F32;
@@ -1637,11 +1918,13 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
       kt.collect();
       const uint32_t *f = (const uint32_t *)stage;
       if (np > 0) {
+         const int64_t pairs = (int64_t)((uint64_t)f[4] | (uint64_t)f[5] << 32);
+         long_lists = (f[2] & 2u) != 0;
+         set_pair_hint(st, scene, pairs | ((radix || long_lists) ? LONG_LISTS : 0));
          if (f[2]) {
-            overflow = true;
+            overflow = (f[2] & 1u) != 0;
             return Status();
          }
-         set_pair_hint(st, scene, (int64_t)((uint64_t)f[4] | (uint64_t)f[5] << 32));
       }
       rerun = (f[1] & 1u) != 0;
       if (!rerun && (f[3] & 4u) && !(f[0] & FPL_WARN_OVERLAP)) {
@@ -1671,11 +1954,13 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
       kt2.collect();
       const uint32_t *f = (const uint32_t *)stage;
       if (np > 0) {
+         const int64_t pairs = (int64_t)((uint64_t)f[4] | (uint64_t)f[5] << 32);
+         long_lists = (f[2] & 2u) != 0;
+         set_pair_hint(st, scene, pairs | ((radix || long_lists) ? LONG_LISTS : 0));
          if (f[2]) {
-            overflow = true;
+            overflow = (f[2] & 1u) != 0;
             return Status();
          }
-         set_pair_hint(st, scene, (int64_t)((uint64_t)f[4] | (uint64_t)f[5] << 32));
       }
    }
    if (warn) *warn = *(uint32_t *)stage;
@@ -1690,9 +1975,21 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
 
 Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outputs &out,
            double *props, uint32_t *warn, int32_t *label_max, hipStream_t st) {
-   bool overflow = false;
-   FPL_TRY(run_once(vol, fbs, out, props, warn, label_max, st, true, overflow));
-   if (overflow) FPL_TRY(run_once(vol, fbs, out, props, warn, label_max, st, false, overflow));
+   // the binning by atomics first (unless the scene's hint says its lists are
+   // too long); a call whose pairs exceeded the assumed capacity runs again
+   // with its exact count, one with a list too long for the per-tile sort
+   // with the radix sort (both leave the hint for the next call)
+   bool overflow = false, long_lists = false;
+   FPL_TRY(run_once(vol, fbs, out, props, warn, label_max, st, true, false, overflow, long_lists));
+   if (overflow)
+      FPL_TRY(run_once(vol, fbs, out, props, warn, label_max, st, false, false, overflow,
+                       long_lists));
+   if (long_lists)
+      FPL_TRY(run_once(vol, fbs, out, props, warn, label_max, st, true, true, overflow,
+                       long_lists));
+   if (overflow)
+      FPL_TRY(run_once(vol, fbs, out, props, warn, label_max, st, false, true, overflow,
+                       long_lists));
    return Status();
 }
 
