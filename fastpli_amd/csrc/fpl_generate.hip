@@ -1768,20 +1768,11 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
    uint32_t *d_flags = (uint32_t *)flags;
 
    const int B = 256;
-   if (nf > 0)
-      hipLaunchKernelGGL(k_fiber_boxes, dim3(grid_for(nf, B / 64)), dim3(B), 0, st, pr,
-                         fbs->points, fbs->fiber_ptr, fiber_box);
-   hipLaunchKernelGGL(k_bundle_boxes, dim3((unsigned)(nb > 0 ? nb : 1)), dim3(B), 0, st, pr,
-                      d_bd, fiber_box, bundle_box, d_flags);
-   if (nf > 0) {
-      hipLaunchKernelGGL(k_fibers, dim3(grid_for(nf, B)), dim3(B), 0, st, pr, fbs->fiber_ptr, d_bd,
-                         fiber_box, bundle_box, (uint8_t *)fok, d_flags);
-   }
-   int64_t total = 0;
    // a hint above 64 tiles per point (a much larger previous scene) would make
    // this call pad and sort mostly padding: it reads its count back instead.
    // Bit 62 of a hint: the scene's tile lists are too long for the per-tile
-   // sort (the radix sort bins it).
+   // sort (the radix sort bins it).  The workspace of the binning is taken
+   // before the first launch, so that the small kernels go out back to back.
    const std::array<int64_t, 5> scene{fbs->n_points, fbs->n_fibers, nx, ny, nz};
    const int64_t raw_hint = pair_hint(st, scene);
    const bool radix = force_radix || (raw_hint & LONG_LISTS) != 0;
@@ -1795,9 +1786,21 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
       FPL_TRY(ws_get(st, WS_KEYS_B, sizeof(uint32_t) * (n_tiles + 1), &kb));
       tile_count = (uint32_t *)ka;
       tile_off = (uint32_t *)kb;
+      if (hint > 0) FPL_TRY(ws_get(st, WS_VALS_B, sizeof(uint32_t) * (hint + 1), &vb));
       hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n_tiles + 1, B)), dim3(B), 0, st, tile_count,
                          n_tiles + 1, 0u);
    }
+   FPL_TRY(ws_get(st, WS_TILE_RANGE, sizeof(uint32_t) * 4 * n_tiles, &rng));
+   if (nf > 0)
+      hipLaunchKernelGGL(k_fiber_boxes, dim3(grid_for(nf, B / 64)), dim3(B), 0, st, pr,
+                         fbs->points, fbs->fiber_ptr, fiber_box);
+   hipLaunchKernelGGL(k_bundle_boxes, dim3((unsigned)(nb > 0 ? nb : 1)), dim3(B), 0, st, pr,
+                      d_bd, fiber_box, bundle_box, d_flags);
+   if (nf > 0) {
+      hipLaunchKernelGGL(k_fibers, dim3(grid_for(nf, B)), dim3(B), 0, st, pr, fbs->fiber_ptr, d_bd,
+                         fiber_box, bundle_box, (uint8_t *)fok, d_flags);
+   }
+   int64_t total = 0;
    if (np > 0) {
       hipLaunchKernelGGL(k_segments, dim3(grid_for(np, B)), dim3(B), 0, st, pr, fbs->points,
                          fbs->fiber_ptr, d_bd, (const uint8_t *)fok, (SegRec *)segs,
@@ -1837,7 +1840,6 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
       return make_status(FPL_E_INVALID, "too many segment/tile pairs: %lld", (long long)total);
 
    FPL_TRY(ws_get(st, WS_VALS_B, sizeof(uint32_t) * (total + 1), &vb));
-   FPL_TRY(ws_get(st, WS_TILE_RANGE, sizeof(uint32_t) * 4 * n_tiles, &rng));
    int2 *tile_bundle = (int2 *)((uint32_t *)rng + 2 * n_tiles); // set with the ranges
    if (!radix) {
       if (np > 0) {
