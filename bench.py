@@ -64,17 +64,33 @@ def voxel_steps(s):
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'hbm_traffic_crossing_600x600x300.txt')
 
 
+def kernel_sources_hash():
+    """sha256 prefix of the kernel sources (fastpli_amd/csrc/*.hip, *.h)."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, 'fastpli_amd', 'csrc')
+    for name in sorted(os.listdir(d)):
+        if name.endswith(('.hip', '.h')):
+            with open(os.path.join(d, name), 'rb') as f:
+                h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(kernel, storage):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE + WRITE_SIZE summary.
 
-    The summary holds lines `traffic <kernel> <storage> <bytes>` (counter KiB scaled by
-    the factors a 1 GiB device copy gives); None when no line matches.
+    The summary holds lines `traffic <kernel> <storage> <bytes> <sources hash>`
+    (counter KiB scaled by the factors a 1 GiB device copy gives), measured on
+    the kernel sources whose hash it names; None when no line matches these
+    sources (the counters are stale).
     """
+    src = kernel_sources_hash()
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
                 w = line.split()
-                if len(w) == 4 and w[0] == 'traffic' and w[1] == kernel and w[2] == storage:
+                if (len(w) == 5 and w[0] == 'traffic' and w[1] == kernel and w[2] == storage and
+                        w[4] == src):
                     return float(w[3])
     except OSError:
         pass
