@@ -16,12 +16,16 @@
 //   k_segments    one thread per segment: FP64 record + integer bbox
 //                 (generator.cpp:293-305) + the 8x8x32 voxel tiles of the bbox that
 //                 can hold a candidate (tile_box): count and mask
-//   scan          hipcub exclusive sum of per-segment tile counts
-//   k_pad         the pair count for the next call; padding up to the count this
-//                 call assumed (the previous call's), or a flag when it is larger
-//   k_emit        (tile, segment) pairs in segment order
-//   radix sort    stable by tile -> per tile list in global segment order
-//   k_ranges      each tile's list range and the bundles at its two ends
+//                 (and each tile's segment count, through an LDS table per
+//                 workgroup)
+//   scan          hipcub exclusive sum of the per-tile counts: list offsets
+//   k_emit_atomic the (tile, segment) pairs placed by per-tile atomic cursors;
+//                 the pair count for the next call, or a flag when it exceeds
+//                 the count this call assumed (the previous call's)
+//   k_tile_sort   one wave per tile: its list in global segment order (bitonic),
+//                 its range and the bundles at its two ends
+//   (lists longer than 256: segment scan, k_pad, k_emit, a stable radix sort
+//   by tile and k_ranges instead)
 //   k_voxelize    one workgroup per tile, the tile's segment records staged in
 //                 LDS; each wave owns every fourth z layer of the tile (8x8x8
 //                 voxels), whose state lives in LDS.
@@ -1620,18 +1624,16 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
-// One generation.  use_hint: size the (tile, segment) pair arrays by the
-// previous call's pair count on this stream instead of reading the count back
-// in the middle of the call (a host round trip that leaves the GPU idle);
-// `overflow` reports a count above it (the outputs are then incomplete and
-// the caller runs again without the hint).
 // A scene hint with this bit: its tile lists are too long for the per-tile sort
 constexpr int64_t LONG_LISTS = (int64_t)1 << 62;
 
-// One generation.  `overflow`: the pairs exceeded the capacity this call
-// assumed (the previous call's count); `long_lists`: a tile list exceeded the
-// per-tile sort.  The caller runs again in either case (the outputs are
-// incomplete), with the hint updated.
+// One generation.  use_hint: size the (tile, segment) pair arrays by the
+// previous call's pair count on this stream instead of reading the count back
+// in the middle of the call (a host round trip that leaves the GPU idle);
+// force_radix: bin with the radix sort.  `overflow`: the pairs exceeded the
+// capacity this call assumed; `long_lists`: a tile list exceeded the per-tile
+// sort.  The caller runs again in either case (the outputs are incomplete),
+// with the hint updated.
 Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outputs &out,
                 double *props, uint32_t *warn, int32_t *label_max, hipStream_t st, bool use_hint,
                 bool force_radix, bool &overflow, bool &long_lists) {
