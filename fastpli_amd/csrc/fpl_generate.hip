@@ -1180,16 +1180,15 @@ __device__ __forceinline__ void note_nonzero(uint64_t any, uint32_t *flags) {
 
 // REC16: the records are in a 16-bit storage (a kernel variant of its own:
 // the encoders' registers would make the f32 build spill)
-template <bool REC16>
-__device__ __forceinline__ void write_voxel(const Outputs &o, const Params &pr, int vx, int vy,
-                                            int vz, int32_t lab, const float vo[3],
-                                            uint32_t *flags) {
+// streaming (nontemporal) stores: nothing here reads the outputs again, and
+// L2 keeps the segment records the tiles stage instead (the arrays and the
+// records of 600 x 600 x 300 voxels, 3.5 GB, would otherwise pass through it)
+__device__ __forceinline__ void write_arrays(const Outputs &o, const Params &pr, int vx, int vy,
+                                             int vz, int32_t lab, const float vo[3],
+                                             uint32_t *flags) {
    // the past-the-end layer index of the last bundle gives label n_props
    // (beyond the property table, which CheckInput rejects): bit 0 of flags[3]
    if (lab >= pr.n_props) atomicOr(&flags[3], 1u);
-   // streaming (nontemporal) stores: nothing here reads the outputs again, and
-   // L2 keeps the segment records the tiles stage instead (the arrays and the
-   // records of 600 x 600 x 300 voxels, 3.5 GB, would otherwise pass through it)
    if (o.label || o.vector) {
       const int64_t ind = ((int64_t)vx * pr.ny + vy) * pr.nz + vz;
       if (o.label) __builtin_nontemporal_store(lab, o.label + ind);
@@ -1199,15 +1198,26 @@ __device__ __forceinline__ void write_voxel(const Outputs &o, const Params &pr, 
          __builtin_nontemporal_store(e, (v3f *)(o.vector + 3 * ind));
       }
    }
-   if (o.rec_a) {
-      const int64_t r = ((int64_t)vx * pr.nz + vz) * pr.ny + vy;
-      if constexpr (REC16)
-         store_record_short(o, r, lab, vo[0], vo[1], vo[2]);
-      else
-         rec::store(rec::ST_F32, o.rec_a, nullptr, r,
-                    make_int4(lab, __float_as_int(vo[0]), __float_as_int(vo[1]),
-                              __float_as_int(vo[2])));
-   }
+}
+
+template <bool REC16>
+__device__ __forceinline__ void write_record(const Outputs &o, const Params &pr, int vx, int vy,
+                                             int vz, int32_t lab, const float vo[3]) {
+   const int64_t r = ((int64_t)vx * pr.nz + vz) * pr.ny + vy;
+   if constexpr (REC16)
+      store_record_short(o, r, lab, vo[0], vo[1], vo[2]);
+   else
+      rec::store(rec::ST_F32, o.rec_a, nullptr, r,
+                 make_int4(lab, __float_as_int(vo[0]), __float_as_int(vo[1]),
+                           __float_as_int(vo[2])));
+}
+
+template <bool REC16>
+__device__ __forceinline__ void write_voxel(const Outputs &o, const Params &pr, int vx, int vy,
+                                            int vz, int32_t lab, const float vo[3],
+                                            uint32_t *flags) {
+   write_arrays(o, pr, vx, vy, vz, lab, vo, flags);
+   if (o.rec_a) write_record<REC16>(o, pr, vx, vy, vz, lab, vo);
 }
 
 // One workgroup per 8x8x32 tile, wave w owning the layers z = 4 l + w, l < 8
@@ -1499,60 +1509,100 @@ __global__ void __launch_bounds__(WG, sizeof(SegT) == 2 ? 6 : 4)
    const bool need_vec = out.vector != nullptr || out.rec_a != nullptr;
    uint64_t nonzero = 0; // wave-uniform: some voxel of the wave got a label != 0
    if (short_seg && free_mode) {
-      // Results of the free mode, eight columns of eight voxels per pass: the
-      // label and layer model come from the key; a 'p' voxel reads its
-      // winner's direction (list entry, then table), an 'r' voxel its points
+      // Results of the free mode.  The label / axis arrays ([x][y][z]) want
+      // whole z columns in one store, the march records ([x][z][y]) whole y
+      // rows: wave w resolves the two x rows xt + 2 w, xt + 2 w + 1 of the tile
+      // with lane = z (0..31) + 32 (y & 1), eight voxels per lane, and stores
+      // labels and axes as 128-byte runs; then, after a barrier (no key is
+      // read again), each x row goes through the wave's own key array (4 KB)
+      // to lane = y (0..7) + 8 (z & 7) and the records are stored as whole
+      // 128-byte rows.  The label and layer model come from the key; a 'p'
+      // voxel reads its winner's direction (list entry, then table), an 'r'
+      // voxel its points.
+      const int lz = lane & 31, yp = lane >> 5;
+      const WaveState<SegT> &KS = wstate[lz & 3]; // the wave that owns layer lz
       int32_t sgs[8];
       uint32_t los[8];
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-         const unsigned long long key = WS.key[lbase + j * 8];
+      for (int i = 0; i < 8; i++) { // voxel (2 wave + (i & 1), 2 (i >> 1) + yp, lz)
+         const int lx = 2 * wave + (i & 1), ly = 2 * (i >> 1) + yp;
+         const unsigned long long key = KS.key[(lz >> 2) * 64 + lx * 8 + ly];
          const bool hit = (uint32_t)(key >> 32) != 0x7F800000u;
-         los[j] = hit ? (uint32_t)key : 0u;
-         sgs[j] = -1;
-         if (need_vec && ((los[j] >> 14) & 3u) != 0u)
-            sgs[j] = (int32_t)list[beg + (0x10000u - (los[j] >> 16)) - 1u];
+         los[i] = hit ? (uint32_t)key : 0u;
+         sgs[i] = -1;
+         if (need_vec && ((los[i] >> 14) & 3u) != 0u)
+            sgs[i] = (int32_t)list[beg + (0x10000u - (los[i] >> 16)) - 1u];
       }
-      // every load of the pass before its first store: on this part a load
-      // issued after a store also waits for the store's completion
+      // every load before the first store: on this part a load issued after a
+      // store also waits for the store's completion
       int32_t labs[8];
       float vos[8][3];
+      const int vz = zt + lz;
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-         const int vx = xt + j, vy = yt + (lane >> 3), vz = zt + zo;
-         const uint32_t lo = los[j];
-         labs[j] = (int32_t)(lo & 0x3FFFu);
-         float *vo = vos[j];
+      for (int i = 0; i < 8; i++) {
+         const int vx = xt + 2 * wave + (i & 1), vy = yt + 2 * (i >> 1) + yp;
+         const uint32_t lo = los[i];
+         labs[i] = (int32_t)(lo & 0x3FFFu);
+         float *vo = vos[i];
          vo[0] = vo[1] = vo[2] = 0.0f;
-         if (vx < pr.nx && vy < pr.ny && vz < pr.nz) {
-            if (need_vec) {
-               const uint32_t m2 = (lo >> 14) & 3u;
-               if (m2 == 1u) {
-                  const float4 d = seg_dir[sgs[j]];
-                  vo[0] = d.x;
-                  vo[1] = d.y;
-                  vo[2] = d.z;
-               } else if (m2 == 2u) {
-                  const SegRec &R = segs[sgs[j]];
-                  const double pt[3] = {vx + 0.5, vy + 0.5, vz + 0.5};
-                  const double pp[3] = {R.p[0], R.p[1], R.p[2]};
-                  const double qq[3] = {R.q[0], R.q[1], R.q[2]};
-                  const SegExact E = seg_exact(pp, qq);
-                  const Closest cp = closest(pt, E);
-                  unit_to_float(pt[0] - cp.c[0], pt[1] - cp.c[1], pt[2] - cp.c[2], vo);
-               }
+         if (vx < pr.nx && vy < pr.ny && vz < pr.nz && need_vec) {
+            const uint32_t m2 = (lo >> 14) & 3u;
+            if (m2 == 1u) {
+               const float4 d = seg_dir[sgs[i]];
+               vo[0] = d.x;
+               vo[1] = d.y;
+               vo[2] = d.z;
+            } else if (m2 == 2u) {
+               const SegRec &R = segs[sgs[i]];
+               const double pt[3] = {vx + 0.5, vy + 0.5, vz + 0.5};
+               const double pp[3] = {R.p[0], R.p[1], R.p[2]};
+               const double qq[3] = {R.q[0], R.q[1], R.q[2]};
+               const SegExact E = seg_exact(pp, qq);
+               const Closest cp = closest(pt, E);
+               unit_to_float(pt[0] - cp.c[0], pt[1] - cp.c[1], pt[2] - cp.c[2], vo);
             }
          }
       }
+      __syncthreads(); // every key is read: each wave's key array is free
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-         const int vx = xt + j, vy = yt + (lane >> 3), vz = zt + zo;
+      for (int i = 0; i < 8; i++) {
+         const int vx = xt + 2 * wave + (i & 1), vy = yt + 2 * (i >> 1) + yp;
          if (vx < pr.nx && vy < pr.ny && vz < pr.nz) {
-            write_voxel<REC16>(out, pr, vx, vy, vz, labs[j], vos[j], flags);
-            nonzero |= __ballot(labs[j] != 0);
+            write_arrays(out, pr, vx, vy, vz, labs[i], vos[i], flags);
+            nonzero |= __ballot(labs[i] != 0);
          }
       }
       note_nonzero(nonzero, flags);
+      if (out.rec_a) {
+         // y-row order through LDS: voxel (y, z) of an x row at 16-byte slot
+         // 8 z + (y ^ (z & 7)) (the xor spreads the column writes over the banks)
+         uint4 *T = (uint4 *)wstate[wave].key;
+         static_assert(sizeof(wstate[0].key) >= 256 * sizeof(uint4), "transpose buffer");
+         const int ry = lane & 7, rz = lane >> 3;
+#pragma unroll
+         for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+               const int i = 2 * k + h, ly = 2 * k + yp;
+               T[8 * lz + (ly ^ (lz & 7))] =
+                   make_uint4((uint32_t)labs[i], __float_as_uint(vos[i][0]),
+                              __float_as_uint(vos[i][1]), __float_as_uint(vos[i][2]));
+            }
+            wave_sync();
+            const int vx = xt + 2 * wave + h, vy = yt + ry;
+#pragma unroll
+            for (int zb = 0; zb < 4; zb++) {
+               const int z = 8 * zb + rz, vz2 = zt + z;
+               const uint4 e = T[8 * z + (ry ^ (z & 7))];
+               if (vx < pr.nx && vy < pr.ny && vz2 < pr.nz) {
+                  const float vo[3] = {__uint_as_float(e.y), __uint_as_float(e.z),
+                                       __uint_as_float(e.w)};
+                  write_record<REC16>(out, pr, vx, vy, vz2, (int32_t)e.x, vo);
+               }
+            }
+            wave_sync(); // the next x row overwrites the buffer
+         }
+      }
       return;
    }
    // Results of the ordered mode, eight columns of eight voxels per pass.  The
