@@ -112,8 +112,8 @@ std::mutex g_prof_mutex;
 bool g_prof_on = false;
 std::map<std::string, std::pair<double, int64_t>> g_prof;
 // timing events are reused (creating and destroying two events per launch
-// held up the launches that follow); only timers whose second event has
-// completed return theirs (collect()), others destroy them
+// held up the launches that follow); a timer that is never collected (an
+// error path) destroys its events
 std::vector<hipEvent_t> g_event_pool;
 
 bool take_event(hipEvent_t *e) {
@@ -128,10 +128,33 @@ bool take_event(hipEvent_t *e) {
    return hipEventCreate(e) == hipSuccess;
 }
 
-void give_events(hipEvent_t a, hipEvent_t b) {
-   std::lock_guard<std::mutex> lock(g_prof_mutex);
-   g_event_pool.push_back(a);
-   g_event_pool.push_back(b);
+// Finished timers wait here (their events recorded, not yet read): reading
+// an elapsed time at every launch costs host time between the launches of a
+// step, so the times are read when they are asked for (fpl_profile_read) or
+// when many have gathered.
+struct PendingTime {
+   const char *name;
+   hipEvent_t a, b;
+};
+std::vector<PendingTime> g_pending;
+
+// caller holds g_prof_mutex
+void drain_pending() {
+   for (const PendingTime &p : g_pending) {
+      float ms = 0.0f;
+      if (hipEventSynchronize(p.b) == hipSuccess &&
+          hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+         auto &e = g_prof[p.name];
+         e.first += ms;
+         e.second += 1;
+         g_event_pool.push_back(p.a);
+         g_event_pool.push_back(p.b);
+      } else {
+         (void)hipEventDestroy(p.a);
+         (void)hipEventDestroy(p.b);
+      }
+   }
+   g_pending.clear();
 }
 } // namespace
 
@@ -154,17 +177,10 @@ void KernelTimer::stop() {
 
 void KernelTimer::collect() {
    if (!on) return;
-   float ms = 0.0f;
-   if (hipEventSynchronize(b) == hipSuccess) {
-      if (hipEventElapsedTime(&ms, a, b) == hipSuccess) {
-         std::lock_guard<std::mutex> lock(g_prof_mutex);
-         auto &e = g_prof[name];
-         e.first += ms;
-         e.second += 1;
-      }
-      give_events(a, b);
-      a = b = nullptr;
-   }
+   std::lock_guard<std::mutex> lock(g_prof_mutex);
+   g_pending.push_back({name, a, b});
+   a = b = nullptr;
+   if (g_pending.size() >= 1024) drain_pending();
    on = false;
 }
 
@@ -180,11 +196,15 @@ extern "C" {
 void fpl_profile_enable(int enable) {
    std::lock_guard<std::mutex> lock(fpl::g_prof_mutex);
    fpl::g_prof_on = enable != 0;
-   if (enable) fpl::g_prof.clear();
+   if (enable) {
+      fpl::drain_pending();
+      fpl::g_prof.clear();
+   }
 }
 
 int fpl_profile_read(const char *name, double *total_ms, int64_t *launches) {
    std::lock_guard<std::mutex> lock(fpl::g_prof_mutex);
+   fpl::drain_pending();
    auto it = fpl::g_prof.find(name ? name : "");
    if (it == fpl::g_prof.end()) {
       if (total_ms) *total_ms = 0;

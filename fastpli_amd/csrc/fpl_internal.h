@@ -99,7 +99,8 @@ Status host_stage(void *stream, size_t bytes, void **ptr);
 
 // Optional kernel timing (fpl_profile_enable).  Usage around one launch:
 //   KernelTimer kt("march", stream);  launch;  kt.stop();
-// and after the call's final stream synchronization: kt.collect().
+// and once the launch is queued for good: kt.collect() hands the two events
+// over; their elapsed time is read when fpl_profile_read asks for it.
 struct KernelTimer {
    const char *name;
    hipStream_t stream;
@@ -107,7 +108,7 @@ struct KernelTimer {
    bool on = false;
    KernelTimer(const char *n, hipStream_t s);
    void stop();
-   void collect(); // requires the stream to have passed event b
+   void collect();
    ~KernelTimer();
 };
 
