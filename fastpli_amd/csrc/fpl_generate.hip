@@ -15,13 +15,10 @@
 //   k_fibers      one thread per fiber: bundle/fiber culling (generator.cpp:208-221)
 //   k_segments    one thread per segment: FP64 record + integer bbox
 //                 (generator.cpp:293-305) + the 8x8x32 voxel tiles of the bbox that
-//                 can hold a candidate (tile_box): count and mask
-//                 (and each tile's segment count, through an LDS table per
-//                 workgroup)
-//   scan          hipcub exclusive sum of the per-tile counts: list offsets
-//   k_emit_atomic the (tile, segment) pairs placed by per-tile atomic cursors;
-//                 the pair count for the next call, or a flag when it exceeds
-//                 the count this call assumed (the previous call's)
+//                 can hold a candidate (tile_box): count and mask, and every
+//                 (tile, segment) pair placed in its tile's fixed-size list by
+//                 per-tile atomic cursors (reserved per workgroup through an
+//                 LDS table)
 //   k_tile_sort   one wave per tile: its list in global segment order (bitonic),
 //                 its range and the bundles at its two ends
 //   (lists longer than 256: segment scan, k_pad, k_emit, a stable radix sort
@@ -227,7 +224,7 @@ __global__ void k_bundle_boxes(Params pr, const BundleDev *__restrict__ bd,
    if (b == 0 && threadIdx.x == 0) {
       flags[0] = 0; // warnings
       flags[1] = 0; // voxelizer: a tile list too long for 16-bit positions
-      flags[2] = 0; // tile pairs beyond the capacity the host assumed (run_once)
+      flags[2] = 0; // binning: bit 0 radix pairs beyond the assumed count, bit 1 a long list
       flags[3] = 0; // voxelizer: a label >= n_props was written
    }
    double lo[3], hi[3];
@@ -439,12 +436,29 @@ __device__ __forceinline__ void hash_clear(TileHash &H) {
    }
 }
 
+// Tile lists of the binning by atomics: SORT_MAX entries per tile, tile t's at
+// list[t * SORT_MAX]; a tile with more segments makes the host bin the volume
+// again with the radix sort (its list is longer than the per-tile sort takes).
+constexpr uint32_t SORT_MAX = 256;
+
+// One thread per segment: its record, direction and slab data, the kept tiles
+// of its box (count, and a mask for k_emit), and with `tile_count` set the
+// binning by atomics: every kept (tile, segment) pair goes into the tile's
+// list at the tile's cursor (tile_count[t], which ends as the tile's segment
+// count), in no particular order (k_tile_sort orders the lists).  The
+// segments of a workgroup are consecutive pieces of a few fibers and share
+// most of their tiles: the workgroup counts its pairs per tile in an LDS
+// table, reserves one range per distinct tile with one global atomic, and
+// places its pairs inside those ranges (same-address global atomics
+// serialise in L2).  A tile without a table slot takes a global atomic per
+// pair.
 __global__ void k_segments(Params pr, const double *__restrict__ P,
                            const int64_t *__restrict__ fiber_ptr,
                            const BundleDev *__restrict__ bd, const uint8_t *__restrict__ fiber_ok,
                            SegRec *segs, float4 *seg_dir, float4 *seg_clip, int64_t *seg_count,
-                           uint32_t *tile_count) {
+                           uint32_t *tile_count, uint32_t *list) {
    __shared__ TileHash H;
+   __shared__ uint32_t base[HASH_N];
    if (tile_count) {
       hash_clear(H);
       __syncthreads();
@@ -453,9 +467,10 @@ __global__ void k_segments(Params pr, const double *__restrict__ P,
    const bool seg = i < pr.n_points;
    const int64_t f = seg ? find_fiber(fiber_ptr, pr.n_fibers, i) : 0;
    int64_t count = 0;
+   SegRec R;
+   uint32_t mask = 0;
    if (seg && fiber_ok[f] && i + 1 < fiber_ptr[f + 1]) {
       const int b = find_bundle(bd, pr.n_bundles, f);
-      SegRec R;
       double rp, rq;
       load_point(P, i, pr, R.p, rp);
       load_point(P, i + 1, pr, R.q, rq);
@@ -501,13 +516,12 @@ __global__ void k_segments(Params pr, const double *__restrict__ P,
          segment_clip(pr, P, i, i > fiber_ptr[f], i + 2 < fiber_ptr[f + 1], R.p, R.q, rp, rq,
                       max_r, seg_clip + 2 * i);
          // the tiles of the box that hold candidates of the segment; for boxes
-         // of up to 31 tiles their mask (bit k: the k-th tile in k_emit's loop
-         // order) goes into the direction entry's w for k_emit, else bit 31
-         // asks k_emit to test the tiles again
+         // of up to 31 tiles their mask (bit k: the k-th tile in the loop
+         // order below) goes into the direction entry's w, else bit 31 asks
+         // for the tiles to be tested again
          const int tx0 = R.lo[0] / TX, tx1 = (R.hi[0] - 1) / TX;
          const int ty0 = R.lo[1] / TY, ty1 = (R.hi[1] - 1) / TY;
          const int tz0 = R.lo[2] / TZ, tz1 = (R.hi[2] - 1) / TZ;
-         uint32_t mask = 0;
          int k = 0;
          for (int tx = tx0; tx <= tx1; tx++)
             for (int ty = ty0; ty <= ty1; ty++)
@@ -516,14 +530,10 @@ __global__ void k_segments(Params pr, const double *__restrict__ P,
                   if (b.za < b.zb) {
                      count++;
                      if (k < 31) mask |= 1u << k;
-                     // the binning by atomics counts the tile's segments
                      if (tile_count) {
                         const uint32_t t = (uint32_t)((tx * pr.nty + ty) * pr.ntz + tz);
                         const int h = hash_insert(H, t);
-                        if (h >= 0)
-                           atomicAdd(&H.cnt[h], 1u);
-                        else
-                           atomicAdd(&tile_count[t], 1u);
+                        if (h >= 0) atomicAdd(&H.cnt[h], 1u);
                      }
                   }
                }
@@ -532,11 +542,36 @@ __global__ void k_segments(Params pr, const double *__restrict__ P,
       }
    }
    if (seg) seg_count[i] = count;
-   if (tile_count) {
-      __syncthreads();
-      for (int k = threadIdx.x; k < HASH_N; k += blockDim.x)
-         if (H.key[k] != HASH_EMPTY) atomicAdd(&tile_count[H.key[k]], H.cnt[k]);
+   if (!tile_count) return;
+   // one reservation per tile of the table, then every pair its place
+   __syncthreads();
+   for (int k = threadIdx.x; k < HASH_N; k += blockDim.x) {
+      if (H.key[k] != HASH_EMPTY) {
+         base[k] = atomicAdd(&tile_count[H.key[k]], H.cnt[k]);
+         H.cnt[k] = 0;
+      }
    }
+   __syncthreads();
+   if (count == 0) return;
+   const int tx0 = R.lo[0] / TX, tx1 = (R.hi[0] - 1) / TX;
+   const int ty0 = R.lo[1] / TY, ty1 = (R.hi[1] - 1) / TY;
+   const int tz0 = R.lo[2] / TZ, tz1 = (R.hi[2] - 1) / TZ;
+   int k = 0;
+   for (int tx = tx0; tx <= tx1; tx++)
+      for (int ty = ty0; ty <= ty1; ty++)
+         for (int tz = tz0; tz <= tz1; tz++, k++) {
+            bool keep = (mask >> k) & 1u;
+            if (mask & 0x80000000u) {
+               const TileBox b = tile_box(R, tx * TX, ty * TY, tz * TZ);
+               keep = b.za < b.zb;
+            }
+            if (!keep) continue;
+            const uint32_t t = (uint32_t)((tx * pr.nty + ty) * pr.ntz + tz);
+            const int h = hash_find(H, t);
+            const uint32_t r =
+                h >= 0 ? base[h] + atomicAdd(&H.cnt[h], 1u) : atomicAdd(&tile_count[t], 1u);
+            if (r < SORT_MAX) list[(size_t)t * SORT_MAX + r] = (uint32_t)i;
+         }
 }
 
 __global__ void k_emit(Params pr, const SegRec *__restrict__ segs,
@@ -614,81 +649,10 @@ __global__ void k_ranges(const uint32_t *__restrict__ keys, const uint32_t *__re
    }
 }
 
-// ---- binning by atomics: tile lists of at most SORT_MAX segments.
-// k_segments counts each tile's segments, an exclusive scan gives every tile
-// its list offset, k_emit_atomic places the (tile, segment) pairs by atomic
-// cursors (in no particular order) and k_tile_sort puts every list in global
-// segment order, one wave per tile.  A longer list makes the host bin the
-// volume again with the radix sort above.
-constexpr uint32_t SORT_MAX = 256;
-
-// Pairs of segment i into their tiles' lists: position tile_off[t] + (the
-// count left in tile_count[t] after taking them; the pairs of a workgroup
-// reserve their places per tile through the LDS table); tile_count ends at
-// zero.
-// The pair count into flags[4..5] for the host's next call; flags[2] bit 0
-// when it exceeds the capacity the host assumed (the pairs beyond it are
-// dropped and the host bins again with the exact count).
-__global__ void k_emit_atomic(Params pr, const SegRec *__restrict__ segs,
-                              const float4 *__restrict__ seg_dir,
-                              const int64_t *__restrict__ seg_count, uint32_t *tile_count,
-                              const uint32_t *__restrict__ tile_off, uint32_t n_tiles, int64_t cap,
-                              uint32_t *list, uint32_t *flags) {
-   __shared__ TileHash H;
-   __shared__ uint32_t base[HASH_N];
-   hash_clear(H);
-   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (i == 0) {
-      const uint32_t n = tile_off[n_tiles];
-      flags[4] = n;
-      flags[5] = 0;
-      if ((int64_t)n > cap) atomicOr(&flags[2], 1u);
-   }
-   __syncthreads();
-   const bool seg = i < pr.n_points && seg_count[i] != 0;
-   // the kept tiles of segment i in k_segments' order, f(tile index)
-   auto for_tiles = [&](auto fn) {
-      const SegRec &R = segs[i];
-      const int tx0 = R.lo[0] / TX, tx1 = (R.hi[0] - 1) / TX;
-      const int ty0 = R.lo[1] / TY, ty1 = (R.hi[1] - 1) / TY;
-      const int tz0 = R.lo[2] / TZ, tz1 = (R.hi[2] - 1) / TZ;
-      const uint32_t mask = __float_as_uint(seg_dir[i].w); // k_segments' tile mask
-      int k = 0;
-      for (int tx = tx0; tx <= tx1; tx++)
-         for (int ty = ty0; ty <= ty1; ty++)
-            for (int tz = tz0; tz <= tz1; tz++, k++) {
-               bool keep = (mask >> k) & 1u;
-               if (mask & 0x80000000u) { // as k_segments counts
-                  const TileBox b = tile_box(R, tx * TX, ty * TY, tz * TZ);
-                  keep = b.za < b.zb;
-               }
-               if (keep) fn((uint32_t)((tx * pr.nty + ty) * pr.ntz + tz));
-            }
-   };
-   // the workgroup's pairs per tile; then one reservation per tile (the count
-   // left in tile_count after taking them); then every pair its place
-   if (seg)
-      for_tiles([&](uint32_t t) {
-         const int h = hash_insert(H, t);
-         if (h >= 0) atomicAdd(&H.cnt[h], 1u);
-      });
-   __syncthreads();
-   for (int k = threadIdx.x; k < HASH_N; k += blockDim.x) {
-      if (H.key[k] != HASH_EMPTY) {
-         base[k] = atomicSub(&tile_count[H.key[k]], H.cnt[k]) - H.cnt[k];
-         H.cnt[k] = 0;
-      }
-   }
-   __syncthreads();
-   if (seg)
-      for_tiles([&](uint32_t t) {
-         const int h = hash_find(H, t);
-         const uint32_t r = h >= 0 ? base[h] + atomicAdd(&H.cnt[h], 1u)
-                                   : atomicSub(&tile_count[t], 1u) - 1u;
-         const int64_t o = (int64_t)tile_off[t] + r;
-         if (o < cap) list[o] = (uint32_t)i;
-      });
-}
+// ---- binning by atomics: k_segments places the (tile, segment) pairs in the
+// tiles' fixed-size lists (in no particular order) and k_tile_sort puts every
+// list in global segment order, one wave per tile.  A longer list makes the
+// host bin the volume again with the radix sort above.
 
 // Bitonic sort of the n <= 64 E values at p (ascending), element e * 64 + lane
 // in register e of the lane; the padding sorts last.
@@ -740,38 +704,38 @@ __device__ __forceinline__ void wave_sort(uint32_t *p, uint32_t n, int lane, uin
    last = (uint32_t)__shfl((int)w, (int)((n - 1) & 63));
 }
 
-// One wave per tile: its list in global segment order, [begin, end) (clamped
-// to the capacity: after an overflow the voxelizer still reads in bounds) and
-// the bundles of its first and last segment.  A list longer than SORT_MAX
-// sets flags[2] bit 1.
+// One wave per tile: its list in global segment order, its range [begin, end)
+// and the bundles of its first and last segment.  A tile with more than
+// SORT_MAX segments sets flags[2] bit 1 (the host bins again); its range
+// stays inside its list, so the voxelizer still reads in bounds.
 __global__ void __launch_bounds__(256) k_tile_sort(uint32_t n_tiles,
-                                                   const uint32_t *__restrict__ tile_off,
-                                                   int64_t cap, uint32_t *list,
+                                                   const uint32_t *__restrict__ tile_count,
+                                                   uint32_t *list,
                                                    const SegRec *__restrict__ segs,
                                                    uint32_t *range, int2 *tile_bundle,
                                                    uint32_t *flags) {
    const uint32_t t = blockIdx.x * 4u + (threadIdx.x >> 6);
    const int lane = threadIdx.x & 63;
    if (t >= n_tiles) return;
-   const int64_t b = tile_off[t], e = tile_off[t + 1];
-   const uint32_t cb = (uint32_t)min(b, cap), ce = (uint32_t)min(e, cap);
+   const uint32_t c = tile_count[t];
+   const uint32_t n = c < SORT_MAX ? c : SORT_MAX;
+   const uint32_t b = t * SORT_MAX;
    if (lane == 0) {
-      range[2 * (int64_t)t] = cb;
-      range[2 * (int64_t)t + 1] = ce;
+      range[2 * (int64_t)t] = b;
+      range[2 * (int64_t)t + 1] = b + n;
    }
-   const uint32_t n = ce - cb;
-   if (n == 0 || e > cap) return; // empty, or the host bins again
-   if (n > SORT_MAX) {
+   if (n == 0) return;
+   if (c > SORT_MAX) {
       if (lane == 0) atomicOr(&flags[2], 2u);
       return;
    }
    uint32_t first, last;
    if (n <= 64)
-      wave_sort<1>(list + cb, n, lane, first, last);
+      wave_sort<1>(list + b, n, lane, first, last);
    else if (n <= 128)
-      wave_sort<2>(list + cb, n, lane, first, last);
+      wave_sort<2>(list + b, n, lane, first, last);
    else
-      wave_sort<4>(list + cb, n, lane, first, last);
+      wave_sort<4>(list + b, n, lane, first, last);
    if (lane == 0) tile_bundle[t] = make_int2(segs[first].bundle, segs[last].bundle);
 }
 
@@ -1677,13 +1641,13 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
 // A scene hint with this bit: its tile lists are too long for the per-tile sort
 constexpr int64_t LONG_LISTS = (int64_t)1 << 62;
 
-// One generation.  use_hint: size the (tile, segment) pair arrays by the
-// previous call's pair count on this stream instead of reading the count back
-// in the middle of the call (a host round trip that leaves the GPU idle);
-// force_radix: bin with the radix sort.  `overflow`: the pairs exceeded the
-// capacity this call assumed; `long_lists`: a tile list exceeded the per-tile
-// sort.  The caller runs again in either case (the outputs are incomplete),
-// with the hint updated.
+// One generation.  use_hint: the radix sort sizes its (tile, segment) pair
+// arrays by the previous call's pair count on this stream instead of reading
+// the count back in the middle of the call (a host round trip that leaves the
+// GPU idle); force_radix: bin with the radix sort.  `overflow`: the radix
+// sort's pairs exceeded the count this call assumed; `long_lists`: a tile list
+// exceeded the fixed-size lists of the binning by atomics.  The caller runs
+// again in either case (the outputs are incomplete), with the hint updated.
 Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outputs &out,
                 double *props, uint32_t *warn, int32_t *label_max, hipStream_t st, bool use_hint,
                 bool force_radix, bool &overflow, bool &long_lists) {
@@ -1820,27 +1784,30 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
    uint32_t *d_flags = (uint32_t *)flags;
 
    const int B = 256;
-   // a hint above 64 tiles per point (a much larger previous scene) would make
-   // this call pad and sort mostly padding: it reads its count back instead.
-   // Bit 62 of a hint: the scene's tile lists are too long for the per-tile
-   // sort (the radix sort bins it).  The workspace of the binning is taken
-   // before the first launch, so that the small kernels go out back to back.
+   // The binning by atomics writes fixed-size tile lists (SORT_MAX entries per
+   // tile: no count to read back); a scene whose hint has bit 62 set (its
+   // tile lists are too long for them), or too many tiles for 32-bit list
+   // positions, is binned by the radix sort, whose pair arrays are sized by
+   // the previous call's pair count on this stream (the rest of the hint)
+   // instead of a read-back of the count in the middle of the call; a hint
+   // above 64 tiles per point (a much larger previous scene) would make this
+   // call pad and sort mostly padding: it reads its count back instead.  The
+   // workspace of the binning is taken before the first launch, so that the
+   // small kernels go out back to back.
    const std::array<int64_t, 5> scene{fbs->n_points, fbs->n_fibers, nx, ny, nz};
    const int64_t raw_hint = pair_hint(st, scene);
-   const bool radix = force_radix || (raw_hint & LONG_LISTS) != 0;
+   const bool radix = force_radix || (raw_hint & LONG_LISTS) != 0 ||
+                      n_tiles > ((int64_t)1 << 31) / SORT_MAX;
    int64_t hint = use_hint ? (raw_hint & ~LONG_LISTS) : 0;
    if (hint > 64 * fbs->n_points) hint = 0;
    void *ka, *va, *kb, *vb, *rng;
-   // the per-tile counts / offsets of the binning by atomics (n_tiles + 1 each)
-   uint32_t *tile_count = nullptr, *tile_off = nullptr;
+   uint32_t *tile_count = nullptr; // the binning by atomics: per-tile cursors
    if (!radix) {
-      FPL_TRY(ws_get(st, WS_KEYS_A, sizeof(uint32_t) * (n_tiles + 1), &ka));
-      FPL_TRY(ws_get(st, WS_KEYS_B, sizeof(uint32_t) * (n_tiles + 1), &kb));
+      FPL_TRY(ws_get(st, WS_KEYS_A, sizeof(uint32_t) * n_tiles, &ka));
+      FPL_TRY(ws_get(st, WS_VALS_B, sizeof(uint32_t) * SORT_MAX * n_tiles, &vb));
       tile_count = (uint32_t *)ka;
-      tile_off = (uint32_t *)kb;
-      if (hint > 0) FPL_TRY(ws_get(st, WS_VALS_B, sizeof(uint32_t) * (hint + 1), &vb));
-      hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n_tiles + 1, B)), dim3(B), 0, st, tile_count,
-                         n_tiles + 1, 0u);
+      hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n_tiles, B)), dim3(B), 0, st, tile_count,
+                         n_tiles, 0u);
    }
    FPL_TRY(ws_get(st, WS_TILE_RANGE, sizeof(uint32_t) * 4 * n_tiles, &rng));
    if (nf > 0)
@@ -1852,61 +1819,44 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
       hipLaunchKernelGGL(k_fibers, dim3(grid_for(nf, B)), dim3(B), 0, st, pr, fbs->fiber_ptr, d_bd,
                          fiber_box, bundle_box, (uint8_t *)fok, d_flags);
    }
-   int64_t total = 0;
-   if (np > 0) {
+   if (np > 0)
       hipLaunchKernelGGL(k_segments, dim3(grid_for(np, B)), dim3(B), 0, st, pr, fbs->points,
                          fbs->fiber_ptr, d_bd, (const uint8_t *)fok, (SegRec *)segs,
-                         seg_dir, seg_clip, (int64_t *)cnt, tile_count);
-      // exclusive scan: per tile (its list offsets; the last entry, 0, gives
-      // the total) or per segment (its pair positions)
-      size_t tmp_bytes = 0;
-      void *tmp;
-      if (!radix) {
-         FPL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, tile_count, tile_off,
-                                                  (int)(n_tiles + 1), st));
-         FPL_TRY(ws_get(st, WS_CUB_TEMP, tmp_bytes, &tmp));
-         FPL_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, tile_count, tile_off,
-                                                  (int)(n_tiles + 1), st));
-      } else {
-         FPL_HIP(hipMemsetAsync((int64_t *)cnt + np, 0, sizeof(int64_t), st));
-         FPL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, (int64_t *)cnt,
-                                                  (int64_t *)off, (int)(np + 1), st));
-         FPL_TRY(ws_get(st, WS_CUB_TEMP, tmp_bytes, &tmp));
-         FPL_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, (int64_t *)cnt, (int64_t *)off,
-                                                  (int)(np + 1), st));
-      }
-      if (hint > 0) {
-         total = hint; // k_pad / k_emit_atomic report a larger count
-      } else {
-         if (!radix)
-            FPL_HIP(hipMemcpyAsync(stage, tile_off + n_tiles, sizeof(uint32_t),
-                                   hipMemcpyDeviceToHost, st));
-         else
-            FPL_HIP(hipMemcpyAsync(stage, (int64_t *)off + np, sizeof(int64_t),
-                                   hipMemcpyDeviceToHost, st));
-         FPL_HIP(hipStreamSynchronize(st));
-         total = !radix ? (int64_t) * (uint32_t *)stage : *(int64_t *)stage;
-      }
-   }
-   if (total >= ((int64_t)1 << 31))
-      return make_status(FPL_E_INVALID, "too many segment/tile pairs: %lld", (long long)total);
-
-   FPL_TRY(ws_get(st, WS_VALS_B, sizeof(uint32_t) * (total + 1), &vb));
+                         seg_dir, seg_clip, (int64_t *)cnt, tile_count, (uint32_t *)vb);
    int2 *tile_bundle = (int2 *)((uint32_t *)rng + 2 * n_tiles); // set with the ranges
    if (!radix) {
       if (np > 0) {
-         hipLaunchKernelGGL(k_emit_atomic, dim3(grid_for(np, B)), dim3(B), 0, st, pr,
-                            (const SegRec *)segs, (const float4 *)seg_dir, (const int64_t *)cnt,
-                            tile_count, (const uint32_t *)tile_off, (uint32_t)n_tiles, total,
-                            (uint32_t *)vb, d_flags);
          hipLaunchKernelGGL(k_tile_sort, dim3((unsigned)((n_tiles + 3) / 4)), dim3(B), 0, st,
-                            (uint32_t)n_tiles, (const uint32_t *)tile_off, total, (uint32_t *)vb,
+                            (uint32_t)n_tiles, (const uint32_t *)tile_count, (uint32_t *)vb,
                             (const SegRec *)segs, (uint32_t *)rng, tile_bundle, d_flags);
       } else {
          hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(2 * n_tiles, B)), dim3(B), 0, st,
                             (uint32_t *)rng, 2 * n_tiles, 0u);
       }
    } else {
+      // exclusive scan of the per-segment counts: the segments' pair positions
+      int64_t total = 0;
+      if (np > 0) {
+         size_t tmp_bytes = 0;
+         void *tmp;
+         FPL_HIP(hipMemsetAsync((int64_t *)cnt + np, 0, sizeof(int64_t), st));
+         FPL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, (int64_t *)cnt,
+                                                  (int64_t *)off, (int)(np + 1), st));
+         FPL_TRY(ws_get(st, WS_CUB_TEMP, tmp_bytes, &tmp));
+         FPL_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, (int64_t *)cnt, (int64_t *)off,
+                                                  (int)(np + 1), st));
+         if (hint > 0) {
+            total = hint; // k_pad reports a larger count
+         } else {
+            FPL_HIP(hipMemcpyAsync(stage, (int64_t *)off + np, sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, st));
+            FPL_HIP(hipStreamSynchronize(st));
+            total = *(int64_t *)stage;
+         }
+      }
+      if (total >= ((int64_t)1 << 31))
+         return make_status(FPL_E_INVALID, "too many segment/tile pairs: %lld", (long long)total);
+      FPL_TRY(ws_get(st, WS_VALS_B, sizeof(uint32_t) * (total + 1), &vb));
       FPL_TRY(ws_get(st, WS_KEYS_A, sizeof(uint32_t) * (total + 1), &ka));
       FPL_TRY(ws_get(st, WS_VALS_A, sizeof(uint32_t) * (total + 1), &va));
       FPL_TRY(ws_get(st, WS_KEYS_B, sizeof(uint32_t) * (total + 1), &kb));
@@ -1962,6 +1912,19 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
    // reach n_props when a float ratio rounds past the last layer)
    const bool short_labels = n_props <= 0x3FFF;
    bool rerun = !short_labels;
+   // the binning's verdict, read with the voxelizer's flags (false: the
+   // outputs are incomplete and the caller runs again): a tile with more
+   // segments than the fixed-size lists hold, or radix-sort pairs beyond the
+   // count this call assumed.  The radix sort's pair count is the scene's hint
+   // for the next call.
+   auto binned = [&](const uint32_t *fl) -> bool {
+      if (np == 0) return true;
+      long_lists = (fl[2] & 2u) != 0;
+      overflow = (fl[2] & 1u) != 0;
+      const int64_t pairs = radix ? (int64_t)((uint64_t)fl[4] | (uint64_t)fl[5] << 32) : 0;
+      set_pair_hint(st, scene, pairs | ((radix || long_lists) ? LONG_LISTS : 0));
+      return fl[2] == 0;
+   };
    if (short_labels) {
       KernelTimer kt("voxelize", st);
       voxelize(true);
@@ -1971,15 +1934,7 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
       FPL_HIP(hipStreamSynchronize(st));
       kt.collect();
       const uint32_t *f = (const uint32_t *)stage;
-      if (np > 0) {
-         const int64_t pairs = (int64_t)((uint64_t)f[4] | (uint64_t)f[5] << 32);
-         long_lists = (f[2] & 2u) != 0;
-         set_pair_hint(st, scene, pairs | ((radix || long_lists) ? LONG_LISTS : 0));
-         if (f[2]) {
-            overflow = (f[2] & 1u) != 0;
-            return Status();
-         }
-      }
+      if (!binned(f)) return Status();
       rerun = (f[1] & 1u) != 0;
       if (!rerun && (f[3] & 4u) && !(f[0] & FPL_WARN_OVERLAP)) {
          // a past-the-end label in a single-bundle tile run without order:
@@ -2006,16 +1961,7 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
       FPL_HIP(hipMemcpyAsync(stage, d_flags, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
       FPL_HIP(hipStreamSynchronize(st));
       kt2.collect();
-      const uint32_t *f = (const uint32_t *)stage;
-      if (np > 0) {
-         const int64_t pairs = (int64_t)((uint64_t)f[4] | (uint64_t)f[5] << 32);
-         long_lists = (f[2] & 2u) != 0;
-         set_pair_hint(st, scene, pairs | ((radix || long_lists) ? LONG_LISTS : 0));
-         if (f[2]) {
-            overflow = (f[2] & 1u) != 0;
-            return Status();
-         }
-      }
+      if (!binned((const uint32_t *)stage)) return Status();
    }
    if (warn) *warn = *(uint32_t *)stage;
    if (label_max) {

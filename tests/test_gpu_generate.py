@@ -239,14 +239,12 @@ def test_generate_bent_fibers_slab_clip(seed):
     _compare(fbs, (30, 30, 30), [0.0, 0.0, 0.0], 1.0)
 
 
-def test_generate_pair_count_from_previous_call():
-    """A generation sizes its (tile, segment) pair arrays by the pair count of
-    the previous call of the same scene shape (points, fibers, volume) on the
-    same stream: the same fibers at a finer voxel size (more pairs than assumed:
-    the call runs again with its own count), then the coarser one again
-    (padding); other scenes in between (no hint); every result is bit-exact vs
-    the oracle and the warnings match."""
-    _say('generate: pair arrays sized by the previous call (larger, smaller, empty scenes)')
+def test_generate_scene_sequence():
+    """Generations of different scenes in turn on one stream (the fixed-size
+    tile lists and the per-stream workspace reused): the same points with
+    larger radii (more pairs), smaller and empty scenes; every result is
+    bit-exact vs the oracle and the warnings match."""
+    _say('generate: a sequence of larger, smaller and empty scenes')
     small = synthetic.single_bundle(seed=5, n_fibers=6, size=(8.0, 8.0, 6.0))
     large = synthetic.crossing_bundles(seed=6, n_fibers=60, size_xy=12.0, size_z=6.0)
     empty = objects.FiberBundles([[np.array([[100.0, 100.0, 100.0, 1.0],
@@ -258,6 +256,31 @@ def test_generate_pair_count_from_previous_call():
     for fbs, dim in ((small, (40, 40, 30)), (large, (60, 60, 30)), (thick, (40, 40, 30)),
                      (small, (40, 40, 30)), (empty, (20, 20, 20)), (large, (60, 60, 30))):
         _compare(fbs, dim, [0.0, 0.0, 0.0], 0.2)
+
+
+def test_generate_radix_pair_count_from_previous_call():
+    """Scenes whose tile lists are longer than the fixed-size lists (390
+    segments in one 8 x 8 x 32 tile) are binned by the radix sort, which sizes
+    its pair arrays by the pair count of the previous call of the same scene
+    shape (points, fibers, volume) on the same stream: the first call finds
+    the long list and runs again, then the same points with larger radii (more
+    pairs than assumed: the call runs again with its own count), then the
+    thinner scene again (padding), with another scene in between; every
+    result is bit-exact vs the oracle."""
+    _say('generate: radix binning, pair arrays sized by the previous call')
+    rng = np.random.default_rng(13)
+    fibers = []
+    for _ in range(10):
+        start = rng.uniform(2.0, 6.0, size=3)
+        pts = np.clip(start + np.cumsum(rng.normal(scale=0.1, size=(40, 3)), axis=0), 0.5, 7.5)
+        fibers.append(np.hstack([pts, np.full((40, 1), 0.8)]))
+    layers = [[(0.6, 0, 1, 'b'), (1.0, -0.004, 1, 'p')]]
+    thin = objects.FiberBundles([fibers], layers)
+    thick = objects.FiberBundles([[f * np.array([1.0, 1.0, 1.0, 2.0]) for f in fibers]], layers)
+    other = synthetic.single_bundle(seed=5, n_fibers=6, size=(8.0, 8.0, 6.0))
+    for fbs, dim in ((thin, (16, 16, 8)), (thick, (16, 16, 8)), (thin, (16, 16, 8)),
+                     (other, (40, 40, 30)), (thin, (16, 16, 8))):
+        _compare(fbs, dim, [0.0, 0.0, 0.0], 1.0)
 
 
 def test_generate_past_the_end_overlap_in_one_bundle():
