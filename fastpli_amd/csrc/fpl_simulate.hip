@@ -315,99 +315,136 @@ __device__ __forceinline__ int32_t clampi(int32_t v, int32_t n) {
    return v < 0 ? 0 : (v > n - 1 ? n - 1 : v);
 }
 
-// PliSimulator::GetVec Lerp/Slerp (simulator.cpp:496-555).  `own` is the
-// record of the ray's (nearest) voxel, label `lab` != 0, so the position lies
-// inside the volume.  When all corner indices coincide they are that voxel.
-// x1 - x0 is 1 whenever the indices differ, so the weight
-// (x - x0 - 0.5) / (x1 - x0) is (x - x0) - 0.5 exactly.  Levels whose two
-// corner indices coincide are skipped: with weight 0 and equal arguments both
-// interpolations return their first argument exactly.
-// With OWN false the record of the ray's voxel is not at hand: when all corners
-// coincide it is read like a corner (it carries the label `lab`).
-// `unit` (optional) is set when the result is a unit axis by construction: a
-// slerp of unit axes in a 16-bit storage (the retarder then needs no length).
-template <typename Idx, int ST, bool OWN = true>
-__device__ float3 get_vec_interp(const Field<Idx, ST> &F, double px, double py, double pz,
-                                 int32_t lab, const float3 &own, int slerp,
-                                 bool *unit = nullptr) {
-   if (unit) *unit = false;
+// PliSimulator::GetVec Lerp/Slerp (simulator.cpp:496-555), any field storage.
+// The position lies inside the volume (label `lab` != 0).  x0 = floor(x - 0.5)
+// and x1 = ceil(x - 0.5), both clamped to the volume; x1 - x0 is 1 whenever
+// the indices differ, so the weight (x - x0 - 0.5) / (x1 - x0) is
+// (x - x0) - 0.5 exactly (0 when they coincide).  Corners of another label
+// are zeroed (simulator.cpp:533-540); the interpolation runs x, y, z (4, 2, 1
+// blends); a level whose two corner indices coincide is skipped: with weight
+// 0 and equal arguments both interpolations return their first argument
+// exactly.  When all corners coincide they are the voxel holding the point,
+// read like a corner.  ST_F16 under Slerp: the corners are used as decoded,
+// and unless a corner used carries the "not normalized" bit the result counts
+// as a unit axis, `unit` (a slerp of unit axes is one; only its direction is
+// used); under Lerp each corner is scaled to unit length (the chord's length
+// enters the retardance).  ST_ANGLE16 decodes unit axes.
+//
+// Every corner load is issued before any corner is used: a load issued after
+// a branch on an earlier load's data waits for that load, one memory round
+// trip per level.  A view whose light spreads over at most one of x and y in
+// the wave (the axis "a"; a tilt along x or y) reads its four corners
+// (a0, a1) x (z0, z1) at once and interpolates a, then z: for such a view the
+// x, y, z order reduces to that, because the other axis' level returns its
+// first argument.  A view spreading over both reads all eight.
+template <typename Idx, int ST>
+__device__ float3 get_vec_queued(const Field<Idx, ST> &F, double px, double py, double pz,
+                                 int32_t lab, int slerp, bool &unit) {
    const double fx = px - 0.5, fy = py - 0.5, fz = pz - 0.5;
    const double flx = floor(fx), fly = floor(fy), flz = floor(fz);
-   // ceil(f) = floor(f) + (f != floor(f)); both clamped to the volume
    const int32_t x0 = clampi((int32_t)flx, F.nx), x1 = clampi((int32_t)flx + (fx != flx), F.nx);
    const int32_t y0 = clampi((int32_t)fly, F.ny), y1 = clampi((int32_t)fly + (fy != fly), F.ny);
    const int32_t z0 = clampi((int32_t)flz, F.nz), z1 = clampi((int32_t)flz + (fz != flz), F.nz);
-   if (OWN && x0 == x1 && y0 == y1 && z0 == z1) return own;
    const bool dx = x0 != x1, dy = y0 != y1, dz = z0 != z1;
    const float xd = dx ? (float)((px - (double)x0) - 0.5) : 0.0f;
    const float yd = dy ? (float)((py - (double)y0) - 0.5) : 0.0f;
    const float zd = dz ? (float)((pz - (double)z0) - 0.5) : 0.0f;
-   // corners with another label are zeroed (simulator.cpp:533-540).  ST_F16
-   // under Slerp: the corners are used as decoded, and unless a corner used
-   // carries the "not normalized" bit the result counts as a unit axis (a
-   // slerp of unit axes is one; only its direction is used); under Lerp each
-   // corner is scaled to unit length (the chord's length enters the
-   // retardance).  ST_ANGLE16 decodes unit axes.
    using Fd = Field<Idx, ST>;
    const bool late = ST == ST_F16 && slerp;
+   // corners of another label are zeroed; `keep` collects the "not
+   // normalized" bits of the corners that are used
    bool keep = false;
-   auto corner = [&](const uint4 &r) {
-      if (Fd::label_of(r) != lab) return make_float3(0.0f, 0.0f, 0.0f);
-      if (late) keep |= Fd::nn_of(r);
-      return Fd::axis_of(r, !late);
+   auto corner = [&](const uint4 &r, bool used) {
+      const bool own = Fd::label_of(r) == lab;
+      if (late) keep |= used & own & Fd::nn_of(r);
+      const float3 a = Fd::axis_of(r, !late);
+      return own ? a : make_float3(0.0f, 0.0f, 0.0f);
    };
    auto ip = [&](const float3 &a, const float3 &b, float t) {
       return slerp ? slerp_orient(a, b, t) : lerp_orient(a, b, t);
    };
-   // the corners of the plane y0 first, those of y1 only when some lane of the
-   // wave has spread along y (a view tilted along x samples voxel centres in y);
-   // an axis without spread in any lane repeats its corners: they are not read
-   // again (wave-uniform branches)
-   const bool xdup = __ballot(dx) == 0, ydup = __ballot(dy) == 0;
-   float3 c00, c01, c10, c11;
-   {
-      const uint4 r000 = F.raw(x0, y0, z0), r001 = F.raw(x0, y0, z1);
-      uint4 r100 = r000, r101 = r001;
-      if (!xdup) {
-         r100 = F.raw(x1, y0, z0);
-         r101 = F.raw(x1, y0, z1);
-      }
-      c00 = corner(r000);
-      if (dx) c00 = ip(c00, corner(r100), xd);
-      c01 = c00;
+   // record indices ([x][z][y]): i(x, y, z) = (x nz + z') ny + y, z' the
+   // flipped z under flip_z
+   const Idx ny = (Idx)F.ny, nzny = (Idx)F.nz * (Idx)F.ny;
+   const int32_t z0f = F.flip_z ? F.nz - 1 - z0 : z0, z1f = F.flip_z ? F.nz - 1 - z1 : z1;
+   const Idx i00 = (Idx)x0 * nzny + (Idx)z0f * ny + (Idx)y0;
+   const Idx i01 = (Idx)x0 * nzny + (Idx)z1f * ny + (Idx)y0;
+   const bool xs = __ballot(dx) != 0, ys = __ballot(dy) != 0;
+   float3 v;
+   if (!(xs && ys)) {
+      // the spreading axis a (x, else y; with neither, a = y repeats y0)
+      const Idx da_idx = xs ? (Idx)(x1 - x0) * nzny : (Idx)(y1 - y0);
+      const bool da = xs ? dx : dy;
+      const float ad = xs ? xd : yd;
+      const uint4 r00 = F.load(i00), r01 = F.load(i01), r10 = F.load(i00 + da_idx),
+                  r11 = F.load(i01 + da_idx);
+      // an empty asm statement reading the four records keeps the loads
+      // ahead of the branches below (the compiler would sink a load into
+      // the branch that uses it)
+      __asm__ volatile("" ::"v"(r00.x), "v"(r01.x), "v"(r10.x), "v"(r11.x));
+      const float3 c00 = corner(r00, true), c01 = corner(r01, dz), c10 = corner(r10, da),
+                   c11 = corner(r11, da && dz);
+      float3 c0 = c00;
+      if (da) c0 = ip(c00, c10, ad);
+      v = c0;
       if (dz) {
-         c01 = corner(r001);
-         if (dx) c01 = ip(c01, corner(r101), xd);
+         float3 c1 = c01;
+         if (da) c1 = ip(c01, c11, ad);
+         v = ip(c0, c1, zd);
       }
-   }
-   c10 = c00;
-   c11 = c00;
-   if (!ydup) {
-      const uint4 r010 = F.raw(x0, y1, z0), r011 = F.raw(x0, y1, z1);
-      uint4 r110 = r010, r111 = r011;
-      if (!xdup) {
-         r110 = F.raw(x1, y1, z0);
-         r111 = F.raw(x1, y1, z1);
+   } else {
+      const Idx dxi = (Idx)(x1 - x0) * nzny, dyi = (Idx)(y1 - y0);
+      const uint4 r000 = F.load(i00), r001 = F.load(i01), r100 = F.load(i00 + dxi),
+                  r101 = F.load(i01 + dxi), r010 = F.load(i00 + dyi), r011 = F.load(i01 + dyi),
+                  r110 = F.load(i00 + dxi + dyi), r111 = F.load(i01 + dxi + dyi);
+      __asm__ volatile("" ::"v"(r000.x), "v"(r001.x), "v"(r100.x), "v"(r101.x), "v"(r010.x),
+                       "v"(r011.x), "v"(r110.x), "v"(r111.x));
+      const float3 a000 = corner(r000, true), a001 = corner(r001, dz),
+                   a100 = corner(r100, dx), a101 = corner(r101, dx && dz),
+                   a010 = corner(r010, dy), a011 = corner(r011, dy && dz),
+                   a110 = corner(r110, dx && dy), a111 = corner(r111, dx && dy && dz);
+      // level x
+      float3 c00 = a000;
+      if (dx) c00 = ip(a000, a100, xd);
+      float3 c01 = c00, c10 = c00, c11 = c00;
+      if (dz) {
+         c01 = a001;
+         if (dx) c01 = ip(a001, a101, xd);
       }
       if (dy) {
-         c10 = corner(r010);
-         if (dx) c10 = ip(c10, corner(r110), xd);
+         c10 = a010;
+         if (dx) c10 = ip(a010, a110, xd);
          if (dz) {
-            c11 = corner(r011);
-            if (dx) c11 = ip(c11, corner(r111), xd);
+            c11 = a011;
+            if (dx) c11 = ip(a011, a111, xd);
          }
       }
+      // level y
+      float3 c0 = c00, c1 = c01;
+      if (dy) {
+         c0 = ip(c00, c10, yd);
+         if (dz) c1 = ip(c01, c11, yd);
+      }
+      // level z
+      v = dz ? ip(c0, c1, zd) : c0;
    }
-   // level y
-   float3 c0 = c00, c1 = c01;
-   if (dy) {
-      c0 = ip(c00, c10, yd);
-      if (dz) c1 = ip(c01, c11, yd);
-   }
-   // level z
-   if (unit) *unit = (late && !keep) || (ST == ST_ANGLE16 && slerp);
-   if (dz) return ip(c0, c1, zd);
-   return c0;
+   unit = (late && !keep) || (ST == ST_ANGLE16 && slerp);
+   return v;
+}
+
+// GetVec on f32 fields when the axis `own` of the voxel holding the point is
+// at hand (the adjoint's forward steps): `own` when all corners coincide,
+// else get_vec_queued.
+template <typename Idx>
+__device__ float3 get_vec_interp(const Field<Idx, ST_F32> &F, double px, double py, double pz,
+                                 int32_t lab, const float3 &own, int slerp) {
+   const double fx = px - 0.5, fy = py - 0.5, fz = pz - 0.5;
+   const int32_t x0 = clampi((int32_t)floor(fx), F.nx), x1 = clampi((int32_t)ceil(fx), F.nx);
+   const int32_t y0 = clampi((int32_t)floor(fy), F.ny), y1 = clampi((int32_t)ceil(fy), F.ny);
+   const int32_t z0 = clampi((int32_t)floor(fz), F.nz), z1 = clampi((int32_t)ceil(fz), F.nz);
+   if (x0 == x1 && y0 == y1 && z0 == z1) return own;
+   bool unit;
+   return get_vec_queued<Idx, ST_F32>(F, px, py, pz, lab, slerp, unit);
 }
 
 // CalcStartingLightPositionsTilted (simulator.cpp:659-703) for one pixel,
@@ -799,9 +836,9 @@ __global__ void __launch_bounds__(WG, 6)
                const int32_t elab = (int32_t)b.z;
                const PropDev epd = prop(elab);
                bool unit;
-               const float3 v = get_vec_interp<Idx, ST, false>(
-                   F, pack_double(a.x, a.y), pack_double(a.z, a.w), pack_double(b.x, b.y), elab,
-                   make_float3(0.0f, 0.0f, 0.0f), slerp, &unit);
+               const float3 v = get_vec_queued<Idx, ST>(F, pack_double(a.x, a.y),
+                                                        pack_double(a.z, a.w),
+                                                        pack_double(b.x, b.y), elab, slerp, unit);
                float C, S, sd, omc, mu = 0.0f;
                if (retarder<ST>(rot, epd.K, v, b.w != 0, unit, warn, C, S, sd, omc))
                   mu = epd.mu_t;
@@ -1024,7 +1061,7 @@ __device__ void interp_back(const float3 &v, const float3 &u, float t, int slerp
    ub = make_float3(sg * wbar.x, sg * wbar.y, sg * wbar.z);
 }
 
-// Reverse of get_vec_interp (OWN = true): vb = dL/d(the interpolated axis) is
+// Reverse of get_vec_interp: vb = dL/d(the interpolated axis) is
 // distributed over the corners, each adding to its voxel's axis gradient
 // (API layout [x][y][z][3], z mirrored under flip_z).  A corner of another
 // label entered the interpolation as 0 and gets no gradient.
@@ -1536,8 +1573,8 @@ __global__ void __launch_bounds__(WG)
                const PropDev epd = prop(elab);
                const double ex = pack_double(a.x, a.y), ey = pack_double(a.z, a.w);
                const double ez = pack_double(b.x, b.y);
-               float3 v = get_vec_interp<Idx, ST_F32, false>(
-                   F, ex, ey, ez, elab, make_float3(0.0f, 0.0f, 0.0f), slerp);
+               bool unit;
+               float3 v = get_vec_queued<Idx, ST_F32>(F, ex, ey, ez, elab, slerp, unit);
                StepRes o;
                o.C = 1.0f;
                o.S = 0.0f;
