@@ -292,23 +292,26 @@ __device__ __forceinline__ void sin_omc(float x, float &s, float &omc) {
    }
 }
 
-// SphericalInterpolation (simulator.cpp:473-494) without trigonometric calls
-__device__ __forceinline__ float3 slerp_orient(const float3 &v, float3 u, float t) {
-   if (is_zero(v)) return u;
-   if (is_zero(u)) return v;
-   float d = dot3(v, u);
-   if (d < 0.0f) {
-      u = make_float3(-u.x, -u.y, -u.z);
-      d = -d;
-   }
+// SphericalInterpolation (simulator.cpp:473-494) without trigonometric calls.
+// Every lane computes the blend and the cases the reference returns early
+// (a zero argument, arguments within 1e-6 of each other, t = 0) are selected
+// at the end: the wave runs the blend for its other lanes anyway, and early
+// exits cost mask bookkeeping.  A zero argument gives c = fminf(NaN, 1) = 1,
+// z = 0, so such lanes never raise the wave's series tier.
+__device__ __forceinline__ float3 slerp_orient(const float3 &v, const float3 &u0, float t) {
+   const bool zv = is_zero(v), zu = is_zero(u0);
+   const float d0 = dot3(v, u0);
+   const bool flip = d0 < 0.0f;
+   const float3 u = flip ? make_float3(-u0.x, -u0.y, -u0.z) : u0;
+   const float d = flip ? -d0 : d0;
    const float3 df = make_float3(v.x - u.x, v.y - u.y, v.z - u.z);
-   if (dot3(df, df) < 1e-12f) return v;
-   if (t == 0.0f) return v;
+   const bool same = dot3(df, df) < 1e-12f || t == 0.0f;
    const float c = fminf(d * rsqrtf(dot3(v, v) * dot3(u, u)), 1.0f);
    const float z = 0.5f * (1.0f - c);
    float wa, wb;
    slerp_weights(t, z, wa, wb);
-   return make_float3(v.x * wa + u.x * wb, v.y * wa + u.y * wb, v.z * wa + u.z * wb);
+   const float3 r = make_float3(v.x * wa + u.x * wb, v.y * wa + u.y * wb, v.z * wa + u.z * wb);
+   return zv ? u0 : ((zu || same) ? v : r);
 }
 
 __device__ __forceinline__ int32_t clampi(int32_t v, int32_t n) {
@@ -346,19 +349,21 @@ __device__ float3 get_vec_queued(const Field<Idx, ST> &F, double px, double py, 
    const int32_t y0 = clampi((int32_t)fly, F.ny), y1 = clampi((int32_t)fly + (fy != fly), F.ny);
    const int32_t z0 = clampi((int32_t)flz, F.nz), z1 = clampi((int32_t)flz + (fz != flz), F.nz);
    const bool dx = x0 != x1, dy = y0 != y1, dz = z0 != z1;
-   const float xd = dx ? (float)((px - (double)x0) - 0.5) : 0.0f;
-   const float yd = dy ? (float)((py - (double)y0) - 0.5) : 0.0f;
-   const float zd = dz ? (float)((pz - (double)z0) - 0.5) : 0.0f;
+   // indices that differ are unclamped (x0 = floor(x - 0.5)), and then
+   // (x - x0) - 0.5 = (x - 0.5) - floor(x - 0.5), both exact in FP64
+   const float xd = dx ? (float)(fx - flx) : 0.0f;
+   const float yd = dy ? (float)(fy - fly) : 0.0f;
+   const float zd = dz ? (float)(fz - flz) : 0.0f;
    using Fd = Field<Idx, ST>;
    const bool late = ST == ST_F16 && slerp;
    // corners of another label are zeroed; `keep` collects the "not
-   // normalized" bits of the corners that are used
+   // normalized" bits of the corners used (a corner is decoded where it is
+   // used: its record has arrived by then)
    bool keep = false;
-   auto corner = [&](const uint4 &r, bool used) {
-      const bool own = Fd::label_of(r) == lab;
-      if (late) keep |= used & own & Fd::nn_of(r);
-      const float3 a = Fd::axis_of(r, !late);
-      return own ? a : make_float3(0.0f, 0.0f, 0.0f);
+   auto corner = [&](const uint4 &r) {
+      if (Fd::label_of(r) != lab) return make_float3(0.0f, 0.0f, 0.0f);
+      if (late) keep |= Fd::nn_of(r);
+      return Fd::axis_of(r, !late);
    };
    auto ip = [&](const float3 &a, const float3 &b, float t) {
       return slerp ? slerp_orient(a, b, t) : lerp_orient(a, b, t);
@@ -382,14 +387,12 @@ __device__ float3 get_vec_queued(const Field<Idx, ST> &F, double px, double py, 
       // ahead of the branches below (the compiler would sink a load into
       // the branch that uses it)
       __asm__ volatile("" ::"v"(r00.x), "v"(r01.x), "v"(r10.x), "v"(r11.x));
-      const float3 c00 = corner(r00, true), c01 = corner(r01, dz), c10 = corner(r10, da),
-                   c11 = corner(r11, da && dz);
-      float3 c0 = c00;
-      if (da) c0 = ip(c00, c10, ad);
+      float3 c0 = corner(r00);
+      if (da) c0 = ip(c0, corner(r10), ad);
       v = c0;
       if (dz) {
-         float3 c1 = c01;
-         if (da) c1 = ip(c01, c11, ad);
+         float3 c1 = corner(r01);
+         if (da) c1 = ip(c1, corner(r11), ad);
          v = ip(c0, c1, zd);
       }
    } else {
@@ -399,24 +402,20 @@ __device__ float3 get_vec_queued(const Field<Idx, ST> &F, double px, double py, 
                   r110 = F.load(i00 + dxi + dyi), r111 = F.load(i01 + dxi + dyi);
       __asm__ volatile("" ::"v"(r000.x), "v"(r001.x), "v"(r100.x), "v"(r101.x), "v"(r010.x),
                        "v"(r011.x), "v"(r110.x), "v"(r111.x));
-      const float3 a000 = corner(r000, true), a001 = corner(r001, dz),
-                   a100 = corner(r100, dx), a101 = corner(r101, dx && dz),
-                   a010 = corner(r010, dy), a011 = corner(r011, dy && dz),
-                   a110 = corner(r110, dx && dy), a111 = corner(r111, dx && dy && dz);
       // level x
-      float3 c00 = a000;
-      if (dx) c00 = ip(a000, a100, xd);
+      float3 c00 = corner(r000);
+      if (dx) c00 = ip(c00, corner(r100), xd);
       float3 c01 = c00, c10 = c00, c11 = c00;
       if (dz) {
-         c01 = a001;
-         if (dx) c01 = ip(a001, a101, xd);
+         c01 = corner(r001);
+         if (dx) c01 = ip(c01, corner(r101), xd);
       }
       if (dy) {
-         c10 = a010;
-         if (dx) c10 = ip(a010, a110, xd);
+         c10 = corner(r010);
+         if (dx) c10 = ip(c10, corner(r110), xd);
          if (dz) {
-            c11 = a011;
-            if (dx) c11 = ip(a011, a111, xd);
+            c11 = corner(r011);
+            if (dx) c11 = ip(c11, corner(r111), xd);
          }
       }
       // level y
