@@ -27,6 +27,8 @@ WARN_SINGLE_POINT = 1 << 1
 WARN_ZERO_VEC = 1 << 2
 WARN_NOT_NORM = 1 << 3
 WARN_NO_LIGHT = 1 << 4
+# not a warning: the march wrote an image value < 0 (raised as ValueError)
+FLAG_NEGATIVE = 1 << 8
 
 # messages of the reference (generator.cpp:215-216,357; simulator.cpp:294-301,715)
 WARN_MESSAGES = {
@@ -201,6 +203,13 @@ def raise_warnings(mask, stacklevel=3):
     for bit, msg in WARN_MESSAGES.items():
         if mask & bit:
             warnings.warn(msg, UserWarning, stacklevel=stacklevel)
+
+
+def check_images(mask):
+    """The reference's check of the images after the native call
+    (simpli.py:184-185), from the march's flag instead of a pass over them."""
+    if mask & FLAG_NEGATIVE:
+        raise ValueError('intensity < 0 detected')
 
 
 def check(rc, msg):

@@ -874,7 +874,13 @@ __global__ void __launch_bounds__(WG, 6)
       const double u1 = (double)h1 + (double)l1, u2 = (double)h2 + (double)l2;
       const double base = P.quarter_I0 * A;
       double *out = images + ((int64_t)T.out * nxy + ray) * P.n_rho;
-      for (int k = 0; k < P.n_rho; k++) out[k] = base * (1.0 - (P.c2r[k] * u1 - P.s2r[k] * u2));
+      bool neg = false; // the caller's "intensity < 0" check (simpli.py:184-185)
+      for (int k = 0; k < P.n_rho; k++) {
+         const double I = base * (1.0 - (P.c2r[k] * u1 - P.s2r[k] * u2));
+         out[k] = I;
+         neg |= I < 0.0;
+      }
+      if (neg) warn |= FPL_FLAG_NEGATIVE;
    }
    const uint32_t f = warn | (err ? 0x80000000u : 0u);
    if (f) atomicOr(&flags[0], f);

@@ -49,8 +49,7 @@ def _function():
             warn = _lib.simulate(setup, dim, tissue, optical_axis.detach(), props_np, tilts,
                                  images)
             _lib.raise_warnings(warn)
-            if bool(torch.any(images < 0)):
-                raise ValueError('intensity < 0 detected')
+            _lib.check_images(warn)
             ctx.save_for_backward(props, optical_axis, tissue)
             # the backward differentiates this forward: its setup (and the
             # rotations it points to), not the simpli's state at backward time

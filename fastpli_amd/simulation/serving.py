@@ -199,6 +199,7 @@ class SimulationServer:
         self._live.pop(ticket.id, None)
         _lib.check(rc, msg)
         _lib.raise_warnings(warn.value)
+        _lib.check_images(warn.value)
         if return_latency:
             return ticket.images, lat.value
         return ticket.images

@@ -270,8 +270,7 @@ class Simpli(_SimpliBase):
                              device=tissue_.device)
         warn = _lib.simulate(setup, dim, tissue_, axis_, props, tilts, images)
         _lib.raise_warnings(warn)
-        if bool(torch.any(images < 0)):
-            raise ValueError('intensity < 0 detected')
+        _lib.check_images(warn)
         if as_tensor:
             return images
         return images.cpu().numpy()
@@ -302,8 +301,7 @@ class Simpli(_SimpliBase):
                              device=fields.buffer.device)
         warn = _lib.simulate_fields(setup, fields.struct(), tissue_properties, tilts, images)
         _lib.raise_warnings(warn)
-        if bool(torch.any(images < 0)):
-            raise ValueError('intensity < 0 detected')
+        _lib.check_images(warn)
         return images
 
     def run_simulation(self, tissue, optical_axis, tissue_properties, theta, phi):

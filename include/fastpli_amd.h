@@ -19,7 +19,8 @@
  *     receives the reference's error text.  The process is never aborted
  *     (the reference calls exit(), simulator.cpp:62-69).
  *   - `warn` (may be NULL) receives a bit mask of FPL_WARN_* that the caller
- *     raises as Python UserWarnings, once per kind and call.
+ *     raises as Python UserWarnings, once per kind and call (and, from the
+ *     march, FPL_FLAG_NEGATIVE).
  */
 #ifndef FASTPLI_AMD_H_
 #define FASTPLI_AMD_H_
@@ -47,6 +48,10 @@ extern "C" {
 #define FPL_WARN_ZERO_VEC (1u << 2)     /* "optical_axis is 0 for tissue > 0" simulator.cpp:294 */
 #define FPL_WARN_NOT_NORM (1u << 3)     /* "optical_axis not normalized" simulator.cpp:300 */
 #define FPL_WARN_NO_LIGHT (1u << 4)     /* "light_positions is empty" simulator.cpp:715 */
+/* Not a warning: the march wrote an image value < 0.  The reference checks the
+ * images after the native call and raises ValueError("intensity < 0 detected")
+ * (simpli.py:184-185); the Python layer raises it for this bit. */
+#define FPL_FLAG_NEGATIVE (1u << 8)
 
 /* -------------------------------------------------------------- types --- */
 

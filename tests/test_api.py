@@ -284,3 +284,21 @@ def test_install_alias_gives_the_reference_module_paths():
             'print("ok")')
     out = subprocess.run([sys.executable, '-c', code], cwd=ROOT, capture_output=True, text=True)
     assert out.returncode == 0 and out.stdout.strip() == 'ok', out.stderr
+
+
+def test_flag_bits_match_the_header_and_negative_images_raise():
+    """The warn-mask bits of the Python layer are the header's; the march's
+    FPL_FLAG_NEGATIVE is no warning but the reference's ValueError
+    "intensity < 0 detected" (simpli.py:184-185)."""
+    header = open(os.path.join(ROOT, 'include', 'fastpli_amd.h')).read()
+    bits = {name: 1 << int(sh) for name, sh in
+            re.findall(r'#define FPL_((?:WARN|FLAG)_[A-Z_]+) \(1u << (\d+)\)', header)}
+    assert bits == {'WARN_OVERLAP': _lib.WARN_OVERLAP, 'WARN_SINGLE_POINT': _lib.WARN_SINGLE_POINT,
+                    'WARN_ZERO_VEC': _lib.WARN_ZERO_VEC, 'WARN_NOT_NORM': _lib.WARN_NOT_NORM,
+                    'WARN_NO_LIGHT': _lib.WARN_NO_LIGHT, 'FLAG_NEGATIVE': _lib.FLAG_NEGATIVE}
+    with warnings.catch_warnings():
+        warnings.simplefilter('error')
+        _lib.raise_warnings(_lib.FLAG_NEGATIVE)  # no UserWarning for it
+        _lib.check_images(_lib.WARN_NOT_NORM | _lib.WARN_ZERO_VEC)
+    with pytest.raises(ValueError, match='intensity < 0 detected'):
+        _lib.check_images(_lib.FLAG_NEGATIVE | _lib.WARN_NOT_NORM)
