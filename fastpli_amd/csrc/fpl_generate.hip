@@ -1976,9 +1976,10 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
 Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outputs &out,
            double *props, uint32_t *warn, int32_t *label_max, hipStream_t st) {
    // the binning by atomics first (unless the scene's hint says its lists are
-   // too long); a call whose pairs exceeded the assumed capacity runs again
-   // with its exact count, one with a list too long for the per-tile sort
-   // with the radix sort (both leave the hint for the next call)
+   // too long); a call with a list longer than the fixed-size lists runs
+   // again with the radix sort, a radix call whose pairs exceeded the count it
+   // assumed runs again with its exact count (both leave the hint for the
+   // next call)
    bool overflow = false, long_lists = false;
    FPL_TRY(run_once(vol, fbs, out, props, warn, label_max, st, true, false, overflow, long_lists));
    if (overflow)
