@@ -25,5 +25,11 @@ struct Outputs {
 Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outputs &out,
            double *props, uint32_t *warn, int32_t *label_max, hipStream_t st);
 
+// An upper estimate of the stream workspace `run` takes for a model of
+// n_points points, n_fibers fibers and n_bundles bundles in a volume of dim
+// voxels (both binnings, so a scene that falls back to the radix sort fits).
+size_t workspace_bytes(int64_t n_points, int64_t n_fibers, int64_t n_bundles,
+                       const int64_t dim[3]);
+
 } // namespace gen
 } // namespace fpl

@@ -1973,6 +1973,24 @@ Status run_once(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outpu
    return Status();
 }
 
+size_t workspace_bytes(int64_t n_points, int64_t n_fibers, int64_t n_bundles,
+                       const int64_t dim[3]) {
+   const size_t np = (size_t)(n_points > 0 ? n_points : 0) + 2;
+   const size_t n_tiles = (size_t)((dim[0] + TX - 1) / TX) * (size_t)((dim[1] + TY - 1) / TY) *
+                          (size_t)((dim[2] + TZ - 1) / TZ);
+   // segment records, directions, slab data, counts and offsets
+   size_t b = (sizeof(SegRec) + 3 * sizeof(float4) + 2 * sizeof(int64_t)) * np;
+   // fixed-size tile lists and cursors; per-tile ranges and end bundles
+   b += (SORT_MAX + 1) * sizeof(uint32_t) * n_tiles + 4 * sizeof(uint32_t) * n_tiles;
+   // the radix sort's four pair arrays and its temporary storage (about 8
+   // pairs per segment)
+   b += 8 * 24 * np;
+   // fiber and bundle boxes, culling flags, small tables, scan temporaries
+   b += 56 * (size_t)((n_fibers > 0 ? n_fibers : 0) + (n_bundles > 0 ? n_bundles : 0) + 1) +
+        ((size_t)1 << 20);
+   return b;
+}
+
 Status run(const fpl_volume *vol, const fpl_fiber_bundles *fbs, const Outputs &out,
            double *props, uint32_t *warn, int32_t *label_max, hipStream_t st) {
    // the binning by atomics first (unless the scene's hint says its lists are

@@ -627,12 +627,7 @@ Status submit_model(fpl_server *S, const fpl_model_request *q, int64_t *ticket) 
    r->need[B_FPTR] = align256(sizeof(int64_t) * (size_t)(fb.n_fibers + 1));
    r->need[B_FIELDS] =
        align256(align256(n_vox * rec::record_bytes(storage)) + n_vox * rec::second_bytes(storage));
-   // generation workspace: segment records and tables (~240 B per point),
-   // (tile, segment) pairs in four arrays with the sort's temporary storage
-   // (~8 pairs per segment), per-tile ranges, fiber and bundle boxes
-   const size_t n_tiles = n_vox / 2048 + 1;
-   r->scratch = 240 * (size_t)(fb.n_points + 1) + 8 * 24 * (size_t)(fb.n_points + 1) +
-                16 * n_tiles + 48 * (size_t)(fb.n_fibers + nb + 1) + ((size_t)1 << 20);
+   r->scratch = gen::workspace_bytes(fb.n_points, fb.n_fibers, nb, v.dim);
    DeviceGuard dg(S->device);
    FPL_TRY(record_after(*r, q->stream));
    r->t0 = Clock::now();
