@@ -385,8 +385,10 @@ __device__ float3 get_vec_queued(const Field<Idx, ST> &F, double px, double py, 
                   r11 = F.load(i01 + da_idx);
       // an empty asm statement reading the four records keeps the loads
       // ahead of the branches below (the compiler would sink a load into
-      // the branch that uses it)
-      __asm__ volatile("" ::"v"(r00.x), "v"(r01.x), "v"(r10.x), "v"(r11.x));
+      // the branch that uses it); words x and y: ST_ANGLE16 reads them with
+      // two loads
+      __asm__ volatile("" ::"v"(r00.x), "v"(r01.x), "v"(r10.x), "v"(r11.x), "v"(r00.y),
+                       "v"(r01.y), "v"(r10.y), "v"(r11.y));
       float3 c0 = corner(r00);
       if (da) c0 = ip(c0, corner(r10), ad);
       v = c0;
@@ -402,6 +404,8 @@ __device__ float3 get_vec_queued(const Field<Idx, ST> &F, double px, double py, 
                   r110 = F.load(i00 + dxi + dyi), r111 = F.load(i01 + dxi + dyi);
       __asm__ volatile("" ::"v"(r000.x), "v"(r001.x), "v"(r100.x), "v"(r101.x), "v"(r010.x),
                        "v"(r011.x), "v"(r110.x), "v"(r111.x));
+      __asm__ volatile("" ::"v"(r000.y), "v"(r001.y), "v"(r100.y), "v"(r101.y), "v"(r010.y),
+                       "v"(r011.y), "v"(r110.y), "v"(r111.y));
       // level x
       float3 c00 = corner(r000);
       if (dx) c00 = ip(c00, corner(r100), xd);
